@@ -1,0 +1,147 @@
+"""ParHDE layout benchmark on one MI355X.
+
+Workload (one "step" = one complete layout): random geometric graph of
+4,194,304 points uniform in the unit square (seed 2), radius for average
+degree 10, largest component, unweighted int32 CSR; ParHDE with s = 50
+k-centers pivots (first pivot 0), c = 2 coordinates, FP64 orthogonalisation
+and SpMM, the reference's power-method eigensolver, traversals run lazily
+(only as many as the orthogonalisation consumes; outputs identical to the
+reference's eager 3s traversals).  The graph is resident in HBM before the
+timed region; each layout returns its coordinates to host memory.
+
+Prints one JSON line (see the `record` dict below for the fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bfs_bytes(n: int, m: int) -> int:
+    # adjacency once, row pointers once, int32 level written once per vertex
+    return 4 * m + 4 * (n + 1) + 4 * n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=4194304)
+    ap.add_argument("--s", type=int, default=50)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--eager", action="store_true")
+    ap.add_argument("--extras", type=int, default=1, help="also time eager and fp32 once")
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(every configuration of this workload fits one MI355X)", file=sys.stderr)
+        sys.exit(2)
+
+    from icpp_2020_parhde_amd import layout as L
+    from icpp_2020_parhde_amd.graphs import Graph
+
+    t0 = time.time()
+    g = Graph.rgg(args.n, 10.0, seed=2)
+    log(f"[bench] rgg n={g.n} nnz={g.m} generated in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ctx = L.Context(g)
+    h2d_s = time.time() - t0
+    log(f"[bench] graph resident in HBM ({h2d_s * 1e3:.1f} ms incl. validation and H2D)")
+    prec = L.PREC_FP32 if args.precision == "fp32" else L.PREC_FP64
+    opts = dict(s=args.s, num_coord=2, precision=prec, eager=1 if args.eager else 0)
+
+    for i in range(args.warmup):
+        X, st = ctx.layout(**opts)
+        log(f"[bench] warmup {i}: {st.ms_total:.1f} ms, {st.n_traversals} traversals")
+    times, stats = [], []
+    t_all = time.perf_counter()
+    for i in range(args.steps):
+        t = time.perf_counter()
+        X, st = ctx.layout(**opts)
+        times.append((time.perf_counter() - t) * 1e3)
+        stats.append(st)
+        log(f"[bench] step {i}: {times[-1]:.1f} ms (bfs {st.kernel_ms['bfs']:.1f} ms over "
+            f"{st.n_traversals} traversals, orth {st.ms_orth:.1f}, mmult {st.ms_mmult:.1f}, "
+            f"eig {st.ms_eig:.1f}, coords {st.ms_coords:.1f})")
+    wall_ms = (time.perf_counter() - t_all) * 1e3
+    ms_per_layout = wall_ms / args.steps
+    st = stats[-1]
+    nbfs = st.n_traversals
+    bfs_ms = np.mean([s.kernel_ms["bfs"] for s in stats]) / nbfs
+    gteps = (g.m / 2) / (bfs_ms * 1e-3) / 1e9
+    achieved = bfs_bytes(g.n, g.m) / (bfs_ms * 1e-3) / 1e9
+
+    extras = {}
+    if args.extras:
+        _, se = ctx.layout(s=args.s, num_coord=2, precision=prec, eager=1)
+        extras["eager_ms_per_layout"] = round(se.ms_total, 2)
+        extras["eager_traversals"] = se.n_traversals
+        if prec == L.PREC_FP64:
+            _, sf = ctx.layout(s=args.s, num_coord=2, precision=L.PREC_FP32)
+            extras["fp32_ms_per_layout"] = round(sf.ms_total, 2)
+        log(f"[bench] extras {extras}")
+
+    record = {
+        "metric": f"ParHDE ms/layout (RGG n={g.n}, s={args.s}, k-centers)",
+        "value": round(ms_per_layout, 3),
+        "unit": "ms/layout",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_layout, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i32 levels / " + args.precision,
+        "data": "synthetic (seeded random geometric graph)",
+        "config": {"workload": "rgg4m_s50", "n": g.n, "nnz": g.m, "s": args.s, "c": 2,
+                   "pivots": "kcenters", "eig": "power", "traversals": "eager" if args.eager else "lazy",
+                   "precision": args.precision},
+        "bfs_gteps": round(gteps, 4),
+        "bfs_ms_mean": round(bfs_ms, 4),
+        "bfs_levels_mean": round(st.levels_total / nbfs, 1),
+        "phases_ms": {"distance": round(st.ms_distance, 2), "orth": round(st.ms_orth, 2),
+                      "mmult": round(st.ms_mmult, 2), "eig": round(st.ms_eig, 2),
+                      "coords": round(st.ms_coords, 2)},
+        "kernel_ms": {k: round(v, 3) for k, v in st.kernel_ms.items() if v},
+        "roofline": {"bound": "hbm", "kernel": "bfs_persistent_kernel", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None},
+        "h2d_ms": round(h2d_s * 1e3, 1),
+        **extras,
+        "cpu_baseline": None,
+    }
+    if args.cpu_baseline:
+        record["cpu_baseline"] = cpu_baseline(g, args.s)
+    print(json.dumps(record), flush=True)
+
+
+def cpu_baseline(g, s):
+    """The oracle (the reference's algorithm restated, OpenMP) on this host's
+    cores, on a bounded sample; see oracle.pyoracle.parhde_sample."""
+    from oracle import pyoracle as O
+    t0 = time.time()
+    r = O.parhde_sample(g, s)
+    log(f"[bench] cpu baseline sample took {time.time() - t0:.1f}s: {r}")
+    return r
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,772 @@
+// api.hip — the C ABI of libparhde.so (include/parhde.h) and the layout
+// drivers that sequence the device stages.
+//
+// The drivers follow the control flow of the reference's run() methods:
+// ParHDE::run (variants/ParHDE.cpp:45-352), WParHDE::run
+// (variants/WParHDE.cpp:40-291), PivotMDS::run (variants/PivotMDS.cpp:68-147)
+// and PHDE::run (variants/PHDE.cpp:50-136), followed by getCoordinates
+// (variants/DrawingBase.cpp:264-273).  All n-sized work runs on the GPU; the
+// host keeps the pivot bookkeeping (the `sources` set of the dedupe rule),
+// the accept/discard decision per candidate and the start values of the
+// power method.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "eig.h"
+#include "orth.h"
+#include "phde_common.h"
+#include "spmm.h"
+#include "traverse.h"
+
+namespace phde {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+hipEvent_t Timer::get() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+void Timer::begin(hipStream_t s, int cls, hipEvent_t *a) {
+  *a = nullptr;
+  launches[cls]++;
+  if (!enabled) return;
+  *a = get();
+  (void)hipEventRecord(*a, s);
+}
+void Timer::end(hipStream_t s, int cls, hipEvent_t a) {
+  if (!enabled || !a) return;
+  hipEvent_t b = get();
+  (void)hipEventRecord(b, s);
+  open.push_back({cls, a, b});
+}
+int Timer::resolve() {
+  for (auto &sp : open) {
+    float ms = 0;
+    if (hipEventSynchronize(sp.b) == hipSuccess && hipEventElapsedTime(&ms, sp.a, sp.b) == hipSuccess)
+      this->ms[sp.cls] += ms;
+    pool.push_back(sp.a);
+    pool.push_back(sp.b);
+  }
+  open.clear();
+  return PHDE_OK;
+}
+void Timer::reset() {
+  resolve();
+  for (double &m : ms) m = 0;
+  for (long long &l : launches) l = 0;
+}
+Timer::~Timer() {
+  for (auto &sp : open) {
+    (void)hipEventDestroy(sp.a);
+    (void)hipEventDestroy(sp.b);
+  }
+  for (auto e : pool) (void)hipEventDestroy(e);
+}
+
+__global__ void degree_kernel(const uint32_t *rowptr, const int32_t *w, int64_t n, int32_t *deg32,
+                              double *degw) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t b = rowptr[i], e = rowptr[i + 1];
+    deg32[i] = (int32_t)(e - b);
+    if (w) {
+      // WParHDE.cpp:53-58: weights summed in adjacency order
+      double d = 0;
+      for (uint32_t k = b; k < e; k++) d += (double)w[k];
+      degw[i] = d;
+    } else {
+      degw[i] = (double)(e - b);  // ParHDE.cpp:58-59
+    }
+  }
+}
+
+__global__ void gather_cols_kernel(const double *K, int dim, int c, bool largest, double *out) {
+  // eigenvector selection of DrawingBase.cpp:138-140 (Largest) and
+  // :168-170 (First)
+  for (int e = threadIdx.x; e < dim * c; e += blockDim.x) {
+    const int i = e % dim, j = e / dim;
+    out[e] = largest ? K[i + (dim - j - 1) * dim] : K[i + j * dim];
+  }
+}
+
+static int check_graph_host(const uint32_t *rowptr, const uint32_t *adj, int64_t n, int64_t m) {
+  if (n < 1 || m < 0 || n > (int64_t)UINT32_MAX || m > (int64_t)UINT32_MAX) return PHDE_ERR_ARG;
+  if (rowptr[0] != 0 || (int64_t)rowptr[n] != m) return PHDE_ERR_ARG;
+  for (int64_t i = 0; i < n; i++)
+    if (rowptr[i + 1] < rowptr[i]) return PHDE_ERR_ARG;
+  uint32_t mx = 0;
+  for (int64_t e = 0; e < m; e++) mx = adj[e] > mx ? adj[e] : mx;
+  if (m > 0 && (int64_t)mx >= n) return PHDE_ERR_ARG;
+  return PHDE_OK;
+}
+
+static int ctx_init(phde_ctx *c, int64_t n, int64_t m, bool weighted) {
+  PHDE_HIP(hipGetDevice(&c->device));
+  PHDE_HIP(hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  PHDE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  PHDE_HIP(hipMalloc(&c->d_rowptr, sizeof(uint32_t) * (n + 1)));
+  PHDE_HIP(hipMalloc(&c->d_adj, sizeof(uint32_t) * std::max<int64_t>(m, 1)));
+  if (weighted) PHDE_HIP(hipMalloc(&c->d_w, sizeof(int32_t) * std::max<int64_t>(m, 1)));
+  PHDE_HIP(hipMalloc(&c->d_deg32, sizeof(int32_t) * n));
+  PHDE_HIP(hipMalloc(&c->d_degw, sizeof(double) * n));
+  PHDE_HIP(hipHostMalloc(&c->h_small, sizeof(double) * 64, hipHostMallocDefault));
+  c->g.n = n;
+  c->g.m = m;
+  c->g.rowptr = c->d_rowptr;
+  c->g.adj = c->d_adj;
+  c->g.w = c->d_w;
+  c->g.deg32 = c->d_deg32;
+  c->g.degw = c->d_degw;
+  return PHDE_OK;
+}
+
+static int ctx_finish(phde_ctx *c) {
+  const int64_t n = c->g.n;
+  const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  degree_kernel<<<blocks, 256, 0, c->stream>>>(c->d_rowptr, c->d_w, n, c->d_deg32, c->d_degw);
+  PHDE_HIP(hipGetLastError());
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  return PHDE_OK;
+}
+
+static void ctx_free(phde_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *dev[] = {c->d_rowptr, c->d_adj, c->d_w, c->d_deg32, c->d_degw, c->d_dist, c->d_q[0],
+                 c->d_q[1], c->d_hq[0], c->d_hq[1], c->d_bm[0], c->d_bm[1], c->d_slots, c->d_kc,
+                 c->d_part};
+  for (void *p : dev)
+    if (p) (void)hipFree(p);
+  if (c->h_kc) (void)hipHostFree(c->h_kc);
+  if (c->h_small) (void)hipHostFree(c->h_small);
+  traverse_release(c);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// RAII device buffer for driver temporaries.
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  int alloc(size_t count) {
+    PHDE_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)));
+    return PHDE_OK;
+  }
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+#define PHDE_TRY(x)     \
+  do {                  \
+    int rc_ = (x);      \
+    if (rc_) return rc_; \
+  } while (0)
+
+using clk = std::chrono::high_resolution_clock;
+static double ms_since(clk::time_point a) {
+  return std::chrono::duration<double, std::milli>(clk::now() - a).count();
+}
+
+// k-centers bookkeeping shared by every algorithm: the first maximiser of
+// the running min distance becomes the next source, with the reference's
+// dedupe rule (ParHDE.cpp:142-155; DrawingBase.cpp:225-237).
+struct KCenters {
+  std::set<int64_t> sources;
+  int64_t next = 0;
+  // record_old: ParHDE / WParHDE insert the argmax (ParHDE.cpp:143,155);
+  // distancePhaseComputation inserts the current source (DrawingBase.cpp:206,237).
+  int step(phde_ctx *c, const int32_t *min_dist_dev, const KcResult &kc, bool check_dup,
+           bool record_argmax, int64_t current) {
+    const int64_t maxval = (int64_t)(kc.key >> 32);
+    int64_t idx = (int64_t)(0xffffffffull - (kc.key & 0xffffffffull));
+    const int64_t old_idx = record_argmax ? idx : current;
+    if (check_dup && sources.count(idx)) {
+      std::vector<int32_t> md(c->g.n);
+      PHDE_HIP(hipMemcpyAsync(md.data(), min_dist_dev, sizeof(int32_t) * c->g.n,
+                              hipMemcpyDeviceToHost, c->stream));
+      PHDE_HIP(hipStreamSynchronize(c->stream));
+      for (int64_t i = 0; i < c->g.n; i++)
+        if (md[i] == maxval && !sources.count(i)) idx = i;
+    }
+    sources.insert(old_idx);
+    next = idx;
+    return PHDE_OK;
+  }
+};
+
+struct LayoutOut {
+  phde_stats *st;
+  double t_dist = 0, t_orth = 0, t_mmult = 0, t_eig = 0, t_coords = 0;
+  int ntrav = 0, nacc = 0, ndisc = 0, iters = 0;
+  long long levels = 0;
+  void pivot(int64_t p) {
+    if (st && ntrav < 192) st->pivots[ntrav] = p;
+    ntrav++;
+  }
+};
+
+// Eigen step + coordinates shared by all algorithms.  K_dev: s*s matrix in
+// the reference's storage (consumed).  For the generalised mode B_dev holds
+// S^T D S.
+static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, bool fp32, int s,
+                         double *K_dev, const double *B_dev, bool largest, double *coords,
+                         LayoutOut &out) {
+  const int nc = o->num_coord;
+  auto t0 = clk::now();
+  DBuf<double> init, evals, work, V, evecs;
+  DBuf<int> status;
+  PHDE_TRY(evals.alloc(s));
+  PHDE_TRY(work.alloc(3 * (size_t)s * s));
+  PHDE_TRY(V.alloc((size_t)s * nc));
+  PHDE_TRY(status.alloc(4));
+  int h_status[4] = {0, 0, 0, 0};
+  if (o->eig_mode == PHDE_EIG_POWER) {
+    std::vector<int32_t> r(s * s);
+    glibc_rand_sequence(1, s * s, r.data());
+    std::vector<double> iv(s * s);
+    for (int i = 0; i < s * s; i++) iv[i] = ((double)r[i]) / 2147483647;
+    PHDE_TRY(init.alloc((size_t)s * s));
+    PHDE_HIP(hipMemcpyAsync(init.p, iv.data(), sizeof(double) * s * s, hipMemcpyHostToDevice,
+                            c->stream));
+    PHDE_TRY(power_eig_dev(c, K_dev, init.p, evals.p, s, 100000, work.p, status.p));
+    gather_cols_kernel<<<1, 256, 0, c->stream>>>(K_dev, s, nc, largest, V.p);
+  } else {
+    PHDE_TRY(evecs.alloc((size_t)s * s));
+    PHDE_TRY(sym_eig_dev(c, K_dev, o->eig_mode == PHDE_EIG_GENERALIZED ? B_dev : nullptr, s,
+                         evals.p, evecs.p, work.p, status.p));
+    // ascending order: First takes the leading columns, Largest the trailing
+    // ones in descending order
+    gather_cols_kernel<<<1, 256, 0, c->stream>>>(evecs.p, s, nc, largest, V.p);
+  }
+  PHDE_HIP(hipGetLastError());
+  PHDE_HIP(hipMemcpyAsync(h_status, status.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  out.t_eig += ms_since(t0);
+  if (h_status[0] != 0) {
+    if (o->verbose) printf("The algorithm failed to compute eigenvalues.\n");
+    set_error("eigensolver failed (status " + std::to_string(h_status[0]) + ")");
+    return PHDE_ERR_EIG;
+  }
+  out.iters = o->eig_mode == PHDE_EIG_POWER ? h_status[1] : 0;
+  auto t1 = clk::now();
+  DBuf<double> X, part, inv;
+  PHDE_TRY(X.alloc((size_t)c->g.n * nc));
+  PHDE_TRY(part.alloc((size_t)nc * c->num_cus * 8 + 64));
+  PHDE_TRY(inv.alloc(nc));
+  PHDE_TRY(coords_compute(c, S_dev, fp32, s, nc, V.p, X.p, part.p, inv.p));
+  PHDE_HIP(hipMemcpyAsync(coords, X.p, sizeof(double) * c->g.n * nc, hipMemcpyDeviceToHost,
+                          c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  out.t_coords += ms_since(t1);
+  return PHDE_OK;
+}
+
+// S^T D S (generalised eigenproblem's right-hand side) via the Gram kernel
+// on D^{1/2} S.
+int weighted_gram(phde_ctx *c, const void *S_dev, bool fp32, int s, double *out_dev);
+
+// ParHDE / WParHDE driver.
+static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, LayoutOut &out) {
+  const bool weighted = o->algorithm == PHDE_ALGO_WPARHDE;
+  if (weighted != (c->g.w != nullptr)) {
+    set_error(weighted ? "Weighted ParHDE code requires the weighted edges"
+                       : "ParHDE code doesn't requires the weighted edges");
+    return PHDE_ERR_WEIGHTS;
+  }
+  const int64_t n = c->g.n;
+  const int s = o->s;
+  const bool fp32 = o->precision == PHDE_PREC_FP32;
+  const int limit = 3 * s;
+  auto t_start = clk::now();
+  Orth orth;
+  PHDE_TRY(orth.setup(c, n, s, fp32));
+  // ones vector U[:,0] = 1 / ||1|| (ParHDE.cpp:94-108)
+  const double u0 = 1.0 * (1 / std::sqrt((double)n));
+  PHDE_TRY(orth.init_ones(c->d_degw, u0));
+  DBuf<int32_t> min_dist, cols;
+  PHDE_TRY(min_dist.alloc(n));
+  PHDE_TRY(fill_i32(c, min_dist.p, n, INT_MAX));
+  const bool eager = o->eager && !weighted;
+  PHDE_TRY(cols.alloc(eager ? (size_t)n * limit : (size_t)n));
+  std::vector<unsigned long long> sumsq(limit + 1, 0);
+  KCenters kc;
+  kc.next = o->first_pivot;
+  int accepted = 0;
+
+  auto traverse = [&](int run_count, int32_t *col) -> int {
+    const int64_t src = kc.next;
+    out.pivot(src);
+    if (weighted)
+      PHDE_TRY(sssp_run(c, src, o->delta, col, min_dist.p, true));
+    else
+      PHDE_TRY(bfs_run(c, src, o->alpha, o->beta, col, min_dist.p, true));
+    const KcResult r = *c->h_kc;
+    if (r.unreached) {
+      set_error("graph is not connected: " + std::to_string(r.unreached) +
+                " vertices unreached from " + std::to_string(src));
+      return PHDE_ERR_UNREACHED;
+    }
+    out.levels += r.depth;
+    sumsq[run_count] = r.sumsq;
+    const bool first = weighted ? accepted == 0 : run_count == 1;
+    return kc.step(c, min_dist.p, r, !first, true, src);
+  };
+  // One candidate through CGS (ParHDE) or MGS (WParHDE); returns via
+  // *acc whether it was accepted.
+  auto orthogonalise = [&](int run_count, const int32_t *col) -> int {
+    // column normalisation: 1 / sqrt(sum d^2) (ParHDE.cpp:176-185)
+    const double scale = 1 / std::sqrt((double)sumsq[run_count]);
+    PHDE_TRY(orth.load_candidate(col, scale));
+    const int passes = fp32 ? 2 : 1;
+    for (int p = 0; p < passes; p++) {
+      if (weighted)
+        PHDE_TRY(orth.mgs_pass(accepted));
+      else
+        PHDE_TRY(orth.cgs_pass(accepted));
+    }
+    double nd = 0;
+    PHDE_TRY(orth.read_norm(&nd));
+    if (nd < 0.001) {
+      if (o->verbose) std::printf("discarding vec %d, normdist %g\n", accepted + 1, nd);
+      out.ndisc++;
+    } else {
+      PHDE_TRY(orth.accept(accepted));
+      accepted++;
+    }
+    return PHDE_OK;
+  };
+
+  if (eager) {
+    auto t0 = clk::now();
+    for (int rc = 1; rc <= limit; rc++) PHDE_TRY(traverse(rc, cols.p + (size_t)(rc - 1) * n));
+    out.t_dist += ms_since(t0);
+    auto t1 = clk::now();
+    for (int rc = 1; rc <= limit && accepted < s; rc++)
+      PHDE_TRY(orthogonalise(rc, cols.p + (size_t)(rc - 1) * n));
+    out.t_orth += ms_since(t1);
+  } else {
+    for (int rc = 1; rc <= limit && accepted < s; rc++) {
+      auto t0 = clk::now();
+      PHDE_TRY(traverse(rc, cols.p));
+      auto t1 = clk::now();
+      PHDE_TRY(orthogonalise(rc, cols.p));
+      out.t_dist += std::chrono::duration<double, std::milli>(t1 - t0).count();
+      out.t_orth += ms_since(t1);
+    }
+  }
+  out.nacc = accepted;
+  if (accepted < s) {
+    if (o->verbose)
+      std::fprintf(stderr,
+                   "Number of Pivots obtained: %d inadequate, increase the Pivots Limits: %d\n",
+                   accepted + 1, limit);
+    set_error("only " + std::to_string(accepted) + " of " + std::to_string(s) +
+              " pivots are independent");
+    return PHDE_ERR_PIVOTS;
+  }
+  auto t2 = clk::now();
+  Lap lap;
+  PHDE_TRY(lap.setup(c, n, s, fp32));
+  PHDE_TRY(lap.run(orth.S));
+  DBuf<double> B;
+  if (o->eig_mode == PHDE_EIG_GENERALIZED) {
+    PHDE_TRY(B.alloc((size_t)s * s));
+    PHDE_TRY(weighted_gram(c, orth.S, fp32, s, B.p));
+  }
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  out.t_mmult += ms_since(t2);
+  (void)t_start;
+  return finish_layout(c, o, orth.S, fp32, s, lap.SLS, B.p, false, coords, out);
+}
+
+__global__ void center_kernel(const int32_t *D, int64_t n, int s, const double *colterm,
+                              const double *rowterm, double total, bool dbl, double *C) {
+  // PivotMDS doubleCentering (PivotMDS.cpp:53-63) when dbl, PHDE
+  // columnCentering (PHDE.cpp:30-47) otherwise.
+  const double rN = (double)1 / (double)n, rK = (double)1 / (double)s;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * s;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = e % n;
+    const int i = (int)(e / n);
+    const double d = (double)D[e];
+    if (dbl)
+      C[e] = -0.5 * (d * d - (rN * colterm[i]) - (rK * rowterm[j]) + (rK * rN * total));
+    else
+      C[e] = d - colterm[i];
+  }
+}
+
+__global__ void center_sums_kernel(const int32_t *D, int64_t n, int s, bool squares,
+                                   double *rowterm, double *colpart) {
+  // row sums of D^2 (one thread per vertex, columns in order) and per-block
+  // column partials
+  __shared__ double sh[64][4];
+  double cp[64];
+  for (int i = 0; i < s; i++) cp[i] = 0;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    double rs = 0;
+    for (int i = 0; i < s; i++) {
+      const double d = (double)D[(int64_t)i * n + j];
+      const double t = squares ? d * d : d;
+      rs += t;
+      cp[i] += t;
+    }
+    if (rowterm) rowterm[j] = rs;
+  }
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  for (int i = 0; i < s; i++) {
+    double v = cp[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) sh[i][w] = v;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < s; i += blockDim.x) {
+    double v = 0;
+    for (int q = 0; q < (int)(blockDim.x / 64); q++) v += sh[i][q];
+    colpart[(int64_t)i * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// PivotMDS / PHDE driver: s k-centers traversals (DrawingBase.cpp:193-240),
+// centring, C^T C, eig(Largest), C * V.
+static int layout_mds(phde_ctx *c, const phde_options *o, double *coords, LayoutOut &out) {
+  const int64_t n = c->g.n;
+  const int s = o->s;
+  const bool dbl = o->algorithm == PHDE_ALGO_PIVOTMDS;
+  DBuf<int32_t> min_dist, D;
+  PHDE_TRY(min_dist.alloc(n));
+  PHDE_TRY(fill_i32(c, min_dist.p, n, INT_MAX));
+  PHDE_TRY(D.alloc((size_t)n * s));
+  KCenters kc;
+  kc.next = o->first_pivot;
+  auto t0 = clk::now();
+  for (int rc = 0; rc < s; rc++) {
+    const int64_t src = kc.next;
+    out.pivot(src);
+    int32_t *col = D.p + (size_t)rc * n;
+    if (c->g.w)
+      PHDE_TRY(sssp_run(c, src, o->delta, col, min_dist.p, true));
+    else
+      PHDE_TRY(bfs_run(c, src, o->alpha, o->beta, col, min_dist.p, true));
+    const KcResult r = *c->h_kc;
+    if (r.unreached) {
+      set_error("graph is not connected");
+      return PHDE_ERR_UNREACHED;
+    }
+    out.levels += r.depth;
+    PHDE_TRY(kc.step(c, min_dist.p, r, rc != 0, false, src));
+  }
+  out.t_dist += ms_since(t0);
+  auto t1 = clk::now();
+  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, c->num_cus * 4));
+  DBuf<double> rowterm, colpart, C;
+  PHDE_TRY(rowterm.alloc(n));
+  PHDE_TRY(colpart.alloc((size_t)s * nb));
+  PHDE_TRY(C.alloc((size_t)n * s));
+  center_sums_kernel<<<nb, 256, 0, c->stream>>>(D.p, n, s, dbl, rowterm.p, colpart.p);
+  PHDE_HIP(hipGetLastError());
+  std::vector<double> cp((size_t)s * nb), colterm(s);
+  PHDE_HIP(hipMemcpyAsync(cp.data(), colpart.p, sizeof(double) * s * nb, hipMemcpyDeviceToHost,
+                          c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  double total = 0;
+  for (int i = 0; i < s; i++) {
+    double v = 0;
+    for (int b = 0; b < nb; b++) v += cp[(size_t)i * nb + b];
+    colterm[i] = dbl ? v : v / (double)n;
+    total += v;
+  }
+  DBuf<double> colterm_d;
+  PHDE_TRY(colterm_d.alloc(s));
+  PHDE_HIP(hipMemcpyAsync(colterm_d.p, colterm.data(), sizeof(double) * s, hipMemcpyHostToDevice,
+                          c->stream));
+  center_kernel<<<(int)std::min<int64_t>((n * s + 255) / 256, 16384), 256, 0, c->stream>>>(
+      D.p, n, s, colterm_d.p, rowterm.p, total, dbl, C.p);
+  PHDE_HIP(hipGetLastError());
+  out.t_orth += ms_since(t1);
+  auto t2 = clk::now();
+  Lap lap;
+  PHDE_TRY(lap.setup(c, n, s, false));
+  PHDE_TRY(lap.gram_self(C.p));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  out.t_mmult += ms_since(t2);
+  out.nacc = s;
+  return finish_layout(c, o, C.p, false, s, lap.SLS, nullptr, true, coords, out);
+}
+
+}  // namespace phde
+
+using namespace phde;
+
+extern "C" {
+
+void phde_default_options(phde_options *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->algorithm = PHDE_ALGO_PARHDE;
+  o->s = 10;
+  o->num_coord = 2;
+  o->strategy = PHDE_PIVOTS_KCENTERS;
+  o->precision = PHDE_PREC_FP64;
+  o->eig_mode = PHDE_EIG_POWER;
+  o->eager = 0;
+  o->delta = 0;
+  o->alpha = 15;
+  o->beta = 18;
+  o->verbose = 0;
+  o->first_pivot = 0;
+  o->seed = 1;
+}
+
+const char *phde_version(void) { return "parhde-mi355x 0.1 (gfx950)"; }
+const char *phde_last_error(void) { return g_err.c_str(); }
+
+int phde_device_count(int *count) {
+  if (!count) return PHDE_ERR_ARG;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *count = c;
+  return PHDE_OK;
+}
+
+int phde_create(const uint32_t *rowptr, const uint32_t *adj, const int32_t *weights, int64_t n,
+                int64_t m, phde_ctx **out) {
+  if (!rowptr || !out || (m > 0 && !adj)) return PHDE_ERR_ARG;
+  *out = nullptr;
+  if (check_graph_host(rowptr, adj, n, m)) {
+    set_error("malformed CSR (row pointers or ids out of range)");
+    return PHDE_ERR_ARG;
+  }
+  phde_ctx *c = new phde_ctx();
+  int rc = ctx_init(c, n, m, weights != nullptr);
+  if (rc == PHDE_OK) {
+    if (hipMemcpy(c->d_rowptr, rowptr, sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        (m > 0 && hipMemcpy(c->d_adj, adj, sizeof(uint32_t) * m, hipMemcpyHostToDevice) != hipSuccess) ||
+        (weights && m > 0 &&
+         hipMemcpy(c->d_w, weights, sizeof(int32_t) * m, hipMemcpyHostToDevice) != hipSuccess)) {
+      set_error("graph upload failed");
+      rc = PHDE_ERR_DEVICE;
+    }
+  }
+  if (rc == PHDE_OK) rc = ctx_finish(c);
+  if (rc != PHDE_OK) {
+    ctx_free(c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return PHDE_OK;
+}
+
+int phde_create_device(const uint32_t *rowptr_dev, const uint32_t *adj_dev,
+                       const int32_t *weights_dev, int64_t n, int64_t m, phde_ctx **out) {
+  if (!rowptr_dev || !out || (m > 0 && !adj_dev)) return PHDE_ERR_ARG;
+  // validate through a host copy: the kernels trust these arrays
+  std::vector<uint32_t> rp(n + 1), ad(std::max<int64_t>(m, 1));
+  if (hipMemcpy(rp.data(), rowptr_dev, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost) !=
+          hipSuccess ||
+      (m > 0 && hipMemcpy(ad.data(), adj_dev, sizeof(uint32_t) * m, hipMemcpyDeviceToHost) != hipSuccess))
+    return PHDE_ERR_DEVICE;
+  if (check_graph_host(rp.data(), ad.data(), n, m)) return PHDE_ERR_ARG;
+  phde_ctx *c = new phde_ctx();
+  int rc = ctx_init(c, n, m, weights_dev != nullptr);
+  if (rc == PHDE_OK) {
+    if (hipMemcpy(c->d_rowptr, rowptr_dev, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToDevice) !=
+            hipSuccess ||
+        (m > 0 && hipMemcpy(c->d_adj, adj_dev, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice) != hipSuccess) ||
+        (weights_dev && m > 0 &&
+         hipMemcpy(c->d_w, weights_dev, sizeof(int32_t) * m, hipMemcpyDeviceToDevice) != hipSuccess))
+      rc = PHDE_ERR_DEVICE;
+  }
+  if (rc == PHDE_OK) rc = ctx_finish(c);
+  if (rc != PHDE_OK) {
+    ctx_free(c);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return PHDE_OK;
+}
+
+int phde_destroy(phde_ctx *ctx) {
+  if (!ctx) return PHDE_ERR_ARG;
+  ctx_free(ctx);
+  delete ctx;
+  return PHDE_OK;
+}
+
+int64_t phde_num_vertices(const phde_ctx *ctx) { return ctx ? ctx->g.n : -1; }
+
+int phde_bfs_dev(phde_ctx *c, int64_t source, int alpha, int beta, int32_t *dist_dev) {
+  if (!c || !dist_dev || source < 0 || source >= c->g.n) return PHDE_ERR_ARG;
+  PHDE_TRY(bfs_run(c, source, alpha, beta, c->d_dist, nullptr, false));
+  PHDE_TRY(dist_finalize(c, c->d_dist, dist_dev, kUnvisited, -1));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  return PHDE_OK;
+}
+
+int phde_bfs(phde_ctx *c, int64_t source, int alpha, int beta, int32_t *dist) {
+  if (!c || !dist || source < 0 || source >= c->g.n) return PHDE_ERR_ARG;
+  PHDE_TRY(bfs_run(c, source, alpha, beta, c->d_dist, nullptr, false));
+  PHDE_TRY(dist_finalize(c, c->d_dist, c->d_dist, kUnvisited, -1));
+  PHDE_HIP(hipMemcpyAsync(dist, c->d_dist, sizeof(int32_t) * c->g.n, hipMemcpyDeviceToHost,
+                          c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  return PHDE_OK;
+}
+
+int phde_sssp(phde_ctx *c, int64_t source, int32_t delta, int32_t *dist) {
+  if (!c || !dist || source < 0 || source >= c->g.n) return PHDE_ERR_ARG;
+  if (!c->g.w) return PHDE_ERR_WEIGHTS;
+  PHDE_TRY(bfs_setup(c));
+  PHDE_TRY(sssp_run(c, source, delta, c->d_dist, nullptr, false));
+  PHDE_HIP(hipMemcpyAsync(dist, c->d_dist, sizeof(int32_t) * c->g.n, hipMemcpyDeviceToHost,
+                          c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  return PHDE_OK;
+}
+
+int phde_msbfs(phde_ctx *c, const int64_t *sources, int k, int32_t *dist) {
+  if (!c || !sources || !dist || k < 1 || k > 64) return PHDE_ERR_ARG;
+  for (int i = 0; i < k; i++)
+    if (sources[i] < 0 || sources[i] >= c->g.n) return PHDE_ERR_ARG;
+  DBuf<int32_t> D;
+  PHDE_TRY(D.alloc((size_t)c->g.n * k));
+  PHDE_TRY(msbfs_run(c, sources, k, D.p));
+  PHDE_HIP(hipMemcpyAsync(dist, D.p, sizeof(int32_t) * c->g.n * k, hipMemcpyDeviceToHost,
+                          c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  return PHDE_OK;
+}
+
+int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats *stats) {
+  if (!c || !opt || !coords) return PHDE_ERR_ARG;
+  const phde_options &o = *opt;
+  if (o.s < 1 || o.s > 64 || o.num_coord < 1 || o.num_coord > 8 || o.num_coord > o.s ||
+      o.first_pivot < 0 || o.first_pivot >= c->g.n || o.algorithm < 0 || o.algorithm > 3 ||
+      o.precision < 0 || o.precision > 1 || o.eig_mode < 0 || o.eig_mode > 2) {
+    set_error("invalid layout options");
+    return PHDE_ERR_ARG;
+  }
+  if (o.strategy != PHDE_PIVOTS_KCENTERS) {
+    set_error("pivot strategy not available");
+    return PHDE_ERR_UNSUPPORTED;
+  }
+  if (c->g.n < 2) {
+    set_error("graph too small");
+    return PHDE_ERR_ARG;
+  }
+  PHDE_TRY(bfs_setup(c));
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  c->timer.reset();
+  c->timer.enabled = stats != nullptr;
+  LayoutOut out;
+  out.st = stats;
+  auto t0 = clk::now();
+  int rc;
+  if (o.algorithm == PHDE_ALGO_PARHDE || o.algorithm == PHDE_ALGO_WPARHDE)
+    rc = layout_hde(c, &o, coords, out);
+  else
+    rc = layout_mds(c, &o, coords, out);
+  const double total = ms_since(t0);
+  c->timer.resolve();
+  c->timer.enabled = false;
+  if (stats) {
+    stats->ms_total = total;
+    stats->ms_distance = out.t_dist;
+    stats->ms_orth = out.t_orth;
+    stats->ms_mmult = out.t_mmult;
+    stats->ms_eig = out.t_eig;
+    stats->ms_coords = out.t_coords;
+    stats->n_traversals = out.ntrav;
+    stats->n_accepted = out.nacc;
+    stats->n_discarded = out.ndisc;
+    stats->eig_iters = out.iters;
+    stats->levels_total = out.levels;
+    for (int k = 0; k < 16; k++) {
+      stats->kernel_ms[k] = c->timer.ms[k];
+      stats->kernel_launches[k] = c->timer.launches[k];
+    }
+  }
+  return rc;
+}
+
+int phde_power_eig(phde_ctx *c, double *K, double *evals, int dim, int32_t *iters) {
+  if (!c || !K || !evals || dim < 1 || dim > 64) return PHDE_ERR_ARG;
+  DBuf<double> Kd, init, ev, work;
+  DBuf<int> status;
+  PHDE_TRY(Kd.alloc((size_t)dim * dim));
+  PHDE_TRY(init.alloc((size_t)dim * dim));
+  PHDE_TRY(ev.alloc(dim));
+  PHDE_TRY(work.alloc((size_t)dim * dim));
+  PHDE_TRY(status.alloc(4));
+  std::vector<int32_t> r(dim * dim);
+  glibc_rand_sequence(1, dim * dim, r.data());
+  std::vector<double> iv(dim * dim);
+  for (int i = 0; i < dim * dim; i++) iv[i] = ((double)r[i]) / 2147483647;
+  PHDE_HIP(hipMemcpyAsync(Kd.p, K, sizeof(double) * dim * dim, hipMemcpyHostToDevice, c->stream));
+  PHDE_HIP(hipMemcpyAsync(init.p, iv.data(), sizeof(double) * dim * dim, hipMemcpyHostToDevice,
+                          c->stream));
+  PHDE_TRY(power_eig_dev(c, Kd.p, init.p, ev.p, dim, 100000, work.p, status.p));
+  int hs[2] = {0, 0};
+  PHDE_HIP(hipMemcpyAsync(hs, status.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipMemcpyAsync(K, Kd.p, sizeof(double) * dim * dim, hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipMemcpyAsync(evals, ev.p, sizeof(double) * dim, hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  if (iters) *iters = hs[1];
+  return hs[0] == 0 ? PHDE_OK : PHDE_ERR_EIG;
+}
+
+int phde_sym_eig(phde_ctx *c, const double *A, const double *B, int dim, double *evals,
+                 double *evecs) {
+  if (!c || !A || !evals || !evecs || dim < 1 || dim > 64) return PHDE_ERR_ARG;
+  DBuf<double> Ad, Bd, ev, V, work;
+  DBuf<int> status;
+  PHDE_TRY(Ad.alloc((size_t)dim * dim));
+  PHDE_TRY(ev.alloc(dim));
+  PHDE_TRY(V.alloc((size_t)dim * dim));
+  PHDE_TRY(work.alloc(3 * (size_t)dim * dim));
+  PHDE_TRY(status.alloc(4));
+  PHDE_HIP(hipMemcpyAsync(Ad.p, A, sizeof(double) * dim * dim, hipMemcpyHostToDevice, c->stream));
+  if (B) {
+    PHDE_TRY(Bd.alloc((size_t)dim * dim));
+    PHDE_HIP(hipMemcpyAsync(Bd.p, B, sizeof(double) * dim * dim, hipMemcpyHostToDevice, c->stream));
+  }
+  PHDE_TRY(sym_eig_dev(c, Ad.p, B ? Bd.p : nullptr, dim, ev.p, V.p, work.p, status.p));
+  int hs[2] = {0, 0};
+  PHDE_HIP(hipMemcpyAsync(hs, status.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipMemcpyAsync(evals, ev.p, sizeof(double) * dim, hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipMemcpyAsync(evecs, V.p, sizeof(double) * dim * dim, hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  return hs[0] == 0 ? PHDE_OK : PHDE_ERR_EIG;
+}
+
+int phde_hall_energy(phde_ctx *c, const double *Y, int nc, double *energy) {
+  if (!c || !Y || !energy || nc < 1) return PHDE_ERR_ARG;
+  DBuf<double> Yd, part;
+  PHDE_TRY(Yd.alloc((size_t)c->g.n * nc));
+  PHDE_TRY(part.alloc(2 * (size_t)c->num_cus * 4 + 16));
+  PHDE_HIP(hipMemcpyAsync(Yd.p, Y, sizeof(double) * c->g.n * nc, hipMemcpyHostToDevice, c->stream));
+  double h[2];
+  PHDE_TRY(hall_energy_dev(c, Yd.p, nc, part.p, h));
+  *energy = h[0] / h[1];
+  return PHDE_OK;
+}
+
+}  // extern "C"

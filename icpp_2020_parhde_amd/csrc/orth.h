@@ -1,0 +1,37 @@
+// orth.h — internal interface of the D-orthogonalisation stage (orth.hip).
+#pragma once
+#include "phde_common.h"
+
+namespace phde {
+
+// Slot of the ones vector in the coefficient / denominator arrays
+// (accepted columns use slots 0..s-1, s <= 64).
+constexpr int kOnesSlot = 64;
+
+// Device state of one D-orthogonalisation: S (n x s, column-major, FP64 or
+// FP32 storage), the candidate a (FP64), cached denominators
+// den_k = sum_i S_ik^2 deg_i, and reduction scratch.
+struct Orth {
+  phde_ctx *ctx = nullptr;
+  int64_t n = 0;
+  int s = 0;
+  bool fp32 = false;
+  int nblk = 0;
+  void *S = nullptr;
+  double *a = nullptr, *part = nullptr, *den = nullptr, *coef = nullptr, *scal = nullptr;
+  double *h_scal = nullptr, *ones = nullptr;
+  const double *deg_ = nullptr;
+  double u0 = 0;
+
+  int setup(phde_ctx *c, int64_t n, int s, bool fp32);
+  void release();
+  ~Orth() { release(); }
+  int init_ones(const double *deg, double u0);
+  int load_candidate(const int32_t *dist, double scale);  // a = dist * scale
+  int cgs_pass(int ncols);  // classical GS of a against ones + S[:, :ncols]; norm in scal
+  int mgs_pass(int ncols);  // modified GS, same operands
+  int read_norm(double *norm);  // synchronises
+  int accept(int j);            // S[:, j] = a / ||a||, den_j
+};
+
+}  // namespace phde

@@ -1,0 +1,351 @@
+// orth.hip — degree-weighted Gram-Schmidt (D-orthogonalisation) of pivot
+// distance columns, as tall-skinny n x s reductions on gfx950.
+//
+// Reference behaviour followed:
+//  - ParHDE CGS (ParHDE.cpp:190-285): every candidate is projected against
+//    the ones vector and all accepted columns using coefficients computed
+//    from the original candidate, a -= sum_k (num_k / den_k) S_k with
+//    num_k = sum_i a_i S_ik deg_i and den_k = sum_i S_ik^2 deg_i; it is
+//    discarded when ||a||_2 < 0.001, else S_j = a / ||a||_2.
+//  - WParHDE MGS (WParHDE.cpp:167-214): the same projection done one column
+//    at a time on the updated candidate.
+//
+// Layout: S is column-major (column k at S + k*n) in FP64 or FP32 storage;
+// every reduction accumulates in FP64 and is combined in a fixed order
+// (per-block partials, then one block), so results are deterministic.
+// The candidate a is an FP64 vector of n entries.
+#include <algorithm>
+
+#include "phde_common.h"
+#include "orth.h"
+
+namespace phde {
+
+constexpr int kOrthBlock = 256;
+constexpr int kColChunk = 16;
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  __syncthreads();
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x / kWave); i++) t += sh[i];
+  return t;  // valid in thread 0
+}
+
+__global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void fill_f64_kernel(double *p, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// a_i = dist_i * scale (the column normalisation of ParHDE.cpp:176-185 /
+// WParHDE.cpp:154-162 applied to an exact integer distance column).
+__global__ void cand_from_dist_kernel(const int32_t *dist, double scale, double *a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = (double)dist[i] * scale;
+}
+
+// Partial sums over a row range of num_k = sum_i (a_i deg_i) S_ik for the
+// columns of one chunk (blockIdx.y), plus the ones-vector term (u0 constant)
+// for chunk 0.  part layout: [col][rowblock], col ncols = ones vector.
+template <typename T>
+__global__ __launch_bounds__(kOrthBlock) void cgs_dots_kernel(const double *a, const double *deg,
+                                                             const T *S, int64_t n, int ncols,
+                                                             double u0, double *part) {
+  __shared__ double sh[kOrthBlock / kWave];
+  const int k0 = blockIdx.y * kColChunk;
+  const int kc = min(kColChunk, ncols - k0);
+  double acc[kColChunk];
+#pragma unroll
+  for (int c = 0; c < kColChunk; c++) acc[c] = 0;
+  double acc0 = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double ad = a[i] * deg[i];
+    acc0 += ad * u0;
+#pragma unroll
+    for (int c = 0; c < kColChunk; c++)
+      if (c < kc) acc[c] += ad * (double)S[(int64_t)(k0 + c) * n + i];
+  }
+  for (int c = 0; c < kColChunk; c++) {
+    if (c >= kc) break;
+    const double t = block_sum(acc[c], sh);
+    if (threadIdx.x == 0) part[(int64_t)(k0 + c) * gridDim.x + blockIdx.x] = t;
+  }
+  if (blockIdx.y == 0) {
+    const double t = block_sum(acc0, sh);
+    if (threadIdx.x == 0) part[(int64_t)ncols * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// coef_k = num_k * (1 / den_k) for k < ncols, and the ones vector in slot
+// kOnesSlot (ParHDE.cpp:239-244).  One block; thread k sums its column's
+// partials in block order.
+__global__ void cgs_coef_kernel(const double *part, int nblk, int ncols, const double *den,
+                                double *coef) {
+  const int k = threadIdx.x;
+  if (k > ncols) return;
+  double s = 0;
+  for (int b = 0; b < nblk; b++) s += part[(int64_t)k * nblk + b];
+  const int slot = k == ncols ? kOnesSlot : k;
+  coef[slot] = s * (1 / den[slot]);
+}
+
+// a_i -= sum_k coef_k S_ik + coef_ones * u0, and per-block sums of a_i^2
+// (ParHDE.cpp:245-265).
+template <typename T>
+__global__ __launch_bounds__(kOrthBlock) void cgs_update_kernel(double *a, const T *S, int64_t n,
+                                                               int ncols, double u0,
+                                                               const double *coef,
+                                                               double *part) {
+  __shared__ double sc[kOnesSlot + 1];
+  __shared__ double sh[kOrthBlock / kWave];
+  for (int k = threadIdx.x; k <= kOnesSlot; k += blockDim.x) sc[k] = coef[k];
+  __syncthreads();
+  double ss = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double acc = 0;
+    for (int k = 0; k < ncols; k++) acc += sc[k] * (double)S[(int64_t)k * n + i];
+    acc += sc[kOnesSlot] * u0;
+    const double v = a[i] - acc;
+    a[i] = v;
+    ss += v * v;
+  }
+  const double t = block_sum(ss, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// out[0] = sqrt(sum of partials) in block order; out[1] = the sum itself.
+__global__ void norm_finalize_kernel(const double *part, int nblk, double *out) {
+  __shared__ double sh[kOrthBlock / kWave];
+  double s = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[b];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    out[0] = sqrt(t);
+    out[1] = t;
+  }
+}
+
+// S_j = a * (1 / ||a||) (ParHDE.cpp:271-274) and partial sums of
+// S_j^2 deg for the cached denominator of column j.
+template <typename T>
+__global__ __launch_bounds__(kOrthBlock) void cgs_accept_kernel(const double *a,
+                                                               const double *norm, const double *deg,
+                                                               T *Sj, int64_t n, double *part) {
+  __shared__ double sh[kOrthBlock / kWave];
+  const double inv = 1 / norm[0];
+  double ss = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const T v = (T)(a[i] * inv);
+    Sj[i] = v;
+    const double dv = (double)v;
+    ss += dv * dv * deg[i];
+  }
+  const double t = block_sum(ss, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void den_finalize_kernel(const double *part, int nblk, double *den_slot) {
+  __shared__ double sh[kOrthBlock / kWave];
+  double s = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[b];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) *den_slot = t;
+}
+
+// Weighted dot sum_i x_i y_i deg_i (x, y may be FP64 or T storage); used by
+// MGS for num and den of one column.
+template <typename TX, typename TY>
+__global__ __launch_bounds__(kOrthBlock) void wdot_kernel(const TX *x, const TY *y,
+                                                         const double *deg, int64_t n,
+                                                         double *part) {
+  __shared__ double sh[kOrthBlock / kWave];
+  double s = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double yv = (double)y[i];
+    s += (double)x[i] * (yv * deg[i]);
+  }
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// MGS coefficient: out = num * (1 / den) with num from partials and den
+// the cached denominator of the column (WParHDE.cpp:175-183).
+__global__ void mgs_coef_kernel(const double *part_num, int nblk, const double *den_slot,
+                                double *out) {
+  __shared__ double sh[kOrthBlock / kWave];
+  double sn = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) sn += part_num[b];
+  const double tn = block_sum(sn, sh);
+  if (threadIdx.x == 0) out[0] = tn * (1 / den_slot[0]);
+}
+
+// Per-block sums of a_i^2.
+__global__ __launch_bounds__(kOrthBlock) void sumsq_kernel(const double *a, int64_t n,
+                                                          double *part) {
+  __shared__ double sh[kOrthBlock / kWave];
+  double s = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    s += a[i] * a[i];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// a_i -= c * y_i (WParHDE.cpp:184-188: (num * U_k) * (1/den)).
+template <typename TY>
+__global__ void mgs_axpy_kernel(double *a, const TY *y, const double *c, int64_t n) {
+  const double cc = c[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    a[i] -= cc * (double)y[i];
+}
+
+// ---------------------------------------------------------------- host side
+
+static int grid_for(int64_t n, int cap) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((n + kOrthBlock - 1) / kOrthBlock, cap));
+}
+
+int fill_i32(phde_ctx *c, int32_t *p, int64_t n, int32_t v) {
+  fill_i32_kernel<<<grid_for(n, 4096), kOrthBlock, 0, c->stream>>>(p, n, v);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+int fill_f64(phde_ctx *c, double *p, int64_t n, double v) {
+  fill_f64_kernel<<<grid_for(n, 4096), kOrthBlock, 0, c->stream>>>(p, n, v);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+int Orth::setup(phde_ctx *c, int64_t n_, int s_, bool fp32_) {
+  ctx = c;
+  n = n_;
+  s = s_;
+  fp32 = fp32_;
+  nblk = grid_for(n, 2 * std::max(c->num_cus, 1) * 4);
+  const size_t esz = fp32 ? sizeof(float) : sizeof(double);
+  PHDE_HIP(hipMalloc(&S, esz * (size_t)n * s));
+  PHDE_HIP(hipMalloc(&a, sizeof(double) * n));
+  PHDE_HIP(hipMalloc(&part, sizeof(double) * (size_t)nblk * (kOnesSlot + 2) * 2));
+  PHDE_HIP(hipMalloc(&den, sizeof(double) * (kOnesSlot + 1)));
+  PHDE_HIP(hipMalloc(&coef, sizeof(double) * (kOnesSlot + 1)));
+  PHDE_HIP(hipMalloc(&scal, sizeof(double) * 8));
+  PHDE_HIP(hipHostMalloc(&h_scal, sizeof(double) * 8, hipHostMallocDefault));
+  return PHDE_OK;
+}
+
+void Orth::release() {
+  if (S) (void)hipFree(S);
+  if (a) (void)hipFree(a);
+  if (part) (void)hipFree(part);
+  if (den) (void)hipFree(den);
+  if (coef) (void)hipFree(coef);
+  if (scal) (void)hipFree(scal);
+  if (ones) (void)hipFree(ones);
+  if (h_scal) (void)hipHostFree(h_scal);
+  S = nullptr;
+  a = part = den = coef = scal = h_scal = ones = nullptr;
+}
+
+// den of the ones vector: sum_i u0*u0*deg_i (ParHDE.cpp:226).
+int Orth::init_ones(const double *deg, double u0_) {
+  deg_ = deg;
+  u0 = u0_;
+  // a = u0 everywhere, then reuse the weighted dot.
+  int rc = fill_f64(ctx, a, n, u0);
+  if (rc) return rc;
+  wdot_kernel<double, double><<<nblk, kOrthBlock, 0, ctx->stream>>>(a, a, deg, n, part);
+  den_finalize_kernel<<<1, kOrthBlock, 0, ctx->stream>>>(part, nblk, den + kOnesSlot);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+int Orth::load_candidate(const int32_t *dist, double scale) {
+  cand_from_dist_kernel<<<grid_for(n, 4096), kOrthBlock, 0, ctx->stream>>>(dist, scale, a, n);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+template <typename T>
+static int cgs_pass_t(Orth &o, int ncols) {
+  hipStream_t st = o.ctx->stream;
+  const T *S = (const T *)o.S;
+  const int nchunk = std::max(1, (ncols + kColChunk - 1) / kColChunk);
+  cgs_dots_kernel<T><<<dim3(o.nblk, nchunk), kOrthBlock, 0, st>>>(o.a, o.deg_, S, o.n, ncols,
+                                                                  o.u0, o.part);
+  cgs_coef_kernel<<<1, 128, 0, st>>>(o.part, o.nblk, ncols, o.den, o.coef);
+  cgs_update_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(o.a, S, o.n, ncols, o.u0, o.coef,
+                                                      o.part);
+  norm_finalize_kernel<<<1, kOrthBlock, 0, st>>>(o.part, o.nblk, o.scal);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+int Orth::cgs_pass(int ncols) { return fp32 ? cgs_pass_t<float>(*this, ncols) : cgs_pass_t<double>(*this, ncols); }
+
+template <typename T>
+static int mgs_pass_t(Orth &o, int ncols) {
+  hipStream_t st = o.ctx->stream;
+  const T *S = (const T *)o.S;
+  // ones vector first (U[:,0] of WParHDE.cpp:170), then accepted columns
+  // in order, each against the updated candidate.
+  wdot_kernel<double, double><<<o.nblk, kOrthBlock, 0, st>>>(o.a, o.ones, o.deg_, o.n, o.part);
+  mgs_coef_kernel<<<1, kOrthBlock, 0, st>>>(o.part, o.nblk, o.den + kOnesSlot, o.coef);
+  mgs_axpy_kernel<double><<<grid_for(o.n, 4096), kOrthBlock, 0, st>>>(o.a, o.ones, o.coef, o.n);
+  for (int k = 0; k < ncols; k++) {
+    const T *Sk = S + (int64_t)k * o.n;
+    wdot_kernel<double, T><<<o.nblk, kOrthBlock, 0, st>>>(o.a, Sk, o.deg_, o.n, o.part);
+    mgs_coef_kernel<<<1, kOrthBlock, 0, st>>>(o.part, o.nblk, o.den + k, o.coef);
+    mgs_axpy_kernel<T><<<grid_for(o.n, 4096), kOrthBlock, 0, st>>>(o.a, Sk, o.coef, o.n);
+  }
+  sumsq_kernel<<<o.nblk, kOrthBlock, 0, st>>>(o.a, o.n, o.part);
+  norm_finalize_kernel<<<1, kOrthBlock, 0, st>>>(o.part, o.nblk, o.scal);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+int Orth::mgs_pass(int ncols) {
+  if (!ones) {
+    PHDE_HIP(hipMalloc(&ones, sizeof(double) * n));
+    int rc = fill_f64(ctx, ones, n, u0);
+    if (rc) return rc;
+  }
+  return fp32 ? mgs_pass_t<float>(*this, ncols) : mgs_pass_t<double>(*this, ncols);
+}
+
+int Orth::read_norm(double *norm) {
+  PHDE_HIP(hipMemcpyAsync(h_scal, scal, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
+  PHDE_HIP(hipStreamSynchronize(ctx->stream));
+  *norm = h_scal[0];
+  return PHDE_OK;
+}
+
+template <typename T>
+static int accept_t(Orth &o, int j) {
+  hipStream_t st = o.ctx->stream;
+  T *Sj = (T *)o.S + (int64_t)j * o.n;
+  cgs_accept_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(o.a, o.scal, o.deg_, Sj, o.n, o.part);
+  den_finalize_kernel<<<1, kOrthBlock, 0, st>>>(o.part, o.nblk, o.den + j);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+int Orth::accept(int j) { return fp32 ? accept_t<float>(*this, j) : accept_t<double>(*this, j); }
+
+}  // namespace phde

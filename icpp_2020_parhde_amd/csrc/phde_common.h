@@ -1,0 +1,117 @@
+// phde_common.h — internal declarations shared by the HIP sources of
+// libparhde.so.  Not installed; the public boundary is include/parhde.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "parhde.h"
+
+namespace phde {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+#define PHDE_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      ::phde::set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+      return PHDE_ERR_DEVICE;                                                       \
+    }                                                                               \
+  } while (0)
+
+constexpr int32_t kUnvisited = 0x7fffffff;  // BFS level of an unreached vertex
+constexpr int kWave = 64;
+
+// Per-level frontier counters of the persistent BFS (three rotating slots).
+struct BfsSlot {
+  unsigned int q;            // frontier vertices
+  unsigned int huge;         // of which degree >= kHugeDegree
+  unsigned long long scout;  // sum of their degrees
+  unsigned int pad[4];
+};
+
+// Result of the fused k-centers epilogue of one traversal.
+struct KcResult {
+  unsigned long long key;        // (min_dist << 32) | (0xffffffff - vertex), maximised
+  unsigned long long sumsq;      // sum of dist^2 (exact)
+  unsigned long long unreached;  // vertices left at kUnvisited
+  long long depth;               // eccentricity of the source (BFS levels)
+};
+
+struct DevGraph {
+  int64_t n, m;
+  const uint32_t *rowptr;
+  const uint32_t *adj;
+  const int32_t *w;      // nullptr when unweighted
+  const int32_t *deg32;  // vertex degree (count of neighbours)
+  const double *degw;    // degree as used by the Laplacian (count or weight sum)
+};
+
+// Timing of kernel classes with HIP events on the context stream.
+struct Timer {
+  bool enabled = false;
+  struct Span {
+    int cls;
+    hipEvent_t a, b;
+  };
+  std::vector<Span> open;
+  std::vector<hipEvent_t> pool;
+  double ms[16] = {0};
+  long long launches[16] = {0};
+  hipEvent_t get();
+  void begin(hipStream_t s, int cls, hipEvent_t *a);
+  void end(hipStream_t s, int cls, hipEvent_t a);
+  int resolve();  // waits for recorded spans, accumulates ms
+  void reset();
+  ~Timer();
+};
+
+}  // namespace phde
+
+struct phde_ctx {
+  int device = 0;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  phde::DevGraph g{};
+  uint32_t *d_rowptr = nullptr, *d_adj = nullptr;
+  int32_t *d_w = nullptr, *d_deg32 = nullptr;
+  double *d_degw = nullptr;
+  // traversal scratch
+  int32_t *d_dist = nullptr;
+  uint32_t *d_q[2] = {nullptr, nullptr};
+  uint32_t *d_hq[2] = {nullptr, nullptr};
+  uint64_t *d_bm[2] = {nullptr, nullptr};
+  phde::BfsSlot *d_slots = nullptr;
+  phde::KcResult *d_kc = nullptr;
+  phde::KcResult *h_kc = nullptr;  // pinned
+  int bfs_grid = 0;
+  // generic pinned staging for small read-backs
+  double *h_small = nullptr;
+  // reduction scratch
+  double *d_part = nullptr;
+  size_t part_elems = 0;
+  phde::Timer timer;
+};
+
+namespace phde {
+
+// bfs.hip
+int bfs_setup(phde_ctx *c);
+// Runs one traversal from `src`, writing levels into dist_out (device,
+// kUnvisited for unreached).  When min_dist != nullptr the k-centers update
+// and argmax are fused; result in c->d_kc (device) and, after the call
+// returns, c->h_kc (host, synchronised).
+int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, int32_t *min_dist,
+            bool sync_result);
+// Unreached -> -1 conversion for API output.
+int dist_finalize(phde_ctx *c, const int32_t *dist_in, int32_t *dist_out, int32_t unreached_val,
+                  int32_t unreached_out);
+
+// orth.hip — D-orthogonalisation and reductions
+int fill_i32(phde_ctx *c, int32_t *p, int64_t n, int32_t v);
+int fill_f64(phde_ctx *c, double *p, int64_t n, double v);
+
+}  // namespace phde

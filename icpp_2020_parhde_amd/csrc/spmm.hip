@@ -1,0 +1,350 @@
+// spmm.hip — Laplacian times the pivot matrix, the s x s Gram product,
+// coordinates and Hall energy, for gfx950.
+//
+// Reference behaviour followed:
+//  - L*S (ParHDE.cpp:288-300, WParHDE.cpp:229-241): LS_ik = deg_i S_ik minus
+//    sum over neighbours j of (w_ij) S_jk, subtracted in adjacency order.
+//  - S^T(LS) (DrawingBase.cpp:289-312 matrixTransMultiplication):
+//    C[k + i*s] = sum_j S_ji LS_jk, the transposed storage the eigensolver
+//    consumes.
+//  - computeCoordinates (DrawingBase.cpp:275-287) and getCoordinates'
+//    column normalisation (DrawingBase.cpp:264-273).
+//
+// Layout: the SpMM gathers whole neighbour rows, so it works on a row-major
+// copy of S (n x s, row i contiguous): one wave per vertex, lane k holds
+// column k (s <= 64 = wave width), each neighbour costs one coalesced
+// s-wide row read.  The Gram product reads row tiles of S and LS through
+// LDS; per-block s*s partials are summed in block order (deterministic).
+#include <algorithm>
+
+#include "phde_common.h"
+#include "spmm.h"
+
+namespace phde {
+
+constexpr int kBlk = 256;
+
+// S column-major [s][n] -> row-major [n][s]; 64 rows per block via LDS.
+template <typename T>
+__global__ __launch_bounds__(kBlk) void col_to_row_kernel(const T *Sc, T *Sr, int64_t n, int s) {
+  __shared__ T tile[64][65];
+  const int t = threadIdx.x;
+  for (int64_t r0 = (int64_t)blockIdx.x * 64; r0 < n; r0 += (int64_t)gridDim.x * 64) {
+    const int rows = (int)std::min<int64_t>(64, n - r0);
+    for (int k = t / 64; k < s; k += kBlk / 64) {
+      const int r = t % 64;
+      if (r < rows) tile[r][k] = Sc[(int64_t)k * n + r0 + r];
+    }
+    __syncthreads();
+    for (int e = t; e < rows * s; e += kBlk) {
+      const int r = e / s, k = e % s;
+      Sr[(r0 + r) * s + k] = tile[r][k];
+    }
+    __syncthreads();
+  }
+}
+
+// LS = L * S, row-major in and out.  One wave per row, lane = column.
+template <typename T, bool W>
+__global__ __launch_bounds__(kBlk) void laplacian_spmm_kernel(DevGraph g, const T *S, T *LS, int s) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t wave = ((int64_t)blockIdx.x * kBlk + threadIdx.x) / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlk / kWave);
+  const bool on = lane < s;
+  for (int64_t i = wave; i < g.n; i += nwaves) {
+    const uint32_t b = g.rowptr[i], e = g.rowptr[i + 1];
+    double v = on ? g.degw[i] * (double)S[i * s + lane] : 0.0;
+    for (uint32_t e0 = b; e0 < e; e0 += kWave) {
+      const uint32_t cnt = min((uint32_t)kWave, e - e0);
+      const uint32_t my = (lane < (int)cnt) ? g.adj[e0 + lane] : 0u;
+      int32_t myw = 0;
+      if (W) myw = (lane < (int)cnt) ? g.w[e0 + lane] : 0;
+      uint32_t t = 0;
+      for (; t + 4 <= cnt; t += 4) {
+        const uint32_t j0 = __shfl(my, t), j1 = __shfl(my, t + 1), j2 = __shfl(my, t + 2),
+                       j3 = __shfl(my, t + 3);
+        double x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+        if (on) {
+          x0 = (double)S[(int64_t)j0 * s + lane];
+          x1 = (double)S[(int64_t)j1 * s + lane];
+          x2 = (double)S[(int64_t)j2 * s + lane];
+          x3 = (double)S[(int64_t)j3 * s + lane];
+        }
+        if (W) {
+          x0 *= (double)__shfl(myw, t);
+          x1 *= (double)__shfl(myw, t + 1);
+          x2 *= (double)__shfl(myw, t + 2);
+          x3 *= (double)__shfl(myw, t + 3);
+        }
+        v -= x0;
+        v -= x1;
+        v -= x2;
+        v -= x3;
+      }
+      for (; t < cnt; t++) {
+        const uint32_t j = __shfl(my, t);
+        double x = on ? (double)S[(int64_t)j * s + lane] : 0.0;
+        if (W) x *= (double)__shfl(myw, t);
+        v -= x;
+      }
+    }
+    if (on) LS[i * s + lane] = (T)v;
+  }
+}
+
+// Per-block partial C[k + i*s] = sum_rows A[r][i] * B[r][k] over row tiles
+// staged in LDS (A, B row-major n x s).
+constexpr int kGramRows = 16;
+constexpr int kGramPer = 16;  // outputs per thread (s*s <= 4096)
+template <typename T>
+__global__ __launch_bounds__(kBlk) void gram_partial_kernel(const T *A, const T *B, int64_t n, int s,
+                                                           double *part) {
+  __shared__ double ta[kGramRows][64];
+  __shared__ double tb[kGramRows][64];
+  const int t = threadIdx.x;
+  const int ss = s * s;
+  double acc[kGramPer];
+  int ii[kGramPer], kk[kGramPer];
+#pragma unroll
+  for (int q = 0; q < kGramPer; q++) {
+    acc[q] = 0;
+    const int e = t + q * kBlk;
+    ii[q] = e < ss ? e / s : 0;
+    kk[q] = e < ss ? e % s : 0;
+  }
+  for (int64_t r0 = (int64_t)blockIdx.x * kGramRows; r0 < n; r0 += (int64_t)gridDim.x * kGramRows) {
+    const int rows = (int)std::min<int64_t>(kGramRows, n - r0);
+    for (int e = t; e < kGramRows * 64; e += kBlk) {
+      const int r = e / 64, k = e % 64;
+      const bool ok = r < rows && k < s;
+      ta[r][k] = ok ? (double)A[(r0 + r) * s + k] : 0.0;
+      tb[r][k] = ok ? (double)B[(r0 + r) * s + k] : 0.0;
+    }
+    __syncthreads();
+    for (int r = 0; r < rows; r++) {
+#pragma unroll
+      for (int q = 0; q < kGramPer; q++) acc[q] += ta[r][ii[q]] * tb[r][kk[q]];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < kGramPer; q++) {
+    const int e = t + q * kBlk;
+    if (e < ss) part[(int64_t)blockIdx.x * ss + e] = acc[q];
+  }
+}
+
+__global__ void gram_reduce_kernel(const double *part, int nblk, int ss, double *C) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ss) return;
+  double s = 0;
+  for (int b = 0; b < nblk; b++) s += part[(int64_t)b * ss + e];
+  // part holds element (k = e % s, i = e / s) of sum_j A[j][i] B[j][k]; the
+  // reference stores it at C[k + i*s] which is exactly index e.
+  C[e] = s;
+}
+
+// X[i + k*n] = sum_j S[i + j*n] V[j + k*s] (S column-major), and per-block
+// sums of X^2 per coordinate.
+template <typename T>
+__global__ __launch_bounds__(kBlk) void coords_kernel(const T *S, const double *V, int64_t n, int s,
+                                                     int c, double *X, double *part) {
+  __shared__ double sv[64 * 8];
+  __shared__ double sh[kBlk / kWave][8];
+  for (int e = threadIdx.x; e < s * c; e += kBlk) sv[e] = V[e];
+  __syncthreads();
+  double ss[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlk + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlk) {
+    for (int k = 0; k < c; k++) {
+      double x = 0;
+      for (int j = 0; j < s; j++) x += (double)S[(int64_t)j * n + i] * sv[j + k * s];
+      X[i + (int64_t)k * n] = x;
+      ss[k] += x * x;
+    }
+  }
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  for (int k = 0; k < c; k++) {
+    double v = ss[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) sh[w][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < c) {
+    double v = 0;
+    for (int q = 0; q < kBlk / kWave; q++) v += sh[q][threadIdx.x];
+    part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = v;
+  }
+}
+
+// inv[k] = 1 / sqrt(sum_b part[k][b]) (getCoordinates, DrawingBase.cpp:268).
+__global__ void coords_norm_kernel(const double *part, int nblk, int c, double *inv) {
+  const int k = threadIdx.x;
+  if (k >= c) return;
+  double s = 0;
+  for (int b = 0; b < nblk; b++) s += part[(int64_t)k * nblk + b];
+  inv[k] = 1 / sqrt(s);
+}
+
+__global__ void scale_cols_kernel(double *X, int64_t n, int c, const double *inv) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * c;
+       e += (int64_t)gridDim.x * blockDim.x)
+    X[e] *= inv[e / n];
+}
+
+// Hall energy partials: num = sum over directed edges of w (y_i - y_j)^2 / 2,
+// den = sum_i degw_i y_i^2, over the c columns of Y (column-major).
+__global__ __launch_bounds__(kBlk) void hall_kernel(DevGraph g, const double *Y, int c, double *part) {
+  __shared__ double sh[kBlk / kWave][2];
+  double num = 0, den = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kBlk + threadIdx.x; i < g.n; i += (int64_t)gridDim.x * kBlk) {
+    for (int k = 0; k < c; k++) {
+      const double yi = Y[i + (int64_t)k * g.n];
+      for (uint32_t e = g.rowptr[i]; e < g.rowptr[i + 1]; e++) {
+        const double d = yi - Y[g.adj[e] + (int64_t)k * g.n];
+        num += 0.5 * (g.w ? (double)g.w[e] : 1.0) * d * d;
+      }
+      den += g.degw[i] * yi * yi;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    num += __shfl_xor(num, o);
+    den += __shfl_xor(den, o);
+  }
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  if (lane == 0) {
+    sh[w][0] = num;
+    sh[w][1] = den;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int q = 0; q < kBlk / kWave; q++) {
+      a += sh[q][0];
+      b += sh[q][1];
+    }
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+
+static int blocks_for(phde_ctx *c, int64_t work, int per_cu = 8) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>(work, (int64_t)c->num_cus * per_cu));
+}
+
+template <typename T>
+static int laplacian_t(phde_ctx *c, const void *Scol, void *Srow, void *LSrow, int s, double *SLS,
+                       double *part, int nblk_gram) {
+  const int64_t n = c->g.n;
+  hipEvent_t ev;
+  c->timer.begin(c->stream, PHDE_KERNEL_OTHER, &ev);
+  col_to_row_kernel<T><<<blocks_for(c, (n + 63) / 64), kBlk, 0, c->stream>>>((const T *)Scol,
+                                                                           (T *)Srow, n, s);
+  c->timer.end(c->stream, PHDE_KERNEL_OTHER, ev);
+  c->timer.begin(c->stream, PHDE_KERNEL_SPMM, &ev);
+  const int sb = blocks_for(c, (n + 3) / 4, 16);
+  if (c->g.w)
+    laplacian_spmm_kernel<T, true><<<sb, kBlk, 0, c->stream>>>(c->g, (const T *)Srow, (T *)LSrow, s);
+  else
+    laplacian_spmm_kernel<T, false><<<sb, kBlk, 0, c->stream>>>(c->g, (const T *)Srow, (T *)LSrow, s);
+  c->timer.end(c->stream, PHDE_KERNEL_SPMM, ev);
+  c->timer.begin(c->stream, PHDE_KERNEL_GRAM, &ev);
+  gram_partial_kernel<T><<<nblk_gram, kBlk, 0, c->stream>>>((const T *)Srow, (const T *)LSrow, n, s,
+                                                           part);
+  gram_reduce_kernel<<<(s * s + 255) / 256, 256, 0, c->stream>>>(part, nblk_gram, s * s, SLS);
+  c->timer.end(c->stream, PHDE_KERNEL_GRAM, ev);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+int Lap::setup(phde_ctx *c, int64_t n, int s_, bool fp32_) {
+  ctx = c;
+  s = s_;
+  fp32 = fp32_;
+  const size_t esz = fp32 ? sizeof(float) : sizeof(double);
+  nblk_gram = blocks_for(c, (n + kGramRows - 1) / kGramRows, 2);
+  PHDE_HIP(hipMalloc(&Srow, esz * (size_t)n * s));
+  PHDE_HIP(hipMalloc(&LSrow, esz * (size_t)n * s));
+  PHDE_HIP(hipMalloc(&part, sizeof(double) * (size_t)std::max(nblk_gram, 1) * s * s));
+  PHDE_HIP(hipMalloc(&SLS, sizeof(double) * s * s));
+  return PHDE_OK;
+}
+
+void Lap::release() {
+  if (Srow) (void)hipFree(Srow);
+  if (LSrow) (void)hipFree(LSrow);
+  if (part) (void)hipFree(part);
+  if (SLS) (void)hipFree(SLS);
+  Srow = LSrow = nullptr;
+  part = SLS = nullptr;
+}
+
+int Lap::run(const void *Scol) {
+  return fp32 ? laplacian_t<float>(ctx, Scol, Srow, LSrow, s, SLS, part, nblk_gram)
+              : laplacian_t<double>(ctx, Scol, Srow, LSrow, s, SLS, part, nblk_gram);
+}
+
+// C = A^T A for a column-major n x s matrix (PivotMDS / PHDE C^T C,
+// PivotMDS.cpp:149-152) using the same row-tiled Gram.
+int Lap::gram_self(const void *Acol) {
+  const int64_t n = ctx->g.n;
+  hipEvent_t ev;
+  ctx->timer.begin(ctx->stream, PHDE_KERNEL_GRAM, &ev);
+  if (fp32) {
+    col_to_row_kernel<float><<<blocks_for(ctx, (n + 63) / 64), kBlk, 0, ctx->stream>>>(
+        (const float *)Acol, (float *)Srow, n, s);
+    gram_partial_kernel<float><<<nblk_gram, kBlk, 0, ctx->stream>>>(
+        (const float *)Srow, (const float *)Srow, n, s, part);
+  } else {
+    col_to_row_kernel<double><<<blocks_for(ctx, (n + 63) / 64), kBlk, 0, ctx->stream>>>(
+        (const double *)Acol, (double *)Srow, n, s);
+    gram_partial_kernel<double><<<nblk_gram, kBlk, 0, ctx->stream>>>(
+        (const double *)Srow, (const double *)Srow, n, s, part);
+  }
+  gram_reduce_kernel<<<(s * s + 255) / 256, 256, 0, ctx->stream>>>(part, nblk_gram, s * s, SLS);
+  ctx->timer.end(ctx->stream, PHDE_KERNEL_GRAM, ev);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+int coords_compute(phde_ctx *c, const void *Scol, bool fp32, int s, int nc, const double *V_dev,
+                   double *X_dev, double *part, double *inv) {
+  const int64_t n = c->g.n;
+  hipEvent_t ev;
+  c->timer.begin(c->stream, PHDE_KERNEL_COORDS, &ev);
+  const int nb = blocks_for(c, (n + kBlk - 1) / kBlk, 4);
+  if (fp32)
+    coords_kernel<float><<<nb, kBlk, 0, c->stream>>>((const float *)Scol, V_dev, n, s, nc, X_dev, part);
+  else
+    coords_kernel<double><<<nb, kBlk, 0, c->stream>>>((const double *)Scol, V_dev, n, s, nc, X_dev,
+                                                      part);
+  coords_norm_kernel<<<1, 64, 0, c->stream>>>(part, nb, nc, inv);
+  scale_cols_kernel<<<blocks_for(c, (n * nc + kBlk - 1) / kBlk, 4), kBlk, 0, c->stream>>>(X_dev, n,
+                                                                                        nc, inv);
+  c->timer.end(c->stream, PHDE_KERNEL_COORDS, ev);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+int hall_energy_dev(phde_ctx *c, const double *Y_dev, int nc, double *part_dev, double *h_out2) {
+  const int nb = blocks_for(c, (c->g.n + kBlk - 1) / kBlk, 4);
+  hall_kernel<<<nb, kBlk, 0, c->stream>>>(c->g, Y_dev, nc, part_dev);
+  PHDE_HIP(hipGetLastError());
+  std::vector<double> h(2 * (size_t)nb);
+  PHDE_HIP(hipMemcpyAsync(h.data(), part_dev, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost,
+                          c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  double a = 0, b = 0;
+  for (int i = 0; i < nb; i++) {
+    a += h[2 * i];
+    b += h[2 * i + 1];
+  }
+  h_out2[0] = a;
+  h_out2[1] = b;
+  return PHDE_OK;
+}
+
+}  // namespace phde

@@ -1,0 +1,258 @@
+"""Python view of the MI355X ParHDE library.
+
+``Context`` owns one graph resident in GPU memory (``phde_create``) and
+exposes the traversals and the layout.  ``DrawingBase`` / ``ParHDE`` /
+``WParHDE`` / ``PivotMDS`` / ``PHDE`` mirror the reference's C++ classes
+(variants/DrawingBase.h:13-48 and subclasses): same constructor arguments,
+``run(tries, alpha, beta, delta)`` and ``getCoordinates()``.  Every call goes
+through libparhde.so; there is no host fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native
+from ._native import PhdeOptions, PhdeStats, as_ptr
+from .graphs import Graph
+
+# include/parhde.h enums
+ALGO_PIVOTMDS, ALGO_PHDE, ALGO_WPARHDE, ALGO_PARHDE = 0, 1, 2, 3
+PIVOTS_RANDCOARSE, PIVOTS_RANDFINE, PIVOTS_KCENTERS, PIVOTS_LEARNED = 0, 1, 2, 3
+PREC_FP64, PREC_FP32 = 0, 1
+EIG_POWER, EIG_JACOBI, EIG_GENERALIZED = 0, 1, 2
+
+STATUS = {
+    0: "ok", -1: "bad argument", -2: "device error", -3: "out of device memory",
+    -4: "too few independent pivots", -5: "eigensolver failed", -6: "weights mismatch",
+    -7: "graph not connected", -8: "unsupported",
+}
+
+KERNEL_CLASSES = ["bfs", "kcenter", "orth", "spmm", "gram", "eig", "coords", "sssp", "msbfs", "other"]
+
+
+class PhdeError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        msg = _native.parhde_lib().phde_last_error().decode(errors="replace")
+        super().__init__(f"{what}: {STATUS.get(code, code)} ({code}) {msg}")
+        self.code = code
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise PhdeError(rc, what)
+
+
+def device_count() -> int:
+    c = ctypes.c_int(0)
+    _native.parhde_lib().phde_device_count(ctypes.byref(c))
+    return c.value
+
+
+@dataclass
+class LayoutStats:
+    ms_total: float = 0.0
+    ms_distance: float = 0.0
+    ms_orth: float = 0.0
+    ms_mmult: float = 0.0
+    ms_eig: float = 0.0
+    ms_coords: float = 0.0
+    n_traversals: int = 0
+    n_accepted: int = 0
+    n_discarded: int = 0
+    eig_iters: int = 0
+    levels_total: int = 0
+    pivots: list = field(default_factory=list)
+    kernel_ms: dict = field(default_factory=dict)
+    kernel_launches: dict = field(default_factory=dict)
+
+    @classmethod
+    def from_c(cls, s: PhdeStats) -> "LayoutStats":
+        nt = int(s.n_traversals)
+        return cls(s.ms_total, s.ms_distance, s.ms_orth, s.ms_mmult, s.ms_eig, s.ms_coords, nt,
+                   int(s.n_accepted), int(s.n_discarded), int(s.eig_iters), int(s.levels_total),
+                   [int(s.pivots[i]) for i in range(min(nt, 192))],
+                   {k: float(s.kernel_ms[i]) for i, k in enumerate(KERNEL_CLASSES)},
+                   {k: int(s.kernel_launches[i]) for i, k in enumerate(KERNEL_CLASSES)})
+
+
+def default_options(**kw) -> PhdeOptions:
+    o = PhdeOptions()
+    _native.parhde_lib().phde_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """One graph in HBM plus a HIP stream (phde_create / phde_destroy)."""
+
+    def __init__(self, g: Graph):
+        self.lib = _native.parhde_lib()
+        self.graph = g
+        self.n = g.n
+        h = ctypes.c_void_p()
+        rp = np.ascontiguousarray(g.rowptr, dtype=np.uint32)
+        ad = np.ascontiguousarray(g.adj, dtype=np.uint32) if g.m > 0 else np.zeros(1, np.uint32)
+        wp = None
+        if g.w is not None:
+            w = np.ascontiguousarray(g.w, dtype=np.int32)
+            wp = as_ptr(w, ctypes.c_int32)
+        _check(self.lib.phde_create(as_ptr(rp, ctypes.c_uint32), as_ptr(ad, ctypes.c_uint32), wp,
+                                    g.n, g.m, ctypes.byref(h)), "phde_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.phde_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ----------------------------------------------------------- traversals
+    def bfs(self, source: int, alpha: int = 15, beta: int = 18) -> np.ndarray:
+        """Levels from `source` (DOBFS, gapbs/bfs.cc:144); -1 when unreached."""
+        d = np.empty(self.n, dtype=np.int32)
+        _check(self.lib.phde_bfs(self.h, source, alpha, beta, as_ptr(d, ctypes.c_int32)), "phde_bfs")
+        return d
+
+    def sssp(self, source: int, delta: int = 0) -> np.ndarray:
+        """Exact distances (DeltaStep, gapbs/sssp.cc:53); 2^30-1 when unreached."""
+        d = np.empty(self.n, dtype=np.int32)
+        _check(self.lib.phde_sssp(self.h, source, delta, as_ptr(d, ctypes.c_int32)), "phde_sssp")
+        return d
+
+    def msbfs(self, sources: Sequence[int]) -> np.ndarray:
+        """Up to 64 BFS in one bit-parallel sweep; n x k column-major levels."""
+        src = np.ascontiguousarray(sources, dtype=np.int64)
+        k = len(src)
+        d = np.empty(self.n * k, dtype=np.int32)
+        _check(self.lib.phde_msbfs(self.h, as_ptr(src, ctypes.c_int64), k, as_ptr(d, ctypes.c_int32)),
+               "phde_msbfs")
+        return d.reshape(k, self.n).T
+
+    # --------------------------------------------------------------- layout
+    def layout(self, options: Optional[PhdeOptions] = None, **kw):
+        """One layout; returns (coords n x c float64, LayoutStats)."""
+        o = options if options is not None else default_options(**kw)
+        if options is not None:
+            for k, v in kw.items():
+                setattr(o, k, v)
+        X = np.empty(self.n * o.num_coord, dtype=np.float64)
+        st = PhdeStats()
+        _check(self.lib.phde_layout(self.h, ctypes.byref(o), as_ptr(X, ctypes.c_double), ctypes.byref(st)),
+               "phde_layout")
+        return X.reshape(o.num_coord, self.n).T.copy(), LayoutStats.from_c(st)
+
+    def power_eig(self, K: np.ndarray):
+        """Reference power method on one s x s matrix (column-major semantics)."""
+        dim = K.shape[0]
+        Kf = np.ascontiguousarray(K.T.reshape(-1), dtype=np.float64).copy()  # column-major buffer
+        ev = np.empty(dim, dtype=np.float64)
+        it = ctypes.c_int32()
+        _check(self.lib.phde_power_eig(self.h, as_ptr(Kf, ctypes.c_double), as_ptr(ev, ctypes.c_double),
+                                       dim, ctypes.byref(it)), "phde_power_eig")
+        return Kf, ev, it.value
+
+    def sym_eig(self, A: np.ndarray, B: Optional[np.ndarray] = None):
+        dim = A.shape[0]
+        Af = np.ascontiguousarray(A.T, dtype=np.float64)
+        Bp = None
+        if B is not None:
+            Bf = np.ascontiguousarray(B.T, dtype=np.float64)
+            Bp = as_ptr(Bf, ctypes.c_double)
+        ev = np.empty(dim)
+        V = np.empty(dim * dim)
+        _check(self.lib.phde_sym_eig(self.h, as_ptr(Af, ctypes.c_double), Bp, dim,
+                                     as_ptr(ev, ctypes.c_double), as_ptr(V, ctypes.c_double)), "phde_sym_eig")
+        return ev, V.reshape(dim, dim).T.copy()
+
+    def hall_energy(self, Y: np.ndarray) -> float:
+        Y = np.asarray(Y, dtype=np.float64)
+        if Y.ndim == 1:
+            Y = Y[:, None]
+        Yc = np.ascontiguousarray(Y.T.reshape(-1))
+        e = ctypes.c_double()
+        _check(self.lib.phde_hall_energy(self.h, as_ptr(Yc, ctypes.c_double), Y.shape[1], ctypes.byref(e)),
+               "phde_hall_energy")
+        return e.value
+
+
+class DrawingBase:
+    """Mirror of the reference's DrawingBase (variants/DrawingBase.h:13-48)."""
+
+    algorithm = ALGO_PARHDE
+
+    def __init__(self, rowPtrs, adj, weights, n: int, m: int, numSources: int, doRand: int = 2,
+                 numCoord: int = 2):
+        self.graph = Graph(int(n), int(m), np.asarray(rowPtrs, np.uint32), np.asarray(adj, np.uint32),
+                           None if weights is None else np.asarray(weights, np.int32))
+        self.n, self.m = int(n), int(m)
+        self.numSources, self.numCoord, self.doRand = numSources, numCoord, doRand
+        self.ctx = Context(self.graph)
+        strategy = {2: PIVOTS_KCENTERS, 1: PIVOTS_RANDFINE, 0: PIVOTS_RANDCOARSE}.get(doRand, PIVOTS_KCENTERS)
+        self.options = default_options(s=numSources, num_coord=numCoord, strategy=strategy,
+                                       algorithm=self.algorithm)
+        self.coordinates = np.zeros((self.n, numCoord))
+        self.stats: Optional[LayoutStats] = None
+
+    def run(self, tries: int = 1, alpha: int = 15, beta: int = 18, delta: int = 1) -> int:
+        self.options.alpha = alpha
+        if delta > 1:
+            self.options.delta = delta
+        for _ in range(max(tries, 1)):
+            X, self.stats = self.ctx.layout(self.options)
+        self.coordinates = X
+        return 0
+
+    def getCoordinates(self) -> np.ndarray:
+        # DrawingBase.cpp:264-273: columns normalised in place on every call
+        self.coordinates = self.coordinates / np.linalg.norm(self.coordinates, axis=0)
+        return self.coordinates
+
+
+class ParHDE(DrawingBase):
+    algorithm = ALGO_PARHDE
+
+    def __init__(self, rowPtrs, adj, weights, n, m, s, r=2, numCoord=2):
+        if weights is not None:
+            raise PhdeError(-6, "ParHDE code doesn't requires the weighted edges")
+        super().__init__(rowPtrs, adj, None, n, m, s, r, numCoord)
+
+
+class WParHDE(DrawingBase):
+    algorithm = ALGO_WPARHDE
+
+    def __init__(self, rowPtrs, adj, weights, n, m, s, numCoord=2):
+        if weights is None:
+            raise PhdeError(-6, "Weighted ParHDE code requires the weighted edges")
+        super().__init__(rowPtrs, adj, weights, n, m, s, 2, numCoord)
+
+
+class PivotMDS(DrawingBase):
+    algorithm = ALGO_PIVOTMDS
+
+    def __init__(self, rowPtrs, adj, weights, n, m, s, r=2, numCoord=2):
+        super().__init__(rowPtrs, adj, weights, n, m, s, r, numCoord)
+
+
+class PHDE(DrawingBase):
+    algorithm = ALGO_PHDE
+
+    def __init__(self, rowPtrs, adj, weights, n, m, s, r=2, numCoord=2):
+        super().__init__(rowPtrs, adj, weights, n, m, s, r, numCoord)

@@ -1,0 +1,162 @@
+/* parhde.h — C ABI of the MI355X ParHDE layout library (libparhde.so).
+ *
+ * This is the drop-in boundary for the reference's layout path.  The
+ * reference has no C ABI of its own: its entry points are the C++ class
+ * `DrawingBase` (variants/DrawingBase.h:13-48) with subclasses ParHDE,
+ * WParHDE, PivotMDS and PHDE, and the two traversal functions DOBFS
+ * (gapbs/bfs.cc:144) and DeltaStep (gapbs/sssp.cc:53), declared at
+ * variants/DrawingBase.h:62-67.  The C++ layer shipped with this library
+ * (include/DrawingBase.h) re-exposes those classes and functions on top of
+ * the calls below; a foreign-language caller binds these calls directly.
+ *
+ * Conventions
+ *  - Graphs are symmetric CSR with uint32 offsets and ids (the binary CSR
+ *    format of spectralDrawing.cpp:136-164); weights are int32 or NULL.
+ *  - Host pointers unless a name ends in _dev (device pointers, HBM).
+ *  - Matrices are column-major, as in the reference (n*c coordinates,
+ *    column k at offset k*n).
+ *  - Every call returns a phde_status; 0 is success.  No call exits the
+ *    process: the C++ layer maps statuses onto the reference's exit codes.
+ *  - A context is single-stream and not reentrant: one thread at a time.
+ */
+#ifndef PARHDE_H
+#define PARHDE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum phde_status {
+  PHDE_OK = 0,
+  PHDE_ERR_ARG = -1,       /* bad argument (size, range, NULL) */
+  PHDE_ERR_DEVICE = -2,    /* HIP runtime error or no usable GPU */
+  PHDE_ERR_NOMEM = -3,     /* device allocation failed */
+  PHDE_ERR_PIVOTS = -4,    /* fewer than s independent pivots (ParHDE.cpp:281-285) */
+  PHDE_ERR_EIG = -5,       /* eigensolver failed (ParallelComponents.cpp:77-81) */
+  PHDE_ERR_WEIGHTS = -6,   /* ParHDE given weights / WParHDE without (ParHDE.cpp:27-31, WParHDE.cpp:22-26) */
+  PHDE_ERR_UNREACHED = -7, /* graph not connected: a traversal left vertices unreached */
+  PHDE_ERR_UNSUPPORTED = -8
+} phde_status;
+
+typedef struct phde_ctx phde_ctx;
+
+/* Algorithms (spectralDrawing.cpp:392-406 `-a`). */
+enum { PHDE_ALGO_PIVOTMDS = 0, PHDE_ALGO_PHDE = 1, PHDE_ALGO_WPARHDE = 2, PHDE_ALGO_PARHDE = 3 };
+/* Pivot strategies (spectralDrawing.cpp:379-391 `-p`). */
+enum { PHDE_PIVOTS_RANDCOARSE = 0, PHDE_PIVOTS_RANDFINE = 1, PHDE_PIVOTS_KCENTERS = 2, PHDE_PIVOTS_LEARNED = 3 };
+/* Arithmetic of the D-orthogonalisation / SpMM stages. */
+enum { PHDE_PREC_FP64 = 0, PHDE_PREC_FP32 = 1 };
+/* Eigensolver: the reference's power method (ParallelComponents.cpp:48-158),
+ * an exact Jacobi solve of the same standard problem, or the generalised
+ * problem (S^T L S) x = lambda (S^T D S) x by Cholesky + Jacobi. */
+enum { PHDE_EIG_POWER = 0, PHDE_EIG_JACOBI = 1, PHDE_EIG_GENERALIZED = 2 };
+
+typedef struct phde_options {
+  int32_t algorithm;   /* PHDE_ALGO_*; default PARHDE */
+  int32_t s;           /* pivots, 1..64 (reference: 1..50, ParHDE.cpp:15) */
+  int32_t num_coord;   /* coordinates c, 1..s */
+  int32_t strategy;    /* PHDE_PIVOTS_*; default KCENTERS */
+  int32_t precision;   /* PHDE_PREC_*; default FP64 */
+  int32_t eig_mode;    /* PHDE_EIG_*; default POWER */
+  int32_t eager;       /* 1: run all 3s traversals like ParHDE.cpp:120; 0: only as many as consumed */
+  int32_t delta;       /* delta-stepping bucket width; <= 0 picks one from the weights */
+  int32_t alpha, beta; /* direction-switch parameters (bfs.cc:161,172); defaults 15, 18 */
+  int32_t verbose;     /* 1: print the reference's progress lines to stdout */
+  int64_t first_pivot; /* first k-centers source; reference: 0 */
+  uint64_t seed;       /* random strategies */
+} phde_options;
+
+typedef struct phde_stats {
+  double ms_total;       /* device time of the layout (HDE mean time semantics) */
+  double ms_distance;    /* traversals + k-centers update + column norms */
+  double ms_orth;        /* D-orthogonalisation */
+  double ms_mmult;       /* L*S and S^T(LS) */
+  double ms_eig;         /* s*s eigensolve */
+  double ms_coords;      /* S*V + normalisation */
+  int32_t n_traversals;  /* BFS / SSSP runs */
+  int32_t n_accepted;
+  int32_t n_discarded;
+  int32_t eig_iters;
+  int64_t levels_total;  /* sum of BFS depths (level-synchronous steps) */
+  int64_t pivots[192];   /* sources in traversal order (first n_traversals valid) */
+  double  kernel_ms[16]; /* per-kernel-class device time, see PHDE_KERNEL_* */
+  int64_t kernel_launches[16];
+} phde_stats;
+
+enum {
+  PHDE_KERNEL_BFS_LEVEL = 0,
+  PHDE_KERNEL_KCENTER = 1,
+  PHDE_KERNEL_ORTH = 2,
+  PHDE_KERNEL_SPMM = 3,
+  PHDE_KERNEL_GRAM = 4,
+  PHDE_KERNEL_EIG = 5,
+  PHDE_KERNEL_COORDS = 6,
+  PHDE_KERNEL_SSSP = 7,
+  PHDE_KERNEL_MSBFS = 8,
+  PHDE_KERNEL_OTHER = 9
+};
+
+/* Fill *o with the reference defaults: ParHDE, s=10, c=2, k-centers, FP64,
+ * power method, lazy traversals, alpha 15, beta 18, first pivot 0. */
+void phde_default_options(phde_options *o);
+
+/* Library / device information. */
+const char *phde_version(void);
+int phde_device_count(int *count);
+
+/* Create a context holding the graph in HBM.  Replaces the borrowed graph
+ * arrays of the DrawingBase constructor (variants/DrawingBase.h:41,
+ * DrawingBase.cpp:84-94); the host arrays may be freed afterwards. */
+int phde_create(const uint32_t *rowptr, const uint32_t *adj, const int32_t *weights, int64_t n,
+                int64_t m, phde_ctx **out);
+/* Same, from device-resident arrays (copied device-to-device). */
+int phde_create_device(const uint32_t *rowptr_dev, const uint32_t *adj_dev,
+                       const int32_t *weights_dev, int64_t n, int64_t m, phde_ctx **out);
+int phde_destroy(phde_ctx *ctx);
+int64_t phde_num_vertices(const phde_ctx *ctx);
+
+/* Breadth-first levels from one source: DOBFS (gapbs/bfs.cc:144-185).
+ * dist[v] = hop count, -1 when unreached.  alpha/beta <= 0 use 15/18. */
+int phde_bfs(phde_ctx *ctx, int64_t source, int alpha, int beta, int32_t *dist);
+int phde_bfs_dev(phde_ctx *ctx, int64_t source, int alpha, int beta, int32_t *dist_dev);
+
+/* Shortest-path distances from one source: DeltaStep (gapbs/sssp.cc:53-140).
+ * dist[v] = exact distance, 2^30-1 when unreached (sssp.cc:50). */
+int phde_sssp(phde_ctx *ctx, int64_t source, int32_t delta, int32_t *dist);
+
+/* Up to 64 breadth-first searches in one sweep (bit-parallel, one 64-bit
+ * mask per vertex).  dist is n*k column-major; -1 when unreached.  No
+ * reference counterpart: it equals k calls of phde_bfs. */
+int phde_msbfs(phde_ctx *ctx, const int64_t *sources, int k, int32_t *dist);
+
+/* One complete layout: DrawingBase::run + getCoordinates (ParHDE.cpp:45-352,
+ * WParHDE.cpp:40-291, PivotMDS.cpp:68-147, PHDE.cpp:50-136,
+ * DrawingBase.cpp:264-273).  coords: n*num_coord column-major, each column
+ * of unit L2 norm.  stats optional. */
+int phde_layout(phde_ctx *ctx, const phde_options *opt, double *coords, phde_stats *stats);
+
+/* The reference's subspace power method (ParallelComponents.cpp:48-158) on
+ * one s*s matrix, run by one GPU workgroup.  K (column-major, dim*dim) is
+ * overwritten with the eigenvector matrix as the reference does; evals gets
+ * the eigenvalue estimates.  iters optional. */
+int phde_power_eig(phde_ctx *ctx, double *K, double *evals, int dim, int32_t *iters);
+
+/* Exact symmetric eigensolve (cyclic Jacobi, one workgroup) of the
+ * symmetric part of A (dim*dim).  With B != NULL, the generalised problem
+ * A x = lambda B x (B symmetric positive definite, Cholesky-reduced).
+ * evals ascending, evecs column-major, B-orthonormal columns. */
+int phde_sym_eig(phde_ctx *ctx, const double *A, const double *B, int dim, double *evals,
+                 double *evecs);
+
+/* Hall energy tr(Y^T L Y) / tr(Y^T D Y) of an n*c column-major layout. */
+int phde_hall_energy(phde_ctx *ctx, const double *Y, int c, double *energy);
+
+/* Last error text of this thread (empty when none). */
+const char *phde_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
