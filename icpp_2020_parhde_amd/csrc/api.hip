@@ -103,6 +103,15 @@ __global__ void gather_cols_kernel(const double *K, int dim, int c, bool largest
   }
 }
 
+// Default delta-stepping bucket width: a few times the largest weight, so a
+// bucket spans several hops (tunable with PHDE_SSSP_DELTA).
+static int32_t pick_delta(const int32_t *w, int64_t m) {
+  if (const char *e = getenv("PHDE_SSSP_DELTA")) return std::max(1, atoi(e));
+  int32_t mx = 1;
+  for (int64_t i = 0; i < m; i++) mx = w[i] > mx ? w[i] : mx;
+  return (int32_t)std::min<int64_t>((int64_t)mx * 4, INT32_MAX / 4);
+}
+
 static int check_graph_host(const uint32_t *rowptr, const uint32_t *adj, int64_t n, int64_t m) {
   if (n < 1 || m < 0 || n > (int64_t)UINT32_MAX || m > (int64_t)UINT32_MAX) return PHDE_ERR_ARG;
   if (rowptr[0] != 0 || (int64_t)rowptr[n] != m) return PHDE_ERR_ARG;
@@ -569,6 +578,16 @@ int phde_create(const uint32_t *rowptr, const uint32_t *adj, const int32_t *weig
     ctx_free(c);
     delete c;
     return rc;
+  }
+  if (weights) {
+    for (int64_t i = 0; i < m; i++)
+      if (weights[i] < 0) {
+        ctx_free(c);
+        delete c;
+        set_error("negative edge weight");
+        return PHDE_ERR_ARG;
+      }
+    c->default_delta = pick_delta(weights, m);
   }
   *out = c;
   return PHDE_OK;

@@ -93,8 +93,24 @@ struct phde_ctx {
   // reduction scratch
   double *d_part = nullptr;
   size_t part_elems = 0;
+  // delta-stepping scratch (traverse.hip)
+  uint32_t *ts_far[2] = {nullptr, nullptr};
+  int32_t *ts_nstamp = nullptr, *ts_fstamp = nullptr;
+  unsigned int *ts_cnt = nullptr;
+  int sssp_grid = 0;
+  int32_t default_delta = 1;
+  // multi-source BFS scratch (traverse.hip)
+  unsigned long long *ms_visit = nullptr, *ms_fr[2] = {nullptr, nullptr};
+  int64_t *ms_src = nullptr;
+  int ms_grid = 0;
   phde::Timer timer;
 };
+
+#define PHDE_TRY_RC(x) \
+  do {                 \
+    int rc__ = (x);    \
+    if (rc__) return rc__; \
+  } while (0)
 
 namespace phde {
 

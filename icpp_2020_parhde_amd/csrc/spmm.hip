@@ -19,6 +19,7 @@
 
 #include "phde_common.h"
 #include "spmm.h"
+#include "traverse.h"
 
 namespace phde {
 
@@ -98,7 +99,7 @@ constexpr int kGramRows = 16;
 constexpr int kGramPer = 16;  // outputs per thread (s*s <= 4096)
 template <typename T>
 __global__ __launch_bounds__(kBlk) void gram_partial_kernel(const T *A, const T *B, int64_t n, int s,
-                                                           double *part) {
+                                                           double *part, const double *rw = nullptr) {
   __shared__ double ta[kGramRows][64];
   __shared__ double tb[kGramRows][64];
   const int t = threadIdx.x;
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(kBlk) void gram_partial_kernel(const T *A, const T 
     for (int e = t; e < kGramRows * 64; e += kBlk) {
       const int r = e / 64, k = e % 64;
       const bool ok = r < rows && k < s;
-      ta[r][k] = ok ? (double)A[(r0 + r) * s + k] : 0.0;
+      ta[r][k] = ok ? (double)A[(r0 + r) * s + k] * (rw ? rw[r0 + r] : 1.0) : 0.0;
       tb[r][k] = ok ? (double)B[(r0 + r) * s + k] : 0.0;
     }
     __syncthreads();
@@ -326,6 +327,34 @@ int coords_compute(phde_ctx *c, const void *Scol, bool fp32, int s, int nc, cons
                                                                                         nc, inv);
   c->timer.end(c->stream, PHDE_KERNEL_COORDS, ev);
   PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+// S^T D S for the generalised eigenproblem: the row-tiled Gram with the
+// rows of the left operand scaled by deg.
+int weighted_gram(phde_ctx *c, const void *S_dev, bool fp32, int s, double *out_dev) {
+  Lap tmp;
+  int rc = tmp.setup(c, c->g.n, s, fp32);
+  if (rc) return rc;
+  const int64_t n = c->g.n;
+  hipEvent_t ev;
+  c->timer.begin(c->stream, PHDE_KERNEL_GRAM, &ev);
+  if (fp32) {
+    col_to_row_kernel<float><<<blocks_for(c, (n + 63) / 64), kBlk, 0, c->stream>>>(
+        (const float *)S_dev, (float *)tmp.Srow, n, s);
+    gram_partial_kernel<float><<<tmp.nblk_gram, kBlk, 0, c->stream>>>(
+        (const float *)tmp.Srow, (const float *)tmp.Srow, n, s, tmp.part, c->g.degw);
+  } else {
+    col_to_row_kernel<double><<<blocks_for(c, (n + 63) / 64), kBlk, 0, c->stream>>>(
+        (const double *)S_dev, (double *)tmp.Srow, n, s);
+    gram_partial_kernel<double><<<tmp.nblk_gram, kBlk, 0, c->stream>>>(
+        (const double *)tmp.Srow, (const double *)tmp.Srow, n, s, tmp.part, c->g.degw);
+  }
+  gram_reduce_kernel<<<(s * s + 255) / 256, 256, 0, c->stream>>>(tmp.part, tmp.nblk_gram, s * s,
+                                                                  out_dev);
+  c->timer.end(c->stream, PHDE_KERNEL_GRAM, ev);
+  PHDE_HIP(hipGetLastError());
+  PHDE_HIP(hipStreamSynchronize(c->stream));
   return PHDE_OK;
 }
 

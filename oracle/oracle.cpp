@@ -233,6 +233,51 @@ void finish_layout(const or_graph *g, const double *deg, const double *S, int s,
   if (r->S) memcpy(r->S, S, sizeof(double) * n * s);
 }
 
+// One CGS D-orthogonalisation step of candidate An against U0 and the
+// `nacc` accepted columns of S (ParHDE.cpp:204-266).  Returns ||An|| after
+// the update (An is updated in place).
+double cgs_step(int64_t n, int nacc, const double *S, const double *U0, double *An,
+                const double *degrees) {
+  const int T = omp_get_max_threads();
+  std::vector<double> part_num((size_t)T * 65, 0.0), part_den((size_t)T * 65, 0.0);
+#pragma omp parallel
+  {
+    const int t = omp_get_thread_num();
+    double num_l[65], den_l[65];
+    for (int k = 0; k < 65; k++) num_l[k] = den_l[k] = 0;
+#pragma omp for schedule(static)
+    for (int64_t ii = 0; ii < n; ii++) {
+      const double a = An[ii];
+      for (int k = 0; k < nacc; k++) {
+        num_l[k] += a * S[ii + n * k] * degrees[ii];
+        den_l[k] += S[ii + n * k] * S[ii + n * k] * degrees[ii];
+      }
+      num_l[64] += a * U0[ii] * degrees[ii];
+      den_l[64] += U0[ii] * U0[ii] * degrees[ii];
+    }
+    for (int k = 0; k < 65; k++) {
+      part_num[(size_t)t * 65 + k] = num_l[k];
+      part_den[(size_t)t * 65 + k] = den_l[k];
+    }
+  }
+  double num[65] = {0}, den[65] = {0}, coef[65];
+  for (int t = 0; t < T; t++)
+    for (int k = 0; k < 65; k++) {
+      num[k] += part_num[(size_t)t * 65 + k];
+      den[k] += part_den[(size_t)t * 65 + k];
+    }
+  for (int k = 0; k < nacc; k++) coef[k] = num[k] * (1 / den[k]);
+  coef[64] = num[64] * (1 / den[64]);
+#pragma omp parallel for schedule(static)
+  for (int64_t ii = 0; ii < n; ii++) {
+    double acc = 0;
+    for (int k = 0; k < nacc; k++) acc += coef[k] * S[ii + n * k];
+    acc += coef[64] * U0[ii];
+    An[ii] -= acc;
+  }
+  return std::sqrt(par_sum(n, [&](int64_t i) { return An[i] * An[i]; }));
+}
+
 }  // namespace
 
 extern "C" {
@@ -543,7 +588,6 @@ int or_parhde(const or_graph *g, int s, int numCoord, int lazy, or_result *r) {
   }
   int64_t start_idx = 0;
   std::set<int64_t> sources;
-  const int T = omp_get_max_threads();
   // Per-column k-centers step + L2 normalisation of the new column.
   auto traverse = [&](int run_count, double *col) {
     if (r->pivots) r->pivots[run_count - 1] = start_idx;
@@ -565,48 +609,9 @@ int or_parhde(const or_graph *g, int s, int numCoord, int lazy, or_result *r) {
   };
   int correct_pivots = 1;
   // CGS D-orthogonalisation of one candidate (ParHDE.cpp:193-277).
-  std::vector<double> part_num((size_t)T * 65), part_den((size_t)T * 65);
   auto cgs = [&](int run_count, double *An) {
     const int j = correct_pivots;
-    const double *U0 = U.data();
-    std::fill(part_num.begin(), part_num.end(), 0.0);
-    std::fill(part_den.begin(), part_den.end(), 0.0);
-#pragma omp parallel
-    {
-      const int t = omp_get_thread_num();
-      double num_l[65], den_l[65];
-      for (int k = 0; k < 65; k++) num_l[k] = den_l[k] = 0;
-#pragma omp for schedule(static)
-      for (int64_t ii = 0; ii < n; ii++) {
-        const double a = An[ii];
-        for (int k = 0; k < j - 1; k++) {
-          num_l[k] += a * S[ii + n * k] * degrees[ii];
-          den_l[k] += S[ii + n * k] * S[ii + n * k] * degrees[ii];
-        }
-        num_l[64] += a * U0[ii] * degrees[ii];
-        den_l[64] += U0[ii] * U0[ii] * degrees[ii];
-      }
-      for (int k = 0; k < 65; k++) {
-        part_num[(size_t)t * 65 + k] = num_l[k];
-        part_den[(size_t)t * 65 + k] = den_l[k];
-      }
-    }
-    double num[65] = {0}, den[65] = {0}, coef[65];
-    for (int t = 0; t < T; t++)
-      for (int k = 0; k < 65; k++) {
-        num[k] += part_num[(size_t)t * 65 + k];
-        den[k] += part_den[(size_t)t * 65 + k];
-      }
-    for (int k = 0; k < j - 1; k++) coef[k] = num[k] * (1 / den[k]);
-    coef[64] = num[64] * (1 / den[64]);
-#pragma omp parallel for schedule(static)
-    for (int64_t ii = 0; ii < n; ii++) {
-      double acc = 0;
-      for (int k = 0; k < j - 1; k++) acc += coef[k] * S[ii + n * k];
-      acc += coef[64] * U0[ii];
-      An[ii] -= acc;
-    }
-    double nd = std::sqrt(par_sum(n, [&](int64_t i) { return An[i] * An[i]; }));
+    double nd = cgs_step(n, j - 1, S.data(), U.data(), An, degrees.data());
     if (r->normdist) r->normdist[run_count - 1] = nd;
     if (nd < 0.001) {
       std::cout << "discarding vec " << j << ", normdist " << nd << std::endl;
@@ -858,6 +863,95 @@ int or_phde(const or_graph *g, int s, int numCoord, or_result *r) {
   r->t_orth = secs(t0, clk::now());
   r->n_accepted = s;
   return finish_mds(g, s, numCoord, C.data(), r, t_start);
+}
+
+int or_parhde_sample(const or_graph *g, int s, int numCoord, int nbfs, or_sample *out) {
+  if (!g || !out || s < 2 || s > 64 || nbfs < 1 || g->w) return OR_ERR_ARG;
+  const int64_t n = g->n;
+  auto t_begin = clk::now();
+  std::vector<double> degrees(n);
+  for (int64_t i = 0; i < n; i++) degrees[i] = (double)(g->rowptr[i + 1] - g->rowptr[i]);
+  // traversals, exactly as or_parhde's k-centers loop
+  std::vector<double> U0(n, 1.0), col(n), min_dists(n, (double)INT_MAX);
+  {
+    const double nrm = 1 / std::sqrt(par_sum(n, [&](int64_t i) { return U0[i] * U0[i]; }));
+    for (int64_t p = 0; p < n; p++) U0[p] *= nrm;
+  }
+  int64_t src = 0;
+  std::set<int64_t> sources;
+  auto t0 = clk::now();
+  for (int k = 0; k < nbfs; k++) {
+    distance_column(g, src, col.data(), 0);
+    double maxval;
+    int64_t idx;
+    kcenter_argmax(n, col.data(), min_dists.data(), &maxval, &idx);
+    const int64_t old = idx;
+    if (k) idx = kcenter_dedupe(n, min_dists.data(), maxval, idx, sources);
+    sources.insert(old);
+    src = idx;
+    double nd = 0;
+    for (int64_t p = 0; p < n; p++) nd += col[p] * col[p];
+    nd = 1 / std::sqrt(nd);
+    for (int64_t p = 0; p < n; p++) col[p] *= nd;
+  }
+  out->t_bfs_each = secs(t0, clk::now()) / nbfs;
+  out->n_bfs_timed = nbfs;
+  // stand-in columns: shifted copies of the last distance column (the cost
+  // of the dense stages does not depend on the values)
+  std::vector<double> S((size_t)n * s), An(n);
+#pragma omp parallel for schedule(static)
+  for (int64_t p = 0; p < n; p++)
+    for (int k = 0; k < s; k++) S[p + (size_t)k * n] = col[(p + 7919 * (int64_t)k) % n] + 1e-3 * k;
+  // CGS at three sizes, least-squares line over j, summed over j = 1..s
+  const int js[3] = {1, (s + 1) / 2, s};
+  double tj[3];
+  for (int q = 0; q < 3; q++) {
+    std::copy(col.begin(), col.end(), An.begin());
+    auto a = clk::now();
+    (void)cgs_step(n, js[q] - 1, S.data(), U0.data(), An.data(), degrees.data());
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; p++) An[p] = An[p] * 0.5;  // the accept-and-scale pass
+    tj[q] = secs(a, clk::now());
+  }
+  {
+    double mx = 0, my = 0;
+    for (int q = 0; q < 3; q++) {
+      mx += js[q];
+      my += tj[q];
+    }
+    mx /= 3;
+    my /= 3;
+    double sxy = 0, sxx = 0;
+    for (int q = 0; q < 3; q++) {
+      sxy += (js[q] - mx) * (tj[q] - my);
+      sxx += (js[q] - mx) * (js[q] - mx);
+    }
+    const double slope = sxx > 0 ? sxy / sxx : 0, icpt = my - slope * mx;
+    double tot = 0;
+    for (int j = 1; j <= s; j++) tot += std::max(0.0, icpt + slope * j);
+    out->t_orth_total = tot;
+  }
+  auto t2 = clk::now();
+  std::vector<double> LS((size_t)n * s), SLS((size_t)s * s);
+  laplacian_mult(g, degrees.data(), S.data(), LS.data(), s);
+  mat_trans_mult(S.data(), LS.data(), SLS.data(), s, n, s);
+  out->t_mmult = secs(t2, clk::now());
+  // power method on a symmetric s x s matrix with the same scale
+  std::vector<double> K((size_t)s * s), ev(s);
+  for (int i = 0; i < s; i++)
+    for (int k = 0; k < s; k++) K[k + i * s] = 0.5 * (SLS[k + i * s] + SLS[i + k * s]);
+  auto t3 = clk::now();
+  int32_t iters = 0;
+  std::vector<double> eig((size_t)s * numCoord);
+  (void)or_compute_eigens(K.data(), s, numCoord, 0, eig.data(), ev.data(), &iters);
+  out->t_eig = secs(t3, clk::now());
+  auto t4 = clk::now();
+  std::vector<double> X((size_t)n * numCoord);
+  compute_coords(n, s, numCoord, S.data(), eig.data(), X.data());
+  normalise_coords(n, numCoord, X.data());
+  out->t_coords = secs(t4, clk::now());
+  out->t_spent = secs(t_begin, clk::now());
+  return OR_OK;
 }
 
 double or_hall_energy(const or_graph *g, const double *Y, int c) {

@@ -73,6 +73,24 @@ int or_wparhde(const or_graph *g, int s, int numCoord, int32_t delta, or_result 
 int or_pivotmds(const or_graph *g, int s, int numCoord, or_result *r);
 int or_phde(const or_graph *g, int s, int numCoord, or_result *r);
 
+/* Bounded CPU-baseline sample of one ParHDE layout on the full graph:
+ * `nbfs` k-centers traversals are timed (each with its argmax and column
+ * normalisation, the reference's "BFS mean time" work), the CGS step is
+ * timed at three numbers of accepted columns and interpolated linearly over
+ * the s candidates, and L*S + S^T(LS), the power method and the
+ * coordinates are timed once at full size on stand-in columns (their cost
+ * does not depend on the values).  Seconds. */
+typedef struct or_sample {
+  double t_bfs_each;   /* one k-centers traversal */
+  double t_orth_total; /* CGS over s accepted candidates */
+  double t_mmult;      /* L*S and S^T(LS) */
+  double t_eig;        /* power method */
+  double t_coords;     /* S*V + normalisation */
+  double t_spent;      /* CPU wall time the sample took */
+  int32_t n_bfs_timed;
+} or_sample;
+int or_parhde_sample(const or_graph *g, int s, int numCoord, int nbfs, or_sample *out);
+
 /* Hall energy tr(Y^T L Y) / tr(Y^T D Y) of an n*c column-major layout. */
 double or_hall_energy(const or_graph *g, const double *Y, int c);
 

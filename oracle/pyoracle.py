@@ -197,3 +197,42 @@ def num_threads():
 
 def set_threads(k: int):
     os.environ["OMP_NUM_THREADS"] = str(k)
+
+
+class OrSample(ctypes.Structure):
+    _fields_ = [("t_bfs_each", ctypes.c_double), ("t_orth_total", ctypes.c_double),
+                ("t_mmult", ctypes.c_double), ("t_eig", ctypes.c_double), ("t_coords", ctypes.c_double),
+                ("t_spent", ctypes.c_double), ("n_bfs_timed", ctypes.c_int32)]
+
+
+def parhde_sample(g, s, nbfs=6, c=2, discards=0):
+    """CPU baseline of one ParHDE layout on graph g from a bounded sample.
+
+    Returns the cpu_baseline record: value = estimated ms per layout with
+    the reference's eager 3s traversals; also the lazy (s + discards)
+    estimate and the measured phase times."""
+    L = lib()
+    L.or_parhde_sample.argtypes = [ctypes.POINTER(OrGraph), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(OrSample)]
+    L.or_parhde_sample.restype = ctypes.c_int
+    og = graph(g)
+    out = OrSample()
+    rc = L.or_parhde_sample(ctypes.byref(og), s, c, nbfs, ctypes.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"or_parhde_sample failed: {rc}")
+    rest = out.t_orth_total + out.t_mmult + out.t_eig + out.t_coords
+    eager = 3 * s * out.t_bfs_each + rest
+    lazy = (s + discards) * out.t_bfs_each + rest
+    return {
+        "value": round(eager * 1e3, 1),
+        "unit": "ms/layout",
+        "cores": num_threads(),
+        "kind": "port",
+        "sample": (f"full graph; {out.n_bfs_timed} of the {3 * s} k-centers BFS timed (x{3 * s}), "
+                   f"CGS timed at 3 column counts and summed over s={s}, L*S + S^T(LS), power method "
+                   f"and coordinates timed once at full size; {out.t_spent:.1f}s of CPU work"),
+        "lazy_value": round(lazy * 1e3, 1),
+        "phases_ms": {"bfs_each": round(out.t_bfs_each * 1e3, 2), "orth": round(out.t_orth_total * 1e3, 1),
+                      "mmult": round(out.t_mmult * 1e3, 1), "eig": round(out.t_eig * 1e3, 2),
+                      "coords": round(out.t_coords * 1e3, 1)},
+    }
