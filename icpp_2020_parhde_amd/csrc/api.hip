@@ -161,6 +161,9 @@ static void ctx_free(phde_ctx *c) {
   for (void *p : dev)
     if (p) (void)hipFree(p);
   if (c->h_kc) (void)hipHostFree(c->h_kc);
+  if (c->d_bar) (void)hipFree(c->d_bar);
+  if (c->d_dummy) (void)hipFree(c->d_dummy);
+  if (c->h_bar) (void)hipHostFree(c->h_bar);
   if (c->h_small) (void)hipHostFree(c->h_small);
   traverse_release(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -1,6 +1,6 @@
 // bfs.hip — direction-optimising breadth-first search for gfx950, one
-// persistent cooperative kernel per traversal, with the k-centers update
-// fused into its epilogue.
+// kernel launch per level, with the k-centers update fused into the last
+// level kernel.
 //
 // Reference behaviour followed: DOBFS (gapbs/bfs.cc:144-185).  Levels are
 // exact hop counts and do not depend on scheduling; the top-down /
@@ -8,25 +8,23 @@
 // 172-173), which only affects speed.  The epilogue restates the k-centers
 // step of ParHDE.cpp:124-140 (min over columns, first maximiser).
 //
-// Design (MI355X): all blocks stay resident (cooperative launch) and meet
-// at a grid barrier once per level, so a 2000-level traversal costs 2000
-// barriers instead of 2000 kernel launches plus host round trips.  The
-// frontier is a vertex queue in HBM for top-down levels; bottom-up levels
-// test frontier membership in a bitmap that the previous bottom-up level
-// wrote with one 64-bit ballot store per wave (no atomics, no clearing).
-// The first bottom-up level after a top-down level tests `level == L`
-// directly on the distance array, so top-down levels never touch a
-// bitmap.  Queue appends are wave-aggregated (one atomic per wave).
-// Vertices of degree >= kHugeDegree are expanded by the whole grid, those
-// of degree >= kBigDegree by one wave, the rest one lane each.
-#include <hip/hip_cooperative_groups.h>
-
+// Design (MI355X): level kernels are enqueued ahead by the host
+// (levels.h); each reads the frontier counts of its level (complete at the
+// kernel boundary), evaluates the direction rule redundantly in every
+// thread, and processes the level.  Block 0 carries the rule's state to the
+// next level through a parity-indexed slot.  The frontier is a vertex queue
+// for top-down levels; bottom-up levels test membership in a bitmap that the
+// previous bottom-up level wrote with one 64-bit ballot store per wave (no
+// atomics, no clearing); the first bottom-up level after a top-down one tests
+// `level == L` on the distance array instead, so top-down levels never touch
+// a bitmap.  Queue appends are wave-aggregated (one atomic per wave).
+// Vertices of degree >= kHugeDegree are expanded by the whole grid, those of
+// degree >= kBigDegree by one wave, the rest one lane each.
 #include <algorithm>
 #include <cstdio>
 
+#include "levels.h"
 #include "phde_common.h"
-
-namespace cg = cooperative_groups;
 
 namespace phde {
 
@@ -51,22 +49,94 @@ __device__ __forceinline__ int wave_max_i32(int v) {
   return v;
 }
 
-__device__ __forceinline__ unsigned int atomic_read_u32(unsigned int *p) {
+// Frontier queue entry: the vertex and its adjacency range, so the next
+// level reads one 16-byte entry instead of entry + row pointers.
+struct QEnt {
+  uint32_t v, b, e, pad;
+};
+
+// Data written by one block and read by another within the persistent
+// kernel go through device-scope (agent) relaxed atomic loads and stores:
+// those are coherent across the XCDs' L2 caches without the L2 write-back /
+// invalidate that an agent-scope fence costs, so the read-only graph stays
+// cached from level to level.
+__device__ __forceinline__ int32_t ld_dev(const int32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ unsigned long long atomic_read_u64(unsigned long long *p) {
+__device__ __forceinline__ uint32_t ld_dev(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_dev(const unsigned long long *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(int32_t *p, int32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(unsigned int *p, unsigned int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(unsigned long long *p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ QEnt ld_qent(const QEnt *q) {
+  const unsigned long long *p = reinterpret_cast<const unsigned long long *>(q);
+  const unsigned long long x = ld_dev(p), y = ld_dev(p + 1);
+  return QEnt{(uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, 0u};
+}
+__device__ __forceinline__ void st_qent(QEnt *q, QEnt v) {
+  unsigned long long *p = reinterpret_cast<unsigned long long *>(q);
+  st_dev(p, (unsigned long long)v.v | ((unsigned long long)v.b << 32));
+  st_dev(p + 1, (unsigned long long)v.e);
 }
 
-// Append the lanes' `found` vertices to the next frontier (queue, huge list,
-// degree sum).  Must be called by all 64 lanes of the wave.
-__device__ __forceinline__ void wave_append(bool found, uint32_t v, const int32_t *deg32,
-                                            uint32_t *q_next, uint32_t *h_next, BfsSlot *slot) {
+// Grid barrier for co-resident blocks (cooperative launch): every block
+// adds one to a monotone arrival counter and waits until it reaches
+// `target` (= blocks x barriers so far).  All of this thread's memory
+// operations complete before the arrival (s_waitcnt), and every
+// cross-block datum is accessed with device-scope operations, so no cache
+// maintenance is needed.  The wait is bounded: on timeout the block flags
+// an error and continues, so the kernel always ends.
+__device__ __forceinline__ void grid_barrier(unsigned int *arrive, unsigned int target,
+                                             unsigned int *err, int backoff) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    long spins = 0;
+    while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      // polling every block on one word congests its memory channel and
+      // delays unrelated loads: back off between polls
+      for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(2);
+      if (++spins > (1L << 25)) {
+        atomicOr(err, 1u);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int wave_excl_scan(int x, int *total) {
+  const int lane = __lane_id();
+  int incl = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  *total = __shfl(incl, 63);
+  return incl - x;
+}
+
+// Append the lanes' `found` vertices (with row ranges) to the next frontier
+// (queue, huge list, degree sum).  Must be called by all 64 lanes.
+__device__ __forceinline__ void wave_append(bool found, uint32_t v, uint32_t b, uint32_t e,
+                                            QEnt *q_next, uint32_t *h_next, BfsSlot *slot) {
   const unsigned long long mask = __ballot(found);
   if (mask == 0) return;
   const int lane = __lane_id();
   const int leader = __ffsll((long long)mask) - 1;
-  const int dv = found ? deg32[v] : 0;
+  const int dv = found ? (int)(e - b) : 0;
   const bool huge = found && dv >= kHugeDegree;
   const unsigned long long hmask = __ballot(huge);
   const unsigned long long dsum = wave_sum_u64((unsigned long long)dv);
@@ -79,196 +149,88 @@ __device__ __forceinline__ void wave_append(bool found, uint32_t v, const int32_
   base = __shfl(base, leader);
   hbase = __shfl(hbase, leader);
   const unsigned long long lt = lanemask_lt();
-  if (found) q_next[base + __popcll(mask & lt)] = v;
-  if (huge) h_next[hbase + __popcll(hmask & lt)] = v;
+  if (found) st_qent(&q_next[base + __popcll(mask & lt)], QEnt{v, b, e, 0u});
+  if (huge) st_dev(&h_next[hbase + __popcll(hmask & lt)], v);
 }
 
-__device__ __forceinline__ bool discover(int32_t *dist, uint32_t v, int32_t next_level) {
-  if (dist[v] != kUnvisited) return false;
-  return atomicCAS(&dist[v], kUnvisited, next_level) == kUnvisited;
+// Claim v for the next level; on success also fetch its row range.
+__device__ __forceinline__ bool discover(int32_t *dist, const uint32_t *rowptr, uint32_t v,
+                                         int32_t next_level, uint32_t *b, uint32_t *e) {
+  if (dist[v] != kUnvisited) return false;  // plain load: see the small-list path
+  if (atomicCAS(&dist[v], kUnvisited, next_level) != kUnvisited) return false;
+  *b = rowptr[v];
+  *e = rowptr[v + 1];
+  return true;
 }
+
+constexpr int kGroup = 16;  // neighbours a lane handles per round trip
+
+// Direction-rule state carried from level to level (bfs.cc:159-183).
+struct DirState {
+  int in_bu, prev_bu, after_bu, pad;
+  long long edges_to_check, awake_old;
+};
 
 struct BfsArgs {
   DevGraph g;
   int32_t *dist;
-  uint32_t *q0, *q1, *h0, *h1;
+  QEnt *q0, *q1;
+  uint32_t *h0, *h1;
   uint64_t *bm0, *bm1;
-  BfsSlot *slots;  // 3
+  BfsSlot *slots;  // 3 rotating frontier counters
+  DirState *dir;   // 2, by level parity
   int64_t src;
   int alpha, beta;
   int32_t *min_dist;  // nullable: fused k-centers update
   KcResult *kc;
+  int *done_dev, *host_flags;
+  int32_t *dummy;  // one scratch word per thread of the grid
+  int backoff;     // barrier poll back-off (units of s_sleep 2)
 };
 
-__global__ __launch_bounds__(kBfsBlock) void bfs_persistent_kernel(BfsArgs a) {
-  cg::grid_group grid = cg::this_grid();
+// Initial state: every vertex unvisited, frontier = {src}.
+__device__ void bfs_init(const BfsArgs &a) {
+  const int64_t n = a.g.n;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = gtid; i < n; i += gsize) st_dev(&a.dist[i], (i == a.src) ? 0 : kUnvisited);
+  if (gtid == 0) {
+    const int dsrc = a.g.deg32[a.src];
+    st_dev(&a.slots[0].q, 1u);
+    st_dev(&a.slots[0].scout, (unsigned long long)dsrc);
+    st_dev(&a.slots[0].huge, dsrc >= kHugeDegree ? 1u : 0u);
+    st_qent(&a.q0[0], QEnt{(uint32_t)a.src, a.g.rowptr[a.src], a.g.rowptr[a.src + 1], 0u});
+    st_dev(&a.h0[0], (unsigned int)a.src);
+    for (int s = 1; s < 3; s++) {
+      st_dev(&a.slots[s].q, 0u);
+      st_dev(&a.slots[s].huge, 0u);
+      st_dev(&a.slots[s].scout, 0ull);
+    }
+    DirState d0{};
+    d0.edges_to_check = a.g.m;
+    a.dir[0] = d0;
+    st_dev(&a.kc->key, 0ull);
+    st_dev(&a.kc->sumsq, 0ull);
+    st_dev(&a.kc->unreached, 0ull);
+    *a.done_dev = -1;
+  }
+}
+
+__global__ __launch_bounds__(kBfsBlock) void bfs_init_kernel(BfsArgs a) { bfs_init(a); }
+
+// Fused epilogue: k-centers update + argmax, sum of squares, unreached.
+__device__ void bfs_epilogue(const BfsArgs &a) {
   const int64_t n = a.g.n;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
   const int lane = __lane_id();
-  const int64_t gwave = gtid / kWave;
-  const int64_t nwaves = gsize / kWave;
-  const uint32_t *rowptr = a.g.rowptr;
-  const uint32_t *adj = a.g.adj;
-  const int32_t *deg32 = a.g.deg32;
-
-  // ---- initialisation: every vertex unvisited, frontier = {src}
-  for (int64_t i = gtid; i < n; i += gsize) a.dist[i] = (i == a.src) ? 0 : kUnvisited;
-  if (gtid == 0) {
-    const int dsrc = deg32[a.src];
-    a.slots[0].q = 1;
-    a.slots[0].scout = (unsigned long long)dsrc;
-    a.slots[0].huge = dsrc >= kHugeDegree ? 1u : 0u;
-    a.q0[0] = (uint32_t)a.src;
-    a.h0[0] = (uint32_t)a.src;
-    for (int s = 1; s < 3; s++) {
-      a.slots[s].q = 0;
-      a.slots[s].huge = 0;
-      a.slots[s].scout = 0;
-    }
-    a.kc->key = 0;
-    a.kc->sumsq = 0;
-    a.kc->unreached = 0;
-    a.kc->depth = 0;
-  }
-  grid.sync();
-
-  // Direction state machine, evaluated identically by every thread from the
-  // same global counters (bfs.cc:159-183).
-  bool in_bu = false, prev_bu = false, after_bu = false;
-  long long edges_to_check = a.g.m;
-  long long awake_old = 0;
-  int L = 0;
-  for (;; L++) {
-    BfsSlot *cur = &a.slots[L % 3];
-    BfsSlot *nxt = &a.slots[(L + 1) % 3];
-    BfsSlot *old = &a.slots[(L + 2) % 3];
-    const unsigned int nq = atomic_read_u32(&cur->q);
-    if (nq == 0) break;
-    const unsigned int nh = atomic_read_u32(&cur->huge);
-    const unsigned long long sc = atomic_read_u64(&cur->scout);
-    bool bu;
-    if (in_bu) {
-      const long long awake = nq;
-      if (awake >= awake_old || awake > n / (long long)a.beta) {
-        awake_old = awake;
-        bu = true;
-      } else {
-        // leave the bottom-up phase with scout_count = 1 (bfs.cc:174-175)
-        in_bu = false;
-        after_bu = true;
-        bu = false;
-      }
-    } else {
-      bu = false;
-    }
-    if (!in_bu) {
-      const long long scout = after_bu ? 1 : (long long)sc;
-      after_bu = false;
-      if (scout > edges_to_check / a.alpha) {
-        in_bu = true;
-        awake_old = nq;
-        bu = true;
-      } else {
-        edges_to_check -= scout;
-        bu = false;
-      }
-    }
-    if (gtid == 0) {
-      old->q = 0;
-      old->huge = 0;
-      old->scout = 0;
-    }
-    const uint32_t *q_cur = (L & 1) ? a.q1 : a.q0;
-    uint32_t *q_next = (L & 1) ? a.q0 : a.q1;
-    const uint32_t *h_cur = (L & 1) ? a.h1 : a.h0;
-    uint32_t *h_next = (L & 1) ? a.h0 : a.h1;
-    const uint64_t *bm_cur = (L & 1) ? a.bm1 : a.bm0;
-    uint64_t *bm_next = (L & 1) ? a.bm0 : a.bm1;
-    const int32_t next_level = L + 1;
-
-    if (!bu) {
-      // ---------------- top-down (bfs.cc:76-106)
-      // grid-wide expansion of huge-degree frontier vertices
-      for (unsigned int hi = 0; hi < nh; hi++) {
-        const uint32_t u = h_cur[hi];
-        const uint32_t b = rowptr[u], e = rowptr[u + 1];
-        for (int64_t k0 = (int64_t)b + gwave * kWave; k0 < (int64_t)e; k0 += nwaves * kWave) {
-          const int64_t k = k0 + lane;
-          const bool ok = k < (int64_t)e;
-          const uint32_t v = ok ? adj[k] : 0u;
-          const bool f = ok && discover(a.dist, v, next_level);
-          wave_append(f, v, deg32, q_next, h_next, nxt);
-        }
-      }
-      for (int64_t base = gwave * kWave; base < (int64_t)nq; base += nwaves * kWave) {
-        const int64_t idx = base + lane;
-        const bool act = idx < (int64_t)nq;
-        const uint32_t u = act ? q_cur[idx] : 0u;
-        uint32_t b = act ? rowptr[u] : 0u;
-        uint32_t e = act ? rowptr[u + 1] : 0u;
-        int d = (int)(e - b);
-        if (d >= kHugeDegree) d = 0;  // expanded above
-        const bool big = d >= kBigDegree;
-        unsigned long long bmask = __ballot(big);
-        while (bmask) {
-          const int l = __ffsll((long long)bmask) - 1;
-          bmask &= bmask - 1;
-          const uint32_t bb = __shfl(b, l), be = __shfl(e, l);
-          for (uint32_t k0 = bb; k0 < be; k0 += kWave) {
-            const uint32_t k = k0 + lane;
-            const bool ok = k < be;
-            const uint32_t v = ok ? adj[k] : 0u;
-            const bool f = ok && discover(a.dist, v, next_level);
-            wave_append(f, v, deg32, q_next, h_next, nxt);
-          }
-        }
-        const int ds = big ? 0 : d;
-        const int maxd = wave_max_i32(ds);
-        for (int t = 0; t < maxd; t++) {
-          const bool ok = t < ds;
-          const uint32_t v = ok ? adj[b + t] : 0u;
-          const bool f = ok && discover(a.dist, v, next_level);
-          wave_append(f, v, deg32, q_next, h_next, nxt);
-        }
-      }
-    } else {
-      // ---------------- bottom-up (bfs.cc:48-73)
-      const int64_t nwords = (n + 63) / 64;
-      for (int64_t w = gwave; w < nwords; w += nwaves) {
-        const int64_t u = w * 64 + lane;
-        bool f = false;
-        if (u < n && a.dist[u] == kUnvisited) {
-          const uint32_t b = rowptr[u], e = rowptr[u + 1];
-          for (uint32_t k = b; k < e; k++) {
-            const uint32_t v = adj[k];
-            const bool in_front =
-                prev_bu ? ((bm_cur[v >> 6] >> (v & 63)) & 1ull) != 0 : a.dist[v] == L;
-            if (in_front) {
-              f = true;
-              break;
-            }
-          }
-          if (f) a.dist[u] = next_level;
-        }
-        const unsigned long long mask = __ballot(f);
-        if (lane == 0) bm_next[w] = mask;
-        wave_append(f, (uint32_t)u, deg32, q_next, h_next, nxt);
-      }
-    }
-    prev_bu = bu;
-    grid.sync();
-  }
-
-  // ---- fused epilogue: k-centers update + argmax, sum of squares, unreached
   unsigned long long key = 0, sumsq = 0, unreached = 0;
   for (int64_t i = gtid; i < n; i += gsize) {
-    const int32_t d = a.dist[i];
-    if (d == kUnvisited) {
+    const int32_t d = ld_dev(&a.dist[i]);
+    if (d == kUnvisited)
       unreached++;
-    } else {
+    else
       sumsq += (unsigned long long)d * (unsigned long long)d;
-    }
     if (a.min_dist) {
       int32_t md = a.min_dist[i];
       if (d < md) {
@@ -287,26 +249,282 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_persistent_kernel(BfsArgs a) {
     sumsq += __shfl_xor(sumsq, o);
     unreached += __shfl_xor(unreached, o);
   }
-  __shared__ unsigned long long s_key[kBfsBlock / kWave], s_sq[kBfsBlock / kWave],
-      s_un[kBfsBlock / kWave];
-  const int wid = threadIdx.x / kWave;
   if (lane == 0) {
-    s_key[wid] = key;
-    s_sq[wid] = sumsq;
-    s_un[wid] = unreached;
+    if (a.min_dist && key) atomicMax(&a.kc->key, key);
+    if (sumsq) atomicAdd(&a.kc->sumsq, sumsq);
+    if (unreached) atomicAdd(&a.kc->unreached, unreached);
+  }
+}
+
+// Frontier counts of level L, read by one thread per block and broadcast
+// through LDS: thousands of waves loading the same counter word at once
+// right after a barrier congest its memory channel and stall every other
+// access behind them.
+struct LevelCounts {
+  unsigned int nq, nh;
+  unsigned long long sc;
+};
+__device__ __forceinline__ LevelCounts read_counts(const BfsSlot *cur) {
+  __shared__ LevelCounts sh;
+  if (threadIdx.x == 0) {
+    sh.nq = ld_dev(&cur->q);
+    sh.nh = ld_dev(&cur->huge);
+    sh.sc = ld_dev(&cur->scout);
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < kBfsBlock / kWave; w++) {
-      key = s_key[w] > key ? s_key[w] : key;
-      sumsq += s_sq[w];
-      unreached += s_un[w];
+  const LevelCounts r = sh;
+  __syncthreads();
+  return r;
+}
+
+// One BFS level.  st holds the direction-rule state on entry (the same in
+// every thread) and is advanced.  Returns false when the frontier of level
+// L is empty (the traversal is complete).
+__device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
+  const int64_t n = a.g.n;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
+  const int lane = __lane_id();
+  const int64_t gwave = gtid / kWave;
+  const int64_t nwaves = gsize / kWave;
+  const uint32_t *rowptr = a.g.rowptr;
+  const uint32_t *adj = a.g.adj;
+
+  BfsSlot *cur = &a.slots[L % 3];
+  BfsSlot *nxt = &a.slots[(L + 1) % 3];
+  BfsSlot *old = &a.slots[(L + 2) % 3];
+  const LevelCounts cnt = read_counts(cur);
+  const unsigned int nq = cnt.nq;
+  if (nq == 0) return false;
+  const unsigned int nh = cnt.nh;
+  const unsigned long long sc = cnt.sc;
+  // ---- direction rule (bfs.cc:159-183), same result in every thread
+  bool bu;
+  if (st.in_bu) {
+    const long long awake = nq;
+    if (awake >= st.awake_old || awake > n / (long long)a.beta) {
+      st.awake_old = awake;
+      bu = true;
+    } else {
+      st.in_bu = 0;  // leave with scout_count = 1 (bfs.cc:174-175)
+      st.after_bu = 1;
+      bu = false;
     }
-    if (a.min_dist) atomicMax(&a.kc->key, key);
-    atomicAdd(&a.kc->sumsq, sumsq);
-    if (unreached) atomicAdd(&a.kc->unreached, unreached);
-    if (blockIdx.x == 0) a.kc->depth = L - 1;
+  } else {
+    bu = false;
   }
+  if (!st.in_bu) {
+    const long long scout = st.after_bu ? 1 : (long long)sc;
+    st.after_bu = 0;
+    if (scout > st.edges_to_check / a.alpha) {
+      st.in_bu = 1;
+      st.awake_old = nq;
+      bu = true;
+    } else {
+      st.edges_to_check -= scout;
+      bu = false;
+    }
+  }
+  const bool prev_bu = st.prev_bu != 0;
+  st.prev_bu = bu ? 1 : 0;
+  if (gtid == 0) {
+    st_dev(&old->q, 0u);
+    st_dev(&old->huge, 0u);
+    st_dev(&old->scout, 0ull);
+  }
+  const QEnt *q_cur = (L & 1) ? a.q1 : a.q0;
+  QEnt *q_next = (L & 1) ? a.q0 : a.q1;
+  const uint32_t *h_cur = (L & 1) ? a.h1 : a.h0;
+  uint32_t *h_next = (L & 1) ? a.h0 : a.h1;
+  const uint64_t *bm_cur = (L & 1) ? a.bm1 : a.bm0;
+  uint64_t *bm_next = (L & 1) ? a.bm0 : a.bm1;
+  const int32_t next_level = L + 1;
+
+  if (!bu) {
+    // ---------------- top-down (bfs.cc:76-106)
+    for (unsigned int hi = 0; hi < nh; hi++) {
+      const uint32_t u = ld_dev(&h_cur[hi]);
+      const uint32_t b = rowptr[u], e = rowptr[u + 1];
+      for (int64_t k0 = (int64_t)b + gwave * kWave; k0 < (int64_t)e; k0 += nwaves * kWave) {
+        const int64_t k = k0 + lane;
+        const bool ok = k < (int64_t)e;
+        const uint32_t v = ok ? adj[k] : 0u;
+        uint32_t vb = 0, ve = 0;
+        const bool f = ok && discover(a.dist, rowptr, v, next_level, &vb, &ve);
+        wave_append(f, v, vb, ve, q_next, h_next, nxt);
+      }
+    }
+    for (int64_t base = gwave * kWave; base < (int64_t)nq; base += nwaves * kWave) {
+      const int64_t idx = base + lane;
+      const bool act = idx < (int64_t)nq;
+      const QEnt q = act ? ld_qent(&q_cur[idx]) : QEnt{0u, 0u, 0u, 0u};
+      int d = (int)(q.e - q.b);
+      if (d >= kHugeDegree) d = 0;  // expanded above
+      const bool big = d >= kBigDegree;
+      unsigned long long bmask = __ballot(big);
+      while (bmask) {
+        const int l = __ffsll((long long)bmask) - 1;
+        bmask &= bmask - 1;
+        const uint32_t bb = __shfl(q.b, l), be = __shfl(q.e, l);
+        for (uint32_t k0 = bb; k0 < be; k0 += kWave) {
+          const uint32_t k = k0 + lane;
+          const bool ok = k < be;
+          const uint32_t v = ok ? adj[k] : 0u;
+          uint32_t vb = 0, ve = 0;
+          const bool f = ok && discover(a.dist, rowptr, v, next_level, &vb, &ve);
+          wave_append(f, v, vb, ve, q_next, h_next, nxt);
+        }
+      }
+      // Small lists: each lane issues the loads, checks and claims of up to
+      // kGroup neighbours at once, then the wave appends all winners with
+      // one atomic.
+      const int ds = big ? 0 : d;
+      const int maxd = wave_max_i32(ds);
+      for (int g0 = 0; g0 < maxd; g0 += kGroup) {
+        uint32_t vv[kGroup], vb[kGroup], ve[kGroup];
+        bool c[kGroup];
+#pragma unroll
+        for (int j = 0; j < kGroup; j++) {
+          c[j] = g0 + j < ds;
+          vv[j] = c[j] ? adj[q.b + g0 + j] : 0u;
+        }
+        // Visited pre-check with plain (cacheable) loads, all issued before
+        // any comparison: a stale line can only still read "unvisited"
+        // (levels go from kUnvisited to a value once per traversal, and
+        // each traversal starts in a new kernel), and the CAS below decides.
+        int32_t dd[kGroup];
+#pragma unroll
+        for (int j = 0; j < kGroup; j++) dd[j] = a.dist[c[j] ? vv[j] : 0u];
+#pragma unroll
+        for (int j = 0; j < kGroup; j++) c[j] = c[j] && dd[j] == kUnvisited;
+        // Claims, all issued before any result is used: lanes without a
+        // candidate aim at a private dummy word, so the atomics need no
+        // branch (a predicated atomic makes the compiler wait for each result
+        // before issuing the next).  min(level) claims v exactly once: only
+        // the first caller sees kUnvisited.
+        int32_t won[kGroup];
+        int32_t *dummy = a.dummy + gtid;
+#pragma unroll
+        for (int j = 0; j < kGroup; j++)
+          won[j] = atomicMin(c[j] ? &a.dist[vv[j]] : dummy, next_level);
+#pragma unroll
+        for (int j = 0; j < kGroup; j++) {
+          const uint32_t r = c[j] ? vv[j] : 0u;
+          vb[j] = rowptr[r];
+          ve[j] = rowptr[r + 1];
+        }
+#pragma unroll
+        for (int j = 0; j < kGroup; j++) c[j] = c[j] && won[j] == kUnvisited;
+        int cnt = 0, hcnt = 0;
+        unsigned long long dsum = 0;
+#pragma unroll
+        for (int j = 0; j < kGroup; j++) {
+          cnt += c[j] ? 1 : 0;
+          const uint32_t dv = c[j] ? ve[j] - vb[j] : 0u;
+          dsum += dv;
+          hcnt += (c[j] && dv >= (uint32_t)kHugeDegree) ? 1 : 0;
+        }
+        int total = 0;
+        const int off = wave_excl_scan(cnt, &total);
+        if (total == 0) continue;
+        unsigned int qb = 0;
+        const unsigned long long tsum = wave_sum_u64(dsum);
+        if (lane == 0) {
+          qb = atomicAdd(&nxt->q, (unsigned int)total);
+          atomicAdd(&nxt->scout, tsum);
+        }
+        qb = __shfl(qb, 0);
+        unsigned int pos = qb + (unsigned int)off;
+#pragma unroll
+        for (int j = 0; j < kGroup; j++)
+          if (c[j]) st_qent(&q_next[pos++], QEnt{vv[j], vb[j], ve[j], 0u});
+        if (__ballot(hcnt > 0)) {
+#pragma unroll
+          for (int j = 0; j < kGroup; j++)
+            if (c[j] && ve[j] - vb[j] >= (uint32_t)kHugeDegree)
+              st_dev(&h_next[atomicAdd(&nxt->huge, 1u)], vv[j]);
+        }
+      }
+    }
+  } else {
+    // ---------------- bottom-up (bfs.cc:48-73)
+    const int64_t nwords = (n + 63) / 64;
+    for (int64_t w = gwave; w < nwords; w += nwaves) {
+      const int64_t u = w * 64 + lane;
+      bool f = false;
+      uint32_t ub = 0, ue = 0;
+      if (u < n && ld_dev(&a.dist[u]) == kUnvisited) {
+        const uint32_t b = rowptr[u], e = rowptr[u + 1];
+        for (uint32_t k = b; k < e; k++) {
+          const uint32_t v = adj[k];
+          const bool in_front = prev_bu
+                                    ? ((ld_dev((const unsigned long long *)&bm_cur[v >> 6]) >> (v & 63)) & 1ull) != 0
+                                    : ld_dev(&a.dist[v]) == L;
+          if (in_front) {
+            f = true;
+            break;
+          }
+        }
+        if (f) st_dev(&a.dist[u], next_level);
+        ub = b;
+        ue = e;
+      }
+      const unsigned long long mask = __ballot(f);
+      if (lane == 0) st_dev((unsigned long long *)&bm_next[w], mask);
+      wave_append(f, (uint32_t)u, ub, ue, q_next, h_next, nxt);
+    }
+  }
+  return true;
+}
+
+// One launch per level (the host enqueues them ahead, levels.h); the
+// direction state travels between launches through a parity slot.
+__global__ __launch_bounds__(kBfsBlock) void bfs_level_kernel(BfsArgs a, int L) {
+  __shared__ int finished;
+  __shared__ DirState st_sh;
+  if (threadIdx.x == 0) {
+    finished = level_finished(a.done_dev, L);
+    st_sh = a.dir[L & 1];
+  }
+  __syncthreads();
+  if (finished) return;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  DirState st = st_sh;
+  if (!bfs_step(a, L, st)) {
+    if (gtid == 0) {
+      level_done(a.done_dev, a.host_flags, L - 1);
+      a.kc->depth = L - 1;
+    }
+    bfs_epilogue(a);
+    return;
+  }
+  if (gtid == 0) {
+    level_publish(a.host_flags, L);
+    a.dir[(L + 1) & 1] = st;
+  }
+}
+
+// The whole traversal in one cooperative launch: levels separated by the
+// grid barrier above.  *arrive must be 0 at launch; *err receives 1 if a
+// barrier wait timed out.
+__global__ __launch_bounds__(kBfsBlock) void bfs_persistent_kernel(BfsArgs a, unsigned int *arrive,
+                                                                   unsigned int *err) {
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned int nb = gridDim.x;
+  unsigned int bar = 0;
+  bfs_init(a);
+  grid_barrier(arrive, nb * ++bar, err, a.backoff);
+  DirState st{};
+  st.edges_to_check = a.g.m;
+  int L = 0;
+  const int64_t maxL = a.g.n + 1;
+  for (; L <= maxL; L++) {
+    if (!bfs_step(a, L, st)) break;
+    grid_barrier(arrive, nb * ++bar, err, a.backoff);
+    if (ld_dev(err)) break;
+  }
+  if (gtid == 0) a.kc->depth = L - 1;
+  bfs_epilogue(a);
 }
 
 __global__ void dist_finalize_kernel(const int32_t *in, int32_t *out, int64_t n, int32_t from,
@@ -318,31 +536,47 @@ __global__ void dist_finalize_kernel(const int32_t *in, int32_t *out, int64_t n,
   }
 }
 
+int flags_setup(phde_ctx *c) {
+  if (c->h_flags) return PHDE_OK;
+  PHDE_HIP(hipHostMalloc(&c->h_flags, sizeof(int) * 16, hipHostMallocMapped));
+  PHDE_HIP(hipHostGetDevicePointer((void **)&c->d_flags, c->h_flags, 0));
+  PHDE_HIP(hipMalloc(&c->d_done, sizeof(int) * 4));
+  return PHDE_OK;
+}
+
 int bfs_setup(phde_ctx *c) {
   if (c->bfs_grid > 0) return PHDE_OK;
-  int per_cu = 0;
-  PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bfs_persistent_kernel,
-                                                         kBfsBlock, 0));
-  if (per_cu < 1) {
-    set_error("bfs kernel cannot be resident");
-    return PHDE_ERR_DEVICE;
-  }
-  int want = 4;
-  if (const char *e = getenv("PHDE_BFS_BLOCKS_PER_CU")) want = std::max(1, atoi(e));
-  c->bfs_grid = c->num_cus * std::min(per_cu, want);
+  // one 256-thread block per CU: measured fastest for the persistent kernel
+  // (fewer barrier participants); the per-level launches use two
+  int per_cu = 1;
+  if (const char *e = getenv("PHDE_BFS_BLOCKS_PER_CU")) per_cu = std::max(1, atoi(e));
+  int resident = 0;
+  PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, bfs_persistent_kernel, kBfsBlock, 0));
+  c->bfs_persistent = resident >= 1;
+  if (const char *e = getenv("PHDE_BFS_PERSISTENT")) c->bfs_persistent = atoi(e) != 0 && resident >= 1;
+  if (c->bfs_persistent)
+    per_cu = std::min(per_cu, resident);
+  else if (!getenv("PHDE_BFS_BLOCKS_PER_CU"))
+    per_cu = 2;
+  c->bar_backoff = 1;
+  if (const char *e = getenv("PHDE_BAR_BACKOFF")) c->bar_backoff = std::max(0, atoi(e));
+  c->bfs_grid = std::max(1, c->num_cus) * per_cu;
   const int64_t n = c->g.n;
   const int64_t nwords = (n + 63) / 64;
   const int64_t hcap = c->g.m / kHugeDegree + 2;
   PHDE_HIP(hipMalloc(&c->d_dist, sizeof(int32_t) * n));
   for (int i = 0; i < 2; i++) {
-    PHDE_HIP(hipMalloc(&c->d_q[i], sizeof(uint32_t) * (n + 1)));
+    PHDE_HIP(hipMalloc(&c->d_q[i], sizeof(QEnt) * (n + 1)));
     PHDE_HIP(hipMalloc(&c->d_hq[i], sizeof(uint32_t) * hcap));
     PHDE_HIP(hipMalloc(&c->d_bm[i], sizeof(uint64_t) * nwords));
   }
-  PHDE_HIP(hipMalloc(&c->d_slots, sizeof(BfsSlot) * 3));
+  PHDE_HIP(hipMalloc(&c->d_slots, sizeof(BfsSlot) * 3 + sizeof(DirState) * 2));
   PHDE_HIP(hipMalloc(&c->d_kc, sizeof(KcResult)));
   PHDE_HIP(hipHostMalloc(&c->h_kc, sizeof(KcResult), hipHostMallocDefault));
-  return PHDE_OK;
+  PHDE_HIP(hipMalloc(&c->d_bar, sizeof(unsigned int) * 64));
+  PHDE_HIP(hipMalloc(&c->d_dummy, sizeof(int32_t) * (size_t)c->bfs_grid * kBfsBlock));
+  PHDE_HIP(hipHostMalloc(&c->h_bar, sizeof(unsigned int) * 4, hipHostMallocDefault));
+  return flags_setup(c);
 }
 
 int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, int32_t *min_dist,
@@ -356,27 +590,55 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
   BfsArgs a;
   a.g = c->g;
   a.dist = dist_out ? dist_out : c->d_dist;
-  a.q0 = c->d_q[0];
-  a.q1 = c->d_q[1];
+  a.q0 = (QEnt *)c->d_q[0];
+  a.q1 = (QEnt *)c->d_q[1];
   a.h0 = c->d_hq[0];
   a.h1 = c->d_hq[1];
   a.bm0 = c->d_bm[0];
   a.bm1 = c->d_bm[1];
   a.slots = c->d_slots;
+  a.dir = (DirState *)(c->d_slots + 3);
   a.src = src;
   a.alpha = alpha > 0 ? alpha : 15;
   a.beta = beta > 0 ? beta : 18;
   a.min_dist = min_dist;
   a.kc = c->d_kc;
-  void *args[] = {&a};
+  a.done_dev = c->d_done;
+  a.host_flags = c->d_flags;
+  a.dummy = c->d_dummy;
+  a.backoff = c->bar_backoff;
+  c->h_flags[kFlagDone] = -1;
+  c->h_flags[kFlagProgress] = -1;
   hipEvent_t ev = nullptr;
   c->timer.begin(c->stream, PHDE_KERNEL_BFS_LEVEL, &ev);
-  PHDE_HIP(hipLaunchCooperativeKernel((const void *)bfs_persistent_kernel, dim3(c->bfs_grid),
-                                      dim3(kBfsBlock), args, 0, c->stream));
+  const int grid = c->bfs_grid;
+  if (c->bfs_persistent) {
+    PHDE_TRY_RC(fill_u32(c, c->d_bar, 2, 0u));
+    unsigned int *arrive = c->d_bar, *err = c->d_bar + 1;
+    void *args[] = {&a, &arrive, &err};
+    PHDE_HIP(hipLaunchCooperativeKernel((const void *)bfs_persistent_kernel, dim3(grid),
+                                        dim3(kBfsBlock), args, 0, c->stream));
+  } else {
+    bfs_init_kernel<<<grid, kBfsBlock, 0, c->stream>>>(a);
+    PHDE_HIP(hipGetLastError());
+    rc = run_levels(c, [&](int L) -> int {
+      bfs_level_kernel<<<grid, kBfsBlock, 0, c->stream>>>(a, L);
+      PHDE_HIP(hipGetLastError());
+      return PHDE_OK;
+    });
+    if (rc) return rc;
+  }
   c->timer.end(c->stream, PHDE_KERNEL_BFS_LEVEL, ev);
-  if (sync_result) {
+  if (sync_result || c->bfs_persistent) {
     PHDE_HIP(hipMemcpyAsync(c->h_kc, c->d_kc, sizeof(KcResult), hipMemcpyDeviceToHost, c->stream));
+    if (c->bfs_persistent)
+      PHDE_HIP(hipMemcpyAsync(c->h_bar, c->d_bar + 1, sizeof(unsigned int), hipMemcpyDeviceToHost,
+                              c->stream));
     PHDE_HIP(hipStreamSynchronize(c->stream));
+    if (c->bfs_persistent && *c->h_bar) {
+      set_error("BFS grid barrier timed out (blocks not co-resident?)");
+      return PHDE_ERR_DEVICE;
+    }
   }
   return PHDE_OK;
 }

@@ -42,6 +42,11 @@ __global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v) {
        i += (int64_t)gridDim.x * blockDim.x)
     p[i] = v;
 }
+__global__ void fill_u32_kernel(unsigned int *p, int64_t n, unsigned int v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
 __global__ void fill_f64_kernel(double *p, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -90,16 +95,19 @@ __global__ __launch_bounds__(kOrthBlock) void cgs_dots_kernel(const double *a, c
 }
 
 // coef_k = num_k * (1 / den_k) for k < ncols, and the ones vector in slot
-// kOnesSlot (ParHDE.cpp:239-244).  One block; thread k sums its column's
-// partials in block order.
+// kOnesSlot (ParHDE.cpp:239-244).  One block per column; the partials are
+// summed in a fixed order (strided per lane, then a wave tree).
 __global__ void cgs_coef_kernel(const double *part, int nblk, int ncols, const double *den,
                                 double *coef) {
-  const int k = threadIdx.x;
-  if (k > ncols) return;
+  __shared__ double sh[kOrthBlock / kWave];
+  const int k = blockIdx.x;
   double s = 0;
-  for (int b = 0; b < nblk; b++) s += part[(int64_t)k * nblk + b];
-  const int slot = k == ncols ? kOnesSlot : k;
-  coef[slot] = s * (1 / den[slot]);
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(int64_t)k * nblk + b];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    const int slot = k == ncols ? kOnesSlot : k;
+    coef[slot] = t * (1 / den[slot]);
+  }
 }
 
 // a_i -= sum_k coef_k S_ik + coef_ones * u0, and per-block sums of a_i^2
@@ -227,6 +235,11 @@ int fill_i32(phde_ctx *c, int32_t *p, int64_t n, int32_t v) {
   PHDE_HIP(hipGetLastError());
   return PHDE_OK;
 }
+int fill_u32(phde_ctx *c, unsigned int *p, int64_t n, unsigned int v) {
+  fill_u32_kernel<<<grid_for(n, 4096), kOrthBlock, 0, c->stream>>>(p, n, v);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
 int fill_f64(phde_ctx *c, double *p, int64_t n, double v) {
   fill_f64_kernel<<<grid_for(n, 4096), kOrthBlock, 0, c->stream>>>(p, n, v);
   PHDE_HIP(hipGetLastError());
@@ -289,7 +302,7 @@ static int cgs_pass_t(Orth &o, int ncols) {
   const int nchunk = std::max(1, (ncols + kColChunk - 1) / kColChunk);
   cgs_dots_kernel<T><<<dim3(o.nblk, nchunk), kOrthBlock, 0, st>>>(o.a, o.deg_, S, o.n, ncols,
                                                                   o.u0, o.part);
-  cgs_coef_kernel<<<1, 128, 0, st>>>(o.part, o.nblk, ncols, o.den, o.coef);
+  cgs_coef_kernel<<<ncols + 1, kOrthBlock, 0, st>>>(o.part, o.nblk, ncols, o.den, o.coef);
   cgs_update_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(o.a, S, o.n, ncols, o.u0, o.coef,
                                                       o.part);
   norm_finalize_kernel<<<1, kOrthBlock, 0, st>>>(o.part, o.nblk, o.scal);

@@ -88,6 +88,13 @@ struct phde_ctx {
   phde::KcResult *d_kc = nullptr;
   phde::KcResult *h_kc = nullptr;  // pinned
   int bfs_grid = 0;
+  bool bfs_persistent = true;
+  int bar_backoff = 1;
+  unsigned int *d_bar = nullptr, *h_bar = nullptr;  // grid-barrier counter + error flag
+  int32_t *d_dummy = nullptr;                       // per-thread scratch of the BFS claims
+  // mapped host flags polled by the level driver (levels.h) and their
+  // device-side aliases; d_done is a device-memory copy for the kernels
+  int *h_flags = nullptr, *d_flags = nullptr, *d_done = nullptr;
   // generic pinned staging for small read-backs
   double *h_small = nullptr;
   // reduction scratch
@@ -97,12 +104,11 @@ struct phde_ctx {
   uint32_t *ts_far[2] = {nullptr, nullptr};
   int32_t *ts_nstamp = nullptr, *ts_fstamp = nullptr;
   unsigned int *ts_cnt = nullptr;
-  int sssp_grid = 0;
   int32_t default_delta = 1;
   // multi-source BFS scratch (traverse.hip)
-  unsigned long long *ms_visit = nullptr, *ms_fr[2] = {nullptr, nullptr};
+  unsigned long long *ms_visit = nullptr, *ms_fr[3] = {nullptr, nullptr, nullptr};
+  uint32_t *ms_q3 = nullptr;
   int64_t *ms_src = nullptr;
-  int ms_grid = 0;
   phde::Timer timer;
 };
 
@@ -128,6 +134,7 @@ int dist_finalize(phde_ctx *c, const int32_t *dist_in, int32_t *dist_out, int32_
 
 // orth.hip — D-orthogonalisation and reductions
 int fill_i32(phde_ctx *c, int32_t *p, int64_t n, int32_t v);
+int fill_u32(phde_ctx *c, unsigned int *p, int64_t n, unsigned int v);
 int fill_f64(phde_ctx *c, double *p, int64_t n, double v);
 
 }  // namespace phde
