@@ -32,6 +32,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def bfs_kernel_name() -> str:
+    """The traversal kernel the library launches: one persistent kernel per
+    traversal unless PHDE_BFS_PERSISTENT=0 selects one launch per level."""
+    return "bfs_level_kernel" if os.environ.get("PHDE_BFS_PERSISTENT", "1") == "0" else "bfs_persistent_kernel"
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload`, as read by rocprofv3
+    FETCH_SIZE + WRITE_SIZE counters (scaled by the plain-copy calibration)
+    and recorded in profiles/traffic.json; None when not measured."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    rec = json.loads(f.read_text()).get(workload, {}).get(kernel)
+    return None if rec is None else rec["bytes_per_launch"]
+
+
+def copy_calibration(mib: int) -> None:
+    """A plain device-to-device copy of `mib` MiB, three times, so that a
+    counter run can relate FETCH_SIZE / WRITE_SIZE to known bytes.  Uses the
+    HIP runtime's own copy kernel through hipMemcpy (no torch kernels)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    size = ctypes.c_size_t(mib << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), size) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(src, 1, size) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, size, 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"[bench] copy calibration: 3 x {mib} MiB device copies")
+
+
 def bfs_bytes(n: int, m: int) -> int:
     # adjacency once, row pointers once, int32 level written once per vertex
     return 4 * m + 4 * (n + 1) + 4 * n
@@ -48,6 +87,8 @@ def main():
     ap.add_argument("--eager", action="store_true")
     ap.add_argument("--extras", type=int, default=1, help="also time eager and fp32 once")
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the layouts, run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -98,6 +139,8 @@ def main():
             extras["fp32_ms_per_layout"] = round(sf.ms_total, 2)
         log(f"[bench] extras {extras}")
 
+    kernel = bfs_kernel_name()
+    workload = f"rgg{args.n >> 20}m_s{args.s}" if args.n >= 1 << 20 else f"rgg{args.n}_s{args.s}"
     record = {
         "metric": f"ParHDE ms/layout (RGG n={g.n}, s={args.s}, k-centers)",
         "value": round(ms_per_layout, 3),
@@ -111,7 +154,7 @@ def main():
         "vs_baseline": None,
         "dtype": "i32 levels / " + args.precision,
         "data": "synthetic (seeded random geometric graph)",
-        "config": {"workload": "rgg4m_s50", "n": g.n, "nnz": g.m, "s": args.s, "c": 2,
+        "config": {"workload": workload, "n": g.n, "nnz": g.m, "s": args.s, "c": 2,
                    "pivots": "kcenters", "eig": "power", "traversals": "eager" if args.eager else "lazy",
                    "precision": args.precision},
         "bfs_gteps": round(gteps, 4),
@@ -121,15 +164,18 @@ def main():
                       "mmult": round(st.ms_mmult, 2), "eig": round(st.ms_eig, 2),
                       "coords": round(st.ms_coords, 2)},
         "kernel_ms": {k: round(v, 3) for k, v in st.kernel_ms.items() if v},
-        "roofline": {"bound": "hbm", "kernel": "bfs_persistent_kernel", "achieved": round(achieved, 2),
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None},
+                     "traffic": measured_traffic(workload, kernel),
+                     "algorithmic_bytes": bfs_bytes(g.n, g.m)},
         "h2d_ms": round(h2d_s * 1e3, 1),
         **extras,
         "cpu_baseline": None,
     }
     if args.cpu_baseline:
         record["cpu_baseline"] = cpu_baseline(g, args.s)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     print(json.dumps(record), flush=True)
 
 
@@ -138,7 +184,7 @@ def cpu_baseline(g, s):
     cores, on a bounded sample; see oracle.pyoracle.parhde_sample."""
     from oracle import pyoracle as O
     t0 = time.time()
-    r = O.parhde_sample(g, s)
+    r = O.parhde_sample(g, s, nbfs=64)
     log(f"[bench] cpu baseline sample took {time.time() - t0:.1f}s: {r}")
     return r
 
