@@ -17,6 +17,7 @@ c_u32p = ctypes.POINTER(ctypes.c_uint32)
 c_i32p = ctypes.POINTER(ctypes.c_int32)
 c_i64p = ctypes.POINTER(ctypes.c_int64)
 c_f64p = ctypes.POINTER(ctypes.c_double)
+c_f32p = ctypes.POINTER(ctypes.c_float)
 
 
 class PgCsr(ctypes.Structure):
@@ -44,6 +45,10 @@ class PhdeOptions(ctypes.Structure):
         ("verbose", ctypes.c_int32),
         ("first_pivot", ctypes.c_int64),
         ("seed", ctypes.c_uint64),
+        ("judge_tol", ctypes.c_double),
+        ("nms_rho", ctypes.c_double),
+        ("judge", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -63,6 +68,11 @@ class PhdeStats(ctypes.Structure):
         ("pivots", ctypes.c_int64 * 192),
         ("kernel_ms", ctypes.c_double * 16),
         ("kernel_launches", ctypes.c_int64 * 16),
+        ("ms_select", ctypes.c_double),
+        ("hall_energy", ctypes.c_double),
+        ("hall_energy_kcenters", ctypes.c_double),
+        ("fallback", ctypes.c_int32),
+        ("n_candidates", ctypes.c_int32),
     ]
 
 
@@ -90,6 +100,24 @@ PARHDE_SYMBOLS = {
     "phde_sym_eig": (ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, ctypes.c_int, c_f64p, c_f64p]),
     "phde_hall_energy": (ctypes.c_int, [ctypes.c_void_p, c_f64p, ctypes.c_int, c_f64p]),
     "phde_last_error": (ctypes.c_char_p, []),
+    "phde_kcenters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, c_i64p, c_i32p]),
+    # learned pivot proposal
+    "phde_net_num_params": (ctypes.c_int, []),
+    "phde_net_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "phde_net_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "phde_net_get_params": (ctypes.c_int, [ctypes.c_void_p, c_f32p]),
+    "phde_net_set_params": (ctypes.c_int, [ctypes.c_void_p, c_f32p]),
+    "phde_net_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "phde_net_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "phde_set_pivot_net": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "phde_pivot_features": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_f32p, c_i64p]),
+    "phde_set_training_data": (ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p]),
+    "phde_net_scores": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_f32p]),
+    "phde_net_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, c_f64p, c_f32p]),
+    "phde_net_adam": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float]),
+    "phde_net_train_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, c_f64p]),
+    "phde_select_pivots": (ctypes.c_int, [ctypes.c_void_p, c_f32p, ctypes.c_int64, ctypes.c_int,
+                                          ctypes.c_int, c_i64p]),
 }
 
 GRAPH_SYMBOLS = {

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "eig.h"
+#include "learned.h"
 #include "orth.h"
 #include "phde_common.h"
 #include "spmm.h"
@@ -166,21 +167,9 @@ static void ctx_free(phde_ctx *c) {
   if (c->h_bar) (void)hipHostFree(c->h_bar);
   if (c->h_small) (void)hipHostFree(c->h_small);
   traverse_release(c);
+  learned_release(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
-
-// RAII device buffer for driver temporaries.
-template <typename T>
-struct DBuf {
-  T *p = nullptr;
-  int alloc(size_t count) {
-    PHDE_HIP(hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)));
-    return PHDE_OK;
-  }
-  ~DBuf() {
-    if (p) (void)hipFree(p);
-  }
-};
 
 #define PHDE_TRY(x)     \
   do {                  \
@@ -221,8 +210,9 @@ struct KCenters {
 };
 
 struct LayoutOut {
-  phde_stats *st;
-  double t_dist = 0, t_orth = 0, t_mmult = 0, t_eig = 0, t_coords = 0;
+  phde_stats *st = nullptr;
+  double t_dist = 0, t_orth = 0, t_mmult = 0, t_eig = 0, t_coords = 0, t_select = 0;
+  double energy = 0;  // Hall energy of the layout, when requested
   int ntrav = 0, nacc = 0, ndisc = 0, iters = 0;
   long long levels = 0;
   void pivot(int64_t p) {
@@ -236,7 +226,7 @@ struct LayoutOut {
 // S^T D S.
 static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, bool fp32, int s,
                          double *K_dev, const double *B_dev, bool largest, double *coords,
-                         LayoutOut &out) {
+                         LayoutOut &out, bool want_energy = false) {
   const int nc = o->num_coord;
   auto t0 = clk::now();
   DBuf<double> init, evals, work, V, evecs;
@@ -284,6 +274,13 @@ static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, 
                           c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
   out.t_coords += ms_since(t1);
+  if (want_energy) {
+    DBuf<double> hp;
+    PHDE_TRY(hp.alloc(2 * (size_t)c->num_cus * 4 + 16));
+    double h[2];
+    PHDE_TRY(hall_energy_dev(c, X.p, nc, hp.p, h));
+    out.energy = h[0] / h[1];
+  }
   return PHDE_OK;
 }
 
@@ -291,19 +288,80 @@ static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, 
 // on D^{1/2} S.
 int weighted_gram(phde_ctx *c, const void *S_dev, bool fp32, int s, double *out_dev);
 
-// ParHDE / WParHDE driver.
-static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, LayoutOut &out) {
-  const bool weighted = o->algorithm == PHDE_ALGO_WPARHDE;
+// One candidate column through CGS (ParHDE) or MGS (WParHDE), with the
+// reference's column normalisation 1/sqrt(sum d^2) (ParHDE.cpp:176-185) and
+// discard rule ||a|| < 0.001 (ParHDE.cpp:269-283); `accepted` counts the
+// columns of S so far.
+static int orth_candidate(Orth &orth, const phde_options *o, bool weighted, bool fp32,
+                          const int32_t *col, unsigned long long sumsq, int &accepted,
+                          LayoutOut &out) {
+  const double scale = 1 / std::sqrt((double)sumsq);
+  PHDE_TRY(orth.load_candidate(col, scale));
+  const int passes = fp32 ? 2 : 1;
+  for (int p = 0; p < passes; p++) {
+    if (weighted)
+      PHDE_TRY(orth.mgs_pass(accepted));
+    else
+      PHDE_TRY(orth.cgs_pass(accepted));
+  }
+  double nd = 0;
+  PHDE_TRY(orth.read_norm(&nd));
+  if (nd < 0.001) {
+    if (o->verbose) std::printf("discarding vec %d, normdist %g\n", accepted + 1, nd);
+    out.ndisc++;
+  } else {
+    PHDE_TRY(orth.accept(accepted));
+    accepted++;
+  }
+  return PHDE_OK;
+}
+
+static int not_enough_pivots(const phde_options *o, int accepted, int limit) {
+  if (o->verbose)
+    std::fprintf(stderr, "Number of Pivots obtained: %d inadequate, increase the Pivots Limits: %d\n",
+                 accepted + 1, limit);
+  set_error("only " + std::to_string(accepted) + " of " + std::to_string(o->s) +
+            " pivots are independent");
+  return PHDE_ERR_PIVOTS;
+}
+
+// L*S, S^T(LS) (+ S^T D S for the generalised mode), eigensolve, coordinates
+// (ParHDE.cpp:288-352).
+static int hde_tail(phde_ctx *c, const phde_options *o, Orth &orth, bool fp32, double *coords,
+                    LayoutOut &out, bool want_energy) {
+  const int s = o->s;
+  auto t2 = clk::now();
+  Lap lap;
+  PHDE_TRY(lap.setup(c, c->g.n, s, fp32));
+  PHDE_TRY(lap.run(orth.S));
+  DBuf<double> B;
+  if (o->eig_mode == PHDE_EIG_GENERALIZED) {
+    PHDE_TRY(B.alloc((size_t)s * s));
+    PHDE_TRY(weighted_gram(c, orth.S, fp32, s, B.p));
+  }
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  out.t_mmult += ms_since(t2);
+  return finish_layout(c, o, orth.S, fp32, s, lap.SLS, B.p, false, coords, out, want_energy);
+}
+
+static int check_weighting(phde_ctx *c, bool weighted) {
   if (weighted != (c->g.w != nullptr)) {
     set_error(weighted ? "Weighted ParHDE code requires the weighted edges"
                        : "ParHDE code doesn't requires the weighted edges");
     return PHDE_ERR_WEIGHTS;
   }
+  return PHDE_OK;
+}
+
+// ParHDE / WParHDE with k-centers pivots.
+static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, LayoutOut &out,
+                      bool want_energy = false) {
+  const bool weighted = o->algorithm == PHDE_ALGO_WPARHDE;
+  PHDE_TRY(check_weighting(c, weighted));
   const int64_t n = c->g.n;
   const int s = o->s;
   const bool fp32 = o->precision == PHDE_PREC_FP32;
   const int limit = 3 * s;
-  auto t_start = clk::now();
   Orth orth;
   PHDE_TRY(orth.setup(c, n, s, fp32));
   // ones vector U[:,0] = 1 / ||1|| (ParHDE.cpp:94-108)
@@ -337,30 +395,6 @@ static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, Layout
     const bool first = weighted ? accepted == 0 : run_count == 1;
     return kc.step(c, min_dist.p, r, !first, true, src);
   };
-  // One candidate through CGS (ParHDE) or MGS (WParHDE); returns via
-  // *acc whether it was accepted.
-  auto orthogonalise = [&](int run_count, const int32_t *col) -> int {
-    // column normalisation: 1 / sqrt(sum d^2) (ParHDE.cpp:176-185)
-    const double scale = 1 / std::sqrt((double)sumsq[run_count]);
-    PHDE_TRY(orth.load_candidate(col, scale));
-    const int passes = fp32 ? 2 : 1;
-    for (int p = 0; p < passes; p++) {
-      if (weighted)
-        PHDE_TRY(orth.mgs_pass(accepted));
-      else
-        PHDE_TRY(orth.cgs_pass(accepted));
-    }
-    double nd = 0;
-    PHDE_TRY(orth.read_norm(&nd));
-    if (nd < 0.001) {
-      if (o->verbose) std::printf("discarding vec %d, normdist %g\n", accepted + 1, nd);
-      out.ndisc++;
-    } else {
-      PHDE_TRY(orth.accept(accepted));
-      accepted++;
-    }
-    return PHDE_OK;
-  };
 
   if (eager) {
     auto t0 = clk::now();
@@ -368,41 +402,160 @@ static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, Layout
     out.t_dist += ms_since(t0);
     auto t1 = clk::now();
     for (int rc = 1; rc <= limit && accepted < s; rc++)
-      PHDE_TRY(orthogonalise(rc, cols.p + (size_t)(rc - 1) * n));
+      PHDE_TRY(orth_candidate(orth, o, weighted, fp32, cols.p + (size_t)(rc - 1) * n, sumsq[rc],
+                              accepted, out));
     out.t_orth += ms_since(t1);
   } else {
     for (int rc = 1; rc <= limit && accepted < s; rc++) {
       auto t0 = clk::now();
       PHDE_TRY(traverse(rc, cols.p));
       auto t1 = clk::now();
-      PHDE_TRY(orthogonalise(rc, cols.p));
+      PHDE_TRY(orth_candidate(orth, o, weighted, fp32, cols.p, sumsq[rc], accepted, out));
       out.t_dist += std::chrono::duration<double, std::milli>(t1 - t0).count();
       out.t_orth += ms_since(t1);
     }
   }
   out.nacc = accepted;
-  if (accepted < s) {
-    if (o->verbose)
-      std::fprintf(stderr,
-                   "Number of Pivots obtained: %d inadequate, increase the Pivots Limits: %d\n",
-                   accepted + 1, limit);
-    set_error("only " + std::to_string(accepted) + " of " + std::to_string(s) +
-              " pivots are independent");
-    return PHDE_ERR_PIVOTS;
+  if (accepted < s) return not_enough_pivots(o, accepted, limit);
+  return hde_tail(c, o, orth, fp32, coords, out, want_energy);
+}
+
+// Sources of the random strategies: srand(seed), then rand() % n per source
+// (ParHDE.cpp:160-162, DrawingBase.cpp:242-244; the reference seeds with
+// time(NULL), ParHDE.cpp:113).  RandFine and RandCoarse differ in the
+// reference only in how each column is traversed (DOBFS vs sequential BFS,
+// ParHDE.cpp:163-171); both give the same levels, here from the multi-source
+// BFS.
+static std::vector<int64_t> random_sources(const phde_options *o, int64_t n, int count) {
+  std::vector<int32_t> r(count);
+  glibc_rand_sequence((unsigned)o->seed, count, r.data());
+  std::vector<int64_t> src(count);
+  for (int i = 0; i < count; i++) src[i] = (int64_t)r[i] % n;
+  return src;
+}
+
+// Distance columns of given sources, 64 per multi-source sweep (SSSP one at
+// a time for weighted graphs).  D must hold n * k entries; sumsq gets the
+// per-column sums of squares.
+static int columns_of(phde_ctx *c, const phde_options *o, const int64_t *src, int k, int32_t *D,
+                      unsigned long long *sumsq, LayoutOut &out) {
+  const int64_t n = c->g.n;
+  for (int i = 0; i < k; i++) out.pivot(src[i]);
+  if (c->g.w) {
+    for (int i = 0; i < k; i++) {
+      PHDE_TRY(sssp_run(c, src[i], o->delta, D + (size_t)i * n, nullptr, false));
+      std::vector<unsigned long long> sq(2);
+      PHDE_TRY(column_sumsq(c, D + (size_t)i * n, 1, sq.data()));
+      if (sq[1]) {
+        set_error("graph is not connected");
+        return PHDE_ERR_UNREACHED;
+      }
+      sumsq[i] = sq[0];
+    }
+    return PHDE_OK;
   }
+  for (int b = 0; b < k; b += 64) {
+    const int kb = std::min(64, k - b);
+    PHDE_TRY(msbfs_run(c, src + b, kb, D + (size_t)b * n));
+    std::vector<unsigned long long> sq(kb + 1);
+    PHDE_TRY(column_sumsq(c, D + (size_t)b * n, kb, sq.data()));
+    if (sq[kb]) {
+      set_error("graph is not connected: " + std::to_string(sq[kb]) + " unreached entries");
+      return PHDE_ERR_UNREACHED;
+    }
+    for (int i = 0; i < kb; i++) sumsq[b + i] = sq[i];
+  }
+  return PHDE_OK;
+}
+
+// ParHDE / WParHDE with random pivots: 3s candidate sources, traversed in
+// batches of 64 until s columns are accepted.
+static int layout_hde_random(phde_ctx *c, const phde_options *o, double *coords, LayoutOut &out,
+                             bool want_energy) {
+  const bool weighted = o->algorithm == PHDE_ALGO_WPARHDE;
+  PHDE_TRY(check_weighting(c, weighted));
+  const int64_t n = c->g.n;
+  const int s = o->s;
+  const bool fp32 = o->precision == PHDE_PREC_FP32;
+  const int limit = 3 * s;
+  const std::vector<int64_t> src = random_sources(o, n, limit);
+  Orth orth;
+  PHDE_TRY(orth.setup(c, n, s, fp32));
+  PHDE_TRY(orth.init_ones(c->d_degw, 1.0 * (1 / std::sqrt((double)n))));
+  const int batch = weighted ? 1 : 64;
+  DBuf<int32_t> D;
+  PHDE_TRY(D.alloc((size_t)n * std::min(batch, limit)));
+  std::vector<unsigned long long> sumsq(64);
+  int accepted = 0;
+  for (int b = 0; b < limit && accepted < s; b += batch) {
+    const int kb = std::min(batch, limit - b);
+    auto t0 = clk::now();
+    PHDE_TRY(columns_of(c, o, src.data() + b, kb, D.p, sumsq.data(), out));
+    auto t1 = clk::now();
+    out.t_dist += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    for (int i = 0; i < kb && accepted < s; i++)
+      PHDE_TRY(orth_candidate(orth, o, weighted, fp32, D.p + (size_t)i * n, sumsq[i], accepted, out));
+    out.t_orth += ms_since(t1);
+  }
+  out.nacc = accepted;
+  if (accepted < s) return not_enough_pivots(o, accepted, limit);
+  return hde_tail(c, o, orth, fp32, coords, out, want_energy);
+}
+
+// ParHDE with learned pivots: the first two k-centers traversals give the
+// features and the first two columns, the network scores the vertices, the
+// greedy suppression proposes up to 64 pivots and one multi-source BFS
+// traverses the rest; CGS then takes the columns in proposal order.
+static int layout_learned(phde_ctx *c, const phde_options *o, double *coords, LayoutOut &out) {
+  if (o->algorithm != PHDE_ALGO_PARHDE) {
+    set_error("learned pivots are available for ParHDE only");
+    return PHDE_ERR_UNSUPPORTED;
+  }
+  PHDE_TRY(check_weighting(c, false));
+  if (!c->net) {
+    set_error("no pivot network attached (phde_set_pivot_net)");
+    return PHDE_ERR_ARG;
+  }
+  const int64_t n = c->g.n;
+  const int s = o->s;
+  const bool fp32 = o->precision == PHDE_PREC_FP32;
+  const int K = (int)std::min<int64_t>(64, n);
+  auto t0 = clk::now();
+  int64_t p2 = 0;
+  int e1 = 0, e2 = 0;
+  PHDE_TRY(pivot_features(c, o->first_pivot, &p2, &e1, &e2));
+  PHDE_TRY(net_score(c, c->net));
+  const double rho = o->nms_rho > 0 ? o->nms_rho : 1.0;
+  const int r = (int)std::max(1L, std::min(2046L, std::lround(rho * e1 / std::sqrt((double)s))));
+  std::vector<int64_t> picks(K);
+  picks[0] = o->first_pivot;
+  picks[1] = p2;
+  PHDE_TRY(greedy_select(c, c->lp_score, 2, K, r, picks.data()));
+  out.t_select += ms_since(t0);
+  auto t1 = clk::now();
+  out.pivot(picks[0]);
+  out.pivot(picks[1]);
+  out.levels += e1 + e2;
+  std::vector<unsigned long long> sumsq(K + 1);
+  PHDE_TRY(column_sumsq(c, c->lp_d12, 2, sumsq.data()));
+  DBuf<int32_t> D;
+  PHDE_TRY(D.alloc((size_t)n * std::max(1, K - 2)));
+  if (K > 2) PHDE_TRY(columns_of(c, o, picks.data() + 2, K - 2, D.p, sumsq.data() + 2, out));
+  out.t_dist += ms_since(t1);
   auto t2 = clk::now();
-  Lap lap;
-  PHDE_TRY(lap.setup(c, n, s, fp32));
-  PHDE_TRY(lap.run(orth.S));
-  DBuf<double> B;
-  if (o->eig_mode == PHDE_EIG_GENERALIZED) {
-    PHDE_TRY(B.alloc((size_t)s * s));
-    PHDE_TRY(weighted_gram(c, orth.S, fp32, s, B.p));
+  Orth orth;
+  PHDE_TRY(orth.setup(c, n, s, fp32));
+  PHDE_TRY(orth.init_ones(c->d_degw, 1.0 * (1 / std::sqrt((double)n))));
+  int accepted = 0;
+  for (int i = 0; i < K && accepted < s; i++) {
+    const int32_t *col = i < 2 ? c->lp_d12 + (size_t)i * n : D.p + (size_t)(i - 2) * n;
+    PHDE_TRY(orth_candidate(orth, o, false, fp32, col, sumsq[i], accepted, out));
   }
-  PHDE_HIP(hipStreamSynchronize(c->stream));
-  out.t_mmult += ms_since(t2);
-  (void)t_start;
-  return finish_layout(c, o, orth.S, fp32, s, lap.SLS, B.p, false, coords, out);
+  out.t_orth += ms_since(t2);
+  out.nacc = accepted;
+  if (c->timer.enabled && out.st) out.st->n_candidates = K;
+  if (accepted < s) return not_enough_pivots(o, accepted, K);
+  return hde_tail(c, o, orth, fp32, coords, out, true);
 }
 
 __global__ void center_kernel(const int32_t *D, int64_t n, int s, const double *colterm,
@@ -467,7 +620,13 @@ static int layout_mds(phde_ctx *c, const phde_options *o, double *coords, Layout
   KCenters kc;
   kc.next = o->first_pivot;
   auto t0 = clk::now();
-  for (int rc = 0; rc < s; rc++) {
+  if (o->strategy == PHDE_PIVOTS_RANDFINE || o->strategy == PHDE_PIVOTS_RANDCOARSE) {
+    // DrawingBase.cpp:240-260: s random sources
+    const std::vector<int64_t> src = random_sources(o, n, s);
+    std::vector<unsigned long long> sq(s);
+    PHDE_TRY(columns_of(c, o, src.data(), s, D.p, sq.data(), out));
+  }
+  for (int rc = 0; rc < s && o->strategy == PHDE_PIVOTS_KCENTERS; rc++) {
     const int64_t src = kc.next;
     out.pivot(src);
     int32_t *col = D.p + (size_t)rc * n;
@@ -543,6 +702,9 @@ void phde_default_options(phde_options *o) {
   o->verbose = 0;
   o->first_pivot = 0;
   o->seed = 1;
+  o->judge_tol = 0.02;
+  o->nms_rho = 1.0;
+  o->judge = 0;
 }
 
 const char *phde_version(void) { return "parhde-mi355x 0.1 (gfx950)"; }
@@ -686,9 +848,9 @@ int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats
     set_error("invalid layout options");
     return PHDE_ERR_ARG;
   }
-  if (o.strategy != PHDE_PIVOTS_KCENTERS) {
-    set_error("pivot strategy not available");
-    return PHDE_ERR_UNSUPPORTED;
+  if (o.strategy < PHDE_PIVOTS_RANDCOARSE || o.strategy > PHDE_PIVOTS_LEARNED) {
+    set_error("unknown pivot strategy");
+    return PHDE_ERR_ARG;
   }
   if (c->g.n < 2) {
     set_error("graph too small");
@@ -702,10 +864,43 @@ int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats
   out.st = stats;
   auto t0 = clk::now();
   int rc;
-  if (o.algorithm == PHDE_ALGO_PARHDE || o.algorithm == PHDE_ALGO_WPARHDE)
-    rc = layout_hde(c, &o, coords, out);
-  else
+  const bool hde = o.algorithm == PHDE_ALGO_PARHDE || o.algorithm == PHDE_ALGO_WPARHDE;
+  const bool random = o.strategy == PHDE_PIVOTS_RANDFINE || o.strategy == PHDE_PIVOTS_RANDCOARSE;
+  int fallback = 0;
+  double e_kc = 0;
+  if (o.strategy == PHDE_PIVOTS_LEARNED) {
+    // k-centers is the fallback (too few independent proposals) and, with
+    // o.judge, the judge: its layout replaces the learned one when the
+    // learned Hall energy is more than judge_tol above it.
+    phde_options ok = o;
+    ok.strategy = PHDE_PIVOTS_KCENTERS;
+    const double tol = o.judge_tol;
+    rc = layout_learned(c, &o, coords, out);
+    if (rc == PHDE_ERR_PIVOTS) {
+      fallback = 1;
+      LayoutOut ko;
+      ko.st = stats;
+      rc = layout_hde(c, &ok, coords, ko, true);
+      out.t_dist += ko.t_dist, out.t_orth += ko.t_orth, out.t_mmult += ko.t_mmult;
+      out.t_eig += ko.t_eig, out.t_coords += ko.t_coords, out.nacc = ko.nacc, out.energy = ko.energy;
+      e_kc = ko.energy;
+    } else if (rc == PHDE_OK && o.judge) {
+      std::vector<double> Xk((size_t)c->g.n * o.num_coord);
+      LayoutOut ko;
+      rc = layout_hde(c, &ok, Xk.data(), ko, true);
+      e_kc = ko.energy;
+      if (rc == PHDE_OK && out.energy > (1 + tol) * e_kc) {
+        fallback = 1;
+        std::memcpy(coords, Xk.data(), sizeof(double) * Xk.size());
+        out.energy = e_kc;
+      }
+    }
+  } else if (hde) {
+    rc = random ? layout_hde_random(c, &o, coords, out, stats != nullptr)
+                : layout_hde(c, &o, coords, out, stats != nullptr);
+  } else {
     rc = layout_mds(c, &o, coords, out);
+  }
   const double total = ms_since(t0);
   c->timer.resolve();
   c->timer.enabled = false;
@@ -721,12 +916,45 @@ int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats
     stats->n_discarded = out.ndisc;
     stats->eig_iters = out.iters;
     stats->levels_total = out.levels;
+    stats->ms_select = out.t_select;
+    stats->hall_energy = out.energy;
+    stats->hall_energy_kcenters = e_kc;
+    stats->fallback = fallback;
     for (int k = 0; k < 16; k++) {
       stats->kernel_ms[k] = c->timer.ms[k];
       stats->kernel_launches[k] = c->timer.launches[k];
     }
   }
   return rc;
+}
+
+int phde_kcenters(phde_ctx *c, int s, int64_t first, int64_t *sources, int32_t *min_dist) {
+  if (!c || s < 1 || first < 0 || first >= c->g.n) return PHDE_ERR_ARG;
+  const int64_t n = c->g.n;
+  PHDE_TRY(bfs_setup(c));
+  DBuf<int32_t> md;
+  PHDE_TRY(md.alloc(n));
+  PHDE_TRY(fill_i32(c, md.p, n, INT_MAX));
+  KCenters kc;
+  kc.next = first;
+  for (int i = 0; i < s; i++) {
+    const int64_t src = kc.next;
+    if (sources) sources[i] = src;
+    if (c->g.w)
+      PHDE_TRY(sssp_run(c, src, 0, c->d_dist, md.p, true));
+    else
+      PHDE_TRY(bfs_run(c, src, 15, 18, c->d_dist, md.p, true));
+    const KcResult r = *c->h_kc;
+    if (r.unreached) {
+      set_error("graph is not connected");
+      return PHDE_ERR_UNREACHED;
+    }
+    PHDE_TRY(kc.step(c, md.p, r, i != 0, true, src));
+  }
+  if (min_dist)
+    PHDE_HIP(hipMemcpyAsync(min_dist, md.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  return PHDE_OK;
 }
 
 int phde_power_eig(phde_ctx *c, double *K, double *evals, int dim, int32_t *iters) {

@@ -109,6 +109,12 @@ struct phde_ctx {
   unsigned long long *ms_visit = nullptr, *ms_fr[3] = {nullptr, nullptr, nullptr};
   uint32_t *ms_q3 = nullptr;
   int64_t *ms_src = nullptr;
+  // learned pivot proposal (learned.hip): features (n float4), the two
+  // feature traversals (2n), distance-to-pivots field, scores, training target
+  float *lp_feat = nullptr;
+  int32_t *lp_d12 = nullptr, *lp_sd = nullptr;
+  float *lp_score = nullptr, *lp_target = nullptr;
+  phde_mpnn *net = nullptr;  // borrowed (phde_set_pivot_net)
   phde::Timer timer;
 };
 
@@ -119,6 +125,19 @@ struct phde_ctx {
   } while (0)
 
 namespace phde {
+
+// RAII device buffer for driver temporaries.
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  int alloc(size_t count) {
+    PHDE_HIP(hipMalloc(&p, sizeof(T) * (count > 0 ? count : 1)));
+    return PHDE_OK;
+  }
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
 
 // bfs.hip
 int bfs_setup(phde_ctx *c);

@@ -68,6 +68,11 @@ class LayoutStats:
     pivots: list = field(default_factory=list)
     kernel_ms: dict = field(default_factory=dict)
     kernel_launches: dict = field(default_factory=dict)
+    ms_select: float = 0.0
+    hall_energy: float = 0.0
+    hall_energy_kcenters: float = 0.0
+    fallback: int = 0
+    n_candidates: int = 0
 
     @classmethod
     def from_c(cls, s: PhdeStats) -> "LayoutStats":
@@ -76,7 +81,9 @@ class LayoutStats:
                    int(s.n_accepted), int(s.n_discarded), int(s.eig_iters), int(s.levels_total),
                    [int(s.pivots[i]) for i in range(min(nt, 192))],
                    {k: float(s.kernel_ms[i]) for i, k in enumerate(KERNEL_CLASSES)},
-                   {k: int(s.kernel_launches[i]) for i, k in enumerate(KERNEL_CLASSES)})
+                   {k: int(s.kernel_launches[i]) for i, k in enumerate(KERNEL_CLASSES)},
+                   float(s.ms_select), float(s.hall_energy), float(s.hall_energy_kcenters),
+                   int(s.fallback), int(s.n_candidates))
 
 
 def default_options(**kw) -> PhdeOptions:
@@ -182,6 +189,43 @@ class Context:
                                      as_ptr(ev, ctypes.c_double), as_ptr(V, ctypes.c_double)), "phde_sym_eig")
         return ev, V.reshape(dim, dim).T.copy()
 
+    # -------------------------------------------------------- learned pivots
+    def kcenters(self, s: int, first: int = 0):
+        """Plain k-centers: (sources[s], final distance-to-nearest-source field)."""
+        src = np.empty(s, dtype=np.int64)
+        md = np.empty(self.n, dtype=np.int32)
+        _check(self.lib.phde_kcenters(self.h, s, first, as_ptr(src, ctypes.c_int64), as_ptr(md, ctypes.c_int32)),
+               "phde_kcenters")
+        return src, md
+
+    def pivot_features(self, first: int = 0):
+        """(n x 4 float32 features, the two traversal sources)."""
+        f = np.empty(self.n * 4, dtype=np.float32)
+        src = np.empty(2, dtype=np.int64)
+        _check(self.lib.phde_pivot_features(self.h, first, as_ptr(f, ctypes.c_float), as_ptr(src, ctypes.c_int64)),
+               "phde_pivot_features")
+        return f.reshape(self.n, 4), [int(x) for x in src]
+
+    def set_training_data(self, feat: np.ndarray, target: np.ndarray) -> None:
+        f = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1)
+        t = np.ascontiguousarray(target, dtype=np.float32).reshape(-1)
+        if f.size != 4 * self.n or t.size != self.n:
+            raise ValueError("features must be n x 4 and the target n")
+        self._train = (f, t)
+        _check(self.lib.phde_set_training_data(self.h, as_ptr(f, ctypes.c_float), as_ptr(t, ctypes.c_float)),
+               "phde_set_training_data")
+
+    def set_pivot_net(self, net: Optional["PivotNet"]) -> None:
+        self._net = net
+        _check(self.lib.phde_set_pivot_net(self.h, net.h if net is not None else None), "phde_set_pivot_net")
+
+    def select_pivots(self, score: np.ndarray, k: int, radius: int, first: int = 0) -> np.ndarray:
+        sc = np.ascontiguousarray(score, dtype=np.float32)
+        out = np.empty(k, dtype=np.int64)
+        _check(self.lib.phde_select_pivots(self.h, as_ptr(sc, ctypes.c_float), first, k, radius,
+                                           as_ptr(out, ctypes.c_int64)), "phde_select_pivots")
+        return out
+
     def hall_energy(self, Y: np.ndarray) -> float:
         Y = np.asarray(Y, dtype=np.float64)
         if Y.ndim == 1:
@@ -191,6 +235,77 @@ class Context:
         _check(self.lib.phde_hall_energy(self.h, as_ptr(Yc, ctypes.c_double), Y.shape[1], ctypes.byref(e)),
                "phde_hall_energy")
         return e.value
+
+
+class PivotNet:
+    """The learned pivot-proposal network (phde_net_*): 3 message-passing
+    layers of width 64 and a linear read-out, trained on the GPU."""
+
+    def __init__(self, seed: int = 1, device: int = 0, path: Optional[str] = None):
+        self.lib = _native.parhde_lib()
+        h = ctypes.c_void_p()
+        _check(self.lib.phde_net_create(device, seed, ctypes.byref(h)), "phde_net_create")
+        self.h = h
+        if path is not None:
+            self.load(path)
+
+    @staticmethod
+    def num_params() -> int:
+        return int(_native.parhde_lib().phde_net_num_params())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.phde_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def params(self) -> np.ndarray:
+        p = np.empty(self.num_params(), dtype=np.float32)
+        _check(self.lib.phde_net_get_params(self.h, as_ptr(p, ctypes.c_float)), "phde_net_get_params")
+        return p
+
+    @params.setter
+    def params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.num_params():
+            raise ValueError("wrong parameter count")
+        _check(self.lib.phde_net_set_params(self.h, as_ptr(p, ctypes.c_float)), "phde_net_set_params")
+
+    def save(self, path: str) -> None:
+        _check(self.lib.phde_net_save(self.h, str(path).encode()), "phde_net_save")
+
+    def load(self, path: str) -> None:
+        _check(self.lib.phde_net_load(self.h, str(path).encode()), "phde_net_load")
+
+    def scores(self, ctx: Context) -> np.ndarray:
+        y = np.empty(ctx.n, dtype=np.float32)
+        _check(self.lib.phde_net_scores(self.h, ctx.h, as_ptr(y, ctypes.c_float)), "phde_net_scores")
+        return y
+
+    def gradient(self, ctx: Context):
+        """(mean squared error, gradient) on the context's training data."""
+        g = np.empty(self.num_params(), dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(self.lib.phde_net_gradient(self.h, ctx.h, ctypes.byref(loss), as_ptr(g, ctypes.c_float)),
+               "phde_net_gradient")
+        return loss.value, g
+
+    def train_step(self, ctx: Context, lr: float) -> float:
+        loss = ctypes.c_double()
+        _check(self.lib.phde_net_train_step(self.h, ctx.h, lr, ctypes.byref(loss)), "phde_net_train_step")
+        return loss.value
 
 
 class DrawingBase:

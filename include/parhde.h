@@ -41,6 +41,8 @@ typedef enum phde_status {
 } phde_status;
 
 typedef struct phde_ctx phde_ctx;
+/* Learned pivot proposal network (learned-pivot strategy). */
+typedef struct phde_mpnn phde_mpnn;
 
 /* Algorithms (spectralDrawing.cpp:392-406 `-a`). */
 enum { PHDE_ALGO_PIVOTMDS = 0, PHDE_ALGO_PHDE = 1, PHDE_ALGO_WPARHDE = 2, PHDE_ALGO_PARHDE = 3 };
@@ -65,7 +67,12 @@ typedef struct phde_options {
   int32_t alpha, beta; /* direction-switch parameters (bfs.cc:161,172); defaults 15, 18 */
   int32_t verbose;     /* 1: print the reference's progress lines to stdout */
   int64_t first_pivot; /* first k-centers source; reference: 0 */
-  uint64_t seed;       /* random strategies */
+  uint64_t seed;       /* random strategies: srand(seed) then rand() % n per source */
+  /* learned strategy */
+  double judge_tol;    /* with judge: keep k-centers when E_learned > (1 + judge_tol) E_kcenters; default 0.02 */
+  double nms_rho;      /* suppression radius r = nms_rho * ecc / sqrt(s) hops; <= 0 picks 1.0 */
+  int32_t judge;       /* 1: also run the k-centers layout and keep the better one (see judge_tol) */
+  int32_t reserved;
 } phde_options;
 
 typedef struct phde_stats {
@@ -83,6 +90,12 @@ typedef struct phde_stats {
   int64_t pivots[192];   /* sources in traversal order (first n_traversals valid) */
   double  kernel_ms[16]; /* per-kernel-class device time, see PHDE_KERNEL_* */
   int64_t kernel_launches[16];
+  /* learned strategy */
+  double ms_select;            /* features, network scores and pivot selection */
+  double hall_energy;          /* tr(Y^T L Y) / tr(Y^T D Y) of the returned layout */
+  double hall_energy_kcenters; /* of the k-centers layout when judged, else 0 */
+  int32_t fallback;            /* 1 when the k-centers layout was returned instead */
+  int32_t n_candidates;        /* pivots proposed by the selection */
 } phde_stats;
 
 enum {
@@ -152,6 +165,52 @@ int phde_sym_eig(phde_ctx *ctx, const double *A, const double *B, int dim, doubl
 
 /* Hall energy tr(Y^T L Y) / tr(Y^T D Y) of an n*c column-major layout. */
 int phde_hall_energy(phde_ctx *ctx, const double *Y, int c, double *energy);
+
+/* ---------------------------------------------------------------- learned
+ * pivots.  No reference counterpart: a message-passing network scores the
+ * vertices from features of two traversals, a greedy graph non-maximum
+ * suppression picks up to 64 spread-out pivots, and one multi-source BFS
+ * gives their distance columns (phde_layout with PHDE_PIVOTS_LEARNED and
+ * PHDE_ALGO_PARHDE).  The network is trained on the GPU with the calls
+ * below against k-centers coverage fields. */
+
+/* Number of FP32 parameters of the network. */
+int phde_net_num_params(void);
+/* New network on `device` with seeded Glorot-uniform weights. */
+int phde_net_create(int device, uint64_t seed, phde_mpnn **out);
+int phde_net_destroy(phde_mpnn *net);
+int phde_net_get_params(phde_mpnn *net, float *params);
+/* Also resets the optimiser state. */
+int phde_net_set_params(phde_mpnn *net, const float *params);
+/* Weight file: "PHDENET1", int32 count, FP32 parameters. */
+int phde_net_save(phde_mpnn *net, const char *path);
+int phde_net_load(phde_mpnn *net, const char *path);
+/* Attach (borrowed) or detach (NULL) the network used by PHDE_PIVOTS_LEARNED. */
+int phde_set_pivot_net(phde_ctx *ctx, phde_mpnn *net);
+
+/* Node features (n*4, row-major): log(1+deg)/4, BFS level from `first` /
+ * its eccentricity, BFS level from the vertex farthest from `first` / its
+ * eccentricity, 1.  sources (optional, 2 entries) gets the two sources. */
+int phde_pivot_features(phde_ctx *ctx, int64_t first, float *feat, int64_t *sources);
+/* Training data of this context's graph: features (n*4) and target (n). */
+int phde_set_training_data(phde_ctx *ctx, const float *feat, const float *target);
+/* Network scores (n) from the context's features. */
+int phde_net_scores(phde_mpnn *net, phde_ctx *ctx, float *score);
+/* Mean squared error against the target and its gradient (optional). */
+int phde_net_gradient(phde_mpnn *net, phde_ctx *ctx, double *loss, float *grad);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient. */
+int phde_net_adam(phde_mpnn *net, phde_ctx *ctx, float lr);
+/* phde_net_gradient + phde_net_adam; loss before the step. */
+int phde_net_train_step(phde_mpnn *net, phde_ctx *ctx, float lr, double *loss);
+/* Greedy suppression on given scores: pivots[0] = first, pivots[1] = the
+ * vertex farthest from it, then k-2 picks each the best-scoring vertex more
+ * than `radius` hops from all previous picks (if none is left, the one
+ * farthest from them).  2 <= k <= 64. */
+int phde_select_pivots(phde_ctx *ctx, const float *score, int64_t first, int k, int radius,
+                       int64_t *pivots);
+/* k-centers alone (farthest-first, lowest index on ties, first source
+ * `first`): s sources and the final distance-to-nearest-source field. */
+int phde_kcenters(phde_ctx *ctx, int s, int64_t first, int64_t *sources, int32_t *min_dist);
 
 /* Last error text of this thread (empty when none). */
 const char *phde_last_error(void);

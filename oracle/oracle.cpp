@@ -560,11 +560,16 @@ int or_compute_eigens(double *K, int64_t dim, int numCoord, int which, double *e
   return OR_OK;
 }
 
-int or_parhde(const or_graph *g, int s, int numCoord, int lazy, or_result *r) {
-  // ParHDE::run (ParHDE.cpp:45-352), doRand == 2 (k-centers), one try.
+// ParHDE::run (ParHDE.cpp:45-352), one try.  given == nullptr: k-centers
+// sources (doRand == 2); otherwise the candidate sources in the given order
+// (the random strategies, ParHDE.cpp:158-172, and the learned proposals),
+// traversed lazily until s columns are accepted.
+static int parhde_impl(const or_graph *g, int s, int numCoord, int lazy, const int64_t *given,
+                       int ngiven, or_result *r) {
   if (!g || !r || s < 1 || s > 64 || numCoord < 1 || numCoord > s || g->w) return OR_ERR_ARG;
   const int64_t n = g->n;
-  const int limit = 3 * s;
+  const int limit = given ? ngiven : 3 * s;
+  if (given) lazy = 1;
   r->n_bfs = r->n_accepted = 0;
   r->t_hde = r->t_dist = r->t_orth = r->t_mmult = 0;
   if (r->accepted)
@@ -590,9 +595,11 @@ int or_parhde(const or_graph *g, int s, int numCoord, int lazy, or_result *r) {
   std::set<int64_t> sources;
   // Per-column k-centers step + L2 normalisation of the new column.
   auto traverse = [&](int run_count, double *col) {
+    if (given) start_idx = given[run_count - 1];
     if (r->pivots) r->pivots[run_count - 1] = start_idx;
     distance_column(g, start_idx, col, 0);
     r->n_bfs++;
+    if (given) return;
     double maxval;
     int64_t idx;
     kcenter_argmax(n, col, min_dists.data(), &maxval, &idx);
@@ -662,6 +669,16 @@ int or_parhde(const or_graph *g, int s, int numCoord, int lazy, or_result *r) {
   int rc = OR_OK;
   finish_layout(g, degrees.data(), S.data(), s, numCoord, r, t_start, &rc);
   return rc;
+}
+
+int or_parhde(const or_graph *g, int s, int numCoord, int lazy, or_result *r) {
+  return parhde_impl(g, s, numCoord, lazy, nullptr, 0, r);
+}
+
+int or_parhde_sources(const or_graph *g, int s, int numCoord, const int64_t *sources, int nsrc,
+                      or_result *r) {
+  if (!sources || nsrc < 1) return OR_ERR_ARG;
+  return parhde_impl(g, s, numCoord, 1, sources, nsrc, r);
 }
 
 int or_wparhde(const or_graph *g, int s, int numCoord, int32_t delta, or_result *r) {

@@ -69,6 +69,10 @@ int or_compute_eigens(double *K, int64_t dim, int numCoord, int which, double *e
  * accepted (bit-identical outputs, fewer traversals); lazy == 0 is the
  * reference's eager 3s traversals.  delta is used for weighted graphs. */
 int or_parhde(const or_graph *g, int s, int numCoord, int lazy, or_result *r);
+/* ParHDE with the candidate sources given in order (random strategies,
+ * ParHDE.cpp:158-172, and learned proposals): CGS until s are accepted. */
+int or_parhde_sources(const or_graph *g, int s, int numCoord, const int64_t *sources, int nsrc,
+                      or_result *r);
 int or_wparhde(const or_graph *g, int s, int numCoord, int32_t delta, or_result *r);
 int or_pivotmds(const or_graph *g, int s, int numCoord, or_result *r);
 int or_phde(const or_graph *g, int s, int numCoord, or_result *r);

@@ -138,9 +138,9 @@ def ref_power_eig(K):
     return Kf, ev, rc
 
 
-def _layout(fn, g, s, c, *extra, want_S=False):
+def _layout(fn, g, s, c, *extra, want_S=False, lim=None):
     n = g.n
-    lim = 3 * s
+    lim = 3 * s if lim is None else lim
     out = {
         "coords": np.zeros(n * c), "coords_raw": np.zeros(n * c), "SLS": np.zeros(s * s),
         "eigvec": np.zeros(s * c), "eigval": np.zeros(s), "pivots": np.full(lim, -1, np.int64),
@@ -168,6 +168,18 @@ def _layout(fn, g, s, c, *extra, want_S=False):
 
 def parhde(g, s, c=2, lazy=True, want_S=False):
     return _layout(lib().or_parhde, g, s, c, 1 if lazy else 0, want_S=want_S)
+
+
+def parhde_sources(g, s, sources, c=2, want_S=False):
+    """ParHDE with the candidate sources in the given order (random / learned
+    pivots): CGS until s columns are accepted."""
+    src = np.ascontiguousarray(sources, dtype=np.int64)
+    L = lib()
+    L.or_parhde_sources.argtypes = [ctypes.POINTER(OrGraph), ctypes.c_int, ctypes.c_int, c_i64p, ctypes.c_int,
+                                    ctypes.POINTER(OrResult)]
+    L.or_parhde_sources.restype = ctypes.c_int
+    return _layout(L.or_parhde_sources, g, s, c, _p(src, ctypes.c_int64), len(src), want_S=want_S,
+                   lim=len(src))
 
 
 def wparhde(g, s, c=2, delta=10000, want_S=False):

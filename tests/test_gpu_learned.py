@@ -1,0 +1,223 @@
+"""The learned pivot proposal on the GPU against its CPU oracle
+(oracle/learned.py and the given-sources ParHDE oracle): features, network
+forward and gradient (BF16 MFMA kernels), training, the greedy selection
+(bit-exact), random-pivot and learned-pivot layouts (FP64 coordinates within
+1e-8), and the k-centers judge / fallback."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from icpp_2020_parhde_amd import layout as L
+from icpp_2020_parhde_amd import learned as LP
+from icpp_2020_parhde_amd.graphs import Graph
+from oracle import learned as OL
+from oracle import pyoracle as O
+
+from test_gpu_core import aligned_rel_err, FP64_TOL
+
+pytestmark = pytest.mark.gpu
+
+GRAPHS = {
+    "mesh": lambda: Graph.mesh(60, 70),
+    "rgg": lambda: Graph.rgg(8000, 10.0, seed=11),
+    "holed": lambda: Graph.holed_grid(80, 90, 6, 12),
+    "rmat": lambda: Graph.rmat(12, 16, seed=13),
+}
+
+
+def _random_params(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(OL.NPARAMS) * scale).astype(np.float32)
+
+
+@pytest.mark.parametrize("name", list(GRAPHS))
+def test_features(gpu, name):
+    g = GRAPHS[name]()
+    with L.Context(g) as ctx:
+        x, (s1, s2) = ctx.pivot_features(0)
+    d1 = O.seq_bfs(g, 0)
+    assert s1 == 0 and s2 == int(np.argmax(d1))
+    want = OL.features(g, d1, O.seq_bfs(g, s2))
+    assert np.array_equal(x[:, 1:], want[:, 1:])
+    np.testing.assert_allclose(x[:, 0], want[:, 0], rtol=2e-7)
+
+
+@pytest.mark.parametrize("name", list(GRAPHS))
+def test_forward_and_gradient(gpu, name):
+    g = GRAPHS[name]()
+    rng = np.random.default_rng(3)
+    with L.Context(g) as ctx, L.PivotNet(seed=1) as net:
+        x, _ = ctx.pivot_features(0)
+        t = rng.random(g.n).astype(np.float32)
+        ctx.set_training_data(x, t)
+        p = _random_params(7)
+        net.params = p
+        y = net.scores(ctx)
+        loss, grad = net.gradient(ctx)
+    loss_o, grad_o, y_o = OL.loss_and_grad(g, x, t, p, emulate=True)
+    err_y = np.max(np.abs(y - y_o)) / np.max(np.abs(y_o))
+    print(f"{name}: forward rel err {err_y:.2e}, loss {loss:.6g} vs {loss_o:.6g}")
+    assert err_y < 5e-3
+    assert loss == pytest.approx(loss_o, rel=5e-3)
+    o = 0
+    for pname, shape in OL.LAYOUT:
+        k = int(np.prod(shape))
+        gg, go = grad[o:o + k], grad_o[o:o + k]
+        rel = np.linalg.norm(gg - go) / max(np.linalg.norm(go), 1e-30)
+        print(f"  grad {pname}: rel err {rel:.2e}")
+        assert rel < 1e-2, pname
+        o += k
+
+
+def test_training_lowers_loss(gpu):
+    g = Graph.rgg(20000, 10.0, seed=21)
+    with L.Context(g) as ctx, L.PivotNet(seed=2) as net:
+        x, _ = ctx.pivot_features(0)
+        _, md = ctx.kcenters(20, 0)
+        ctx.set_training_data(x, OL.coverage_target(md))
+        first = net.train_step(ctx, 3e-3)
+        for _ in range(150):
+            last = net.train_step(ctx, 3e-3)
+    print(f"loss {first:.5f} -> {last:.5f}")
+    assert last < 0.5 * first
+
+
+def test_params_roundtrip(gpu, tmp_path):
+    with L.PivotNet(seed=3) as a, L.PivotNet(seed=4) as b:
+        assert not np.array_equal(a.params, b.params)
+        a.save(tmp_path / "w.bin")
+        b.load(tmp_path / "w.bin")
+        assert np.array_equal(a.params, b.params)
+
+
+def test_kcenters_api(gpu):
+    g = Graph.mesh(40, 50)
+    ref = O.parhde(g, 20, 2)
+    with L.Context(g) as ctx:
+        src, md = ctx.kcenters(12, 0)
+    assert src.tolist() == ref["pivots"][:12]
+    want = np.min(np.stack([O.seq_bfs(g, int(s)) for s in src]), axis=0)
+    assert np.array_equal(md, want)
+
+
+@pytest.mark.parametrize("name", list(GRAPHS))
+@pytest.mark.parametrize("kind", ["random", "constant"])
+def test_greedy_selection_bit_exact(gpu, name, kind):
+    g = GRAPHS[name]()
+    rng = np.random.default_rng(9)
+    score = rng.random(g.n).astype(np.float32) if kind == "random" else np.zeros(g.n, np.float32)
+    d1 = O.seq_bfs(g, 0)
+    second = int(np.argmax(d1))
+    d2 = O.seq_bfs(g, second)
+    for k, r in [(64, 3), (20, 1), (40, 9)]:
+        with L.Context(g) as ctx:
+            got = ctx.select_pivots(score, k, r, 0)
+        want = OL.greedy_select(g, score, d1, d2, 0, second, k, r)
+        assert got.tolist() == want.tolist(), (k, r)
+
+
+def _libc_sources(seed, count, n):
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(seed)
+    return [libc.rand() % n for _ in range(count)]
+
+
+@pytest.mark.parametrize("strategy", [L.PIVOTS_RANDFINE, L.PIVOTS_RANDCOARSE])
+def test_random_pivots_layout(gpu, strategy):
+    g = Graph.rgg(20000, 10.0, seed=2)
+    s = 20
+    with L.Context(g) as ctx:
+        X, st = ctx.layout(s=s, num_coord=2, strategy=strategy, seed=17)
+    src = _libc_sources(17, 3 * s, g.n)
+    ref = O.parhde_sources(g, s, src, 2)
+    assert st.pivots[: len(ref["pivots"])] == ref["pivots"]
+    err = aligned_rel_err(X, ref["coords"])
+    print(f"random pivots: rel err {err:.2e}")
+    assert err < FP64_TOL
+
+
+def test_random_pivots_pivotmds(gpu):
+    g = Graph.mesh(50, 60)
+    with L.Context(g) as ctx:
+        X, st = ctx.layout(s=12, num_coord=2, strategy=L.PIVOTS_RANDFINE, algorithm=L.ALGO_PIVOTMDS, seed=5)
+    assert st.pivots[:12] == _libc_sources(5, 12, g.n)
+    assert np.all(np.isfinite(X))
+
+
+@pytest.mark.parametrize("name", ["mesh", "rgg", "holed"])
+def test_learned_layout_parity(gpu, name):
+    """The learned layout equals ParHDE on its proposed sources (oracle)."""
+    g = GRAPHS[name]()
+    s = 20
+    with L.Context(g) as ctx, L.PivotNet(seed=5) as net:
+        net.params = _random_params(8, 0.2)
+        ctx.set_pivot_net(net)
+        X, st = ctx.layout(s=s, num_coord=2, strategy=L.PIVOTS_LEARNED)
+        ctx.set_pivot_net(None)
+    assert st.fallback == 0 and st.n_candidates == 64
+    ref = O.parhde_sources(g, s, st.pivots[: st.n_traversals], 2)
+    assert ref["rc"] == 0
+    err = aligned_rel_err(X, ref["coords"])
+    print(f"{name}: learned layout rel err {err:.2e}, energy {st.hall_energy:.6g}")
+    assert err < FP64_TOL
+    assert st.hall_energy == pytest.approx(O.hall_energy(g, ref["coords"]), rel=1e-9)
+
+
+def test_learned_selection_matches_oracle(gpu):
+    """Pivots of the learned layout = oracle greedy selection on the GPU's own scores."""
+    g = GRAPHS["rgg"]()
+    s = 20
+    with L.Context(g) as ctx, L.PivotNet(seed=5) as net:
+        net.params = _random_params(8, 0.2)
+        ctx.set_pivot_net(net)
+        _, st = ctx.layout(s=s, num_coord=2, strategy=L.PIVOTS_LEARNED)
+        ctx.set_pivot_net(None)
+        x, (s1, s2) = ctx.pivot_features(0)
+        score = net.scores(ctx)
+    d1 = O.seq_bfs(g, s1)
+    d2 = O.seq_bfs(g, s2)
+    r = max(1, int(np.floor(1.0 * d1.max() / np.sqrt(s) + 0.5)))  # lround
+    want = OL.greedy_select(g, score, d1, d2, s1, s2, 64, r)
+    assert st.pivots[:64] == want.tolist()
+
+
+def test_judge_forced_fallback(gpu):
+    g = GRAPHS["mesh"]()
+    with L.Context(g) as ctx, L.PivotNet(seed=5) as net:
+        ctx.set_pivot_net(net)
+        Xk, stk = ctx.layout(s=10, num_coord=2)
+        # learned must beat k-centers by 50%: never, so the k-centers layout returns
+        X, st = ctx.layout(s=10, num_coord=2, strategy=L.PIVOTS_LEARNED, judge=1, judge_tol=-0.5)
+        assert st.fallback == 1
+        assert np.array_equal(X, Xk)
+        assert st.hall_energy_kcenters == pytest.approx(ctx.hall_energy(Xk), rel=1e-9)
+        # generous tolerance: the learned layout is kept
+        X2, st2 = ctx.layout(s=10, num_coord=2, strategy=L.PIVOTS_LEARNED, judge=1, judge_tol=10.0)
+        assert st2.fallback == 0 and st2.hall_energy_kcenters > 0
+        ctx.set_pivot_net(None)
+
+
+def test_learned_without_network_is_an_error(gpu):
+    g = GRAPHS["mesh"]()
+    with L.Context(g) as ctx:
+        with pytest.raises(L.PhdeError):
+            ctx.layout(s=10, num_coord=2, strategy=L.PIVOTS_LEARNED)
+
+
+def test_union_trains_like_parts(gpu):
+    """A disjoint union's scores are its parts' scores (the batching of training)."""
+    a, b = Graph.mesh(20, 30), Graph.rgg(3000, 8.0, seed=3)
+    u = LP.union([a, b])
+    with L.PivotNet(seed=6) as net:
+        net.params = _random_params(9, 0.2)
+        ys = []
+        for g in (a, b):
+            with L.Context(g) as ctx:
+                x, _ = ctx.pivot_features(0)
+                ctx.set_training_data(x, np.zeros(g.n, np.float32))
+                ys.append((x, net.scores(ctx)))
+        with L.Context(u) as ctx:
+            ctx.set_training_data(np.concatenate([ys[0][0], ys[1][0]]), np.zeros(u.n, np.float32))
+            yu = net.scores(ctx)
+    np.testing.assert_allclose(yu, np.concatenate([ys[0][1], ys[1][1]]), rtol=1e-6, atol=1e-6)
