@@ -87,6 +87,10 @@ def main():
     ap.add_argument("--eager", action="store_true")
     ap.add_argument("--extras", type=int, default=1, help="also time eager and fp32 once")
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--learned", type=int, default=1,
+                    help="also time the learned-pivot layout against k-centers (needs the weights file)")
+    ap.add_argument("--learned-n", type=int, default=1 << 24,
+                    help="points of the second, held-out RGG for the learned comparison (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the layouts, run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -138,6 +142,20 @@ def main():
             _, sf = ctx.layout(s=args.s, num_coord=2, precision=L.PREC_FP32)
             extras["fp32_ms_per_layout"] = round(sf.ms_total, 2)
         log(f"[bench] extras {extras}")
+    learned = None
+    if args.learned:
+        learned = {"rgg4m": learned_vs_kcenters(ctx, X, ms_per_layout, args.s, prec)}
+        ctx.close()
+        if args.learned_n:
+            t0 = time.time()
+            g5 = Graph.rgg(args.learned_n, 10.0, seed=5)
+            log(f"[bench] rgg n={g5.n} nnz={g5.m} (seed 5) generated in {time.time() - t0:.1f}s")
+            with L.Context(g5) as c5:
+                t = time.perf_counter()
+                X5, s5 = c5.layout(s=args.s, num_coord=2, precision=prec)
+                ms5 = (time.perf_counter() - t) * 1e3
+                log(f"[bench] rgg{g5.n} k-centers layout {ms5:.1f} ms")
+                learned[f"rgg{args.learned_n >> 20}m"] = learned_vs_kcenters(c5, X5, ms5, args.s, prec)
 
     kernel = bfs_kernel_name()
     workload = f"rgg{args.n >> 20}m_s{args.s}" if args.n >= 1 << 20 else f"rgg{args.n}_s{args.s}"
@@ -170,6 +188,7 @@ def main():
                      "algorithmic_bytes": bfs_bytes(g.n, g.m)},
         "h2d_ms": round(h2d_s * 1e3, 1),
         **extras,
+        "learned_pivots": learned,
         "cpu_baseline": None,
     }
     if args.cpu_baseline:
@@ -177,6 +196,37 @@ def main():
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
     print(json.dumps(record), flush=True)
+
+
+def learned_vs_kcenters(ctx, X_kc, ms_kc, s, prec):
+    """Learned-pivot layout (network + greedy suppression + one multi-source
+    BFS) on the context's graph against its k-centers layout X_kc: time and
+    Hall energy gap tr(Y'LY)/tr(Y'DY)."""
+    from icpp_2020_parhde_amd import layout as L
+    from icpp_2020_parhde_amd.learned import DEFAULT_WEIGHTS
+    if not DEFAULT_WEIGHTS.exists():
+        log(f"[bench] no network weights at {DEFAULT_WEIGHTS}: learned comparison skipped")
+        return None
+    net = L.PivotNet(path=str(DEFAULT_WEIGHTS))
+    ctx.set_pivot_net(net)
+    opts = dict(s=s, num_coord=2, precision=prec, strategy=L.PIVOTS_LEARNED)
+    ctx.layout(**opts)
+    times = []
+    for _ in range(2):
+        t = time.perf_counter()
+        X, st = ctx.layout(**opts)
+        times.append((time.perf_counter() - t) * 1e3)
+    ms = float(np.mean(times))
+    e_l = ctx.hall_energy(X)
+    e_k = ctx.hall_energy(X_kc)
+    ctx.set_pivot_net(None)
+    net.close()
+    r = {"ms_per_layout": round(ms, 2), "ms_kcenters": round(ms_kc, 2), "speedup": round(ms_kc / ms, 3),
+         "ms_select": round(st.ms_select, 2), "ms_distance": round(st.ms_distance, 2),
+         "hall_energy": e_l, "hall_energy_kcenters": e_k, "gap": round(e_l / e_k - 1, 6),
+         "within_2pct": bool(e_l <= 1.02 * e_k), "n_candidates": st.n_candidates}
+    log(f"[bench] learned pivots: {r}")
+    return r
 
 
 def cpu_baseline(g, s):

@@ -120,13 +120,19 @@ struct BfsArgs {
   int *done_dev, *host_flags;
   int32_t *dummy;  // one scratch word per thread of the grid
   int backoff;     // barrier poll back-off (units of s_sleep 2)
+  int bstride;     // persistent kernel: only blocks with blockIdx % bstride == 0 work
 };
+
+// Block index and grid size of the working blocks (all of them unless the
+// persistent kernel runs on every bstride-th block only).
+__device__ __forceinline__ int64_t work_block(const BfsArgs &a) { return blockIdx.x / a.bstride; }
+__device__ __forceinline__ int64_t work_grid(const BfsArgs &a) { return gridDim.x / a.bstride; }
 
 // Initial state: every vertex unvisited, frontier = {src}.
 __device__ void bfs_init(const BfsArgs &a) {
   const int64_t n = a.g.n;
-  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
+  const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
+  const int64_t gsize = work_grid(a) * blockDim.x;
   for (int64_t i = gtid; i < n; i += gsize) st_dev(&a.dist[i], (i == a.src) ? 0 : kUnvisited);
   if (gtid == 0) {
     const int dsrc = a.g.deg32[a.src];
@@ -155,8 +161,8 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_init_kernel(BfsArgs a) { bfs_in
 // Fused epilogue: k-centers update + argmax, sum of squares, unreached.
 __device__ void bfs_epilogue(const BfsArgs &a) {
   const int64_t n = a.g.n;
-  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
+  const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
+  const int64_t gsize = work_grid(a) * blockDim.x;
   const int lane = __lane_id();
   unsigned long long key = 0, sumsq = 0, unreached = 0;
   for (int64_t i = gtid; i < n; i += gsize) {
@@ -216,8 +222,8 @@ __device__ __forceinline__ LevelCounts read_counts(const BfsSlot *cur) {
 // L is empty (the traversal is complete).
 __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
   const int64_t n = a.g.n;
-  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
+  const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
+  const int64_t gsize = work_grid(a) * blockDim.x;
   const int lane = __lane_id();
   const int64_t gwave = gtid / kWave;
   const int64_t nwaves = gsize / kWave;
@@ -443,8 +449,9 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_level_kernel(BfsArgs a, int L) 
 // barrier wait timed out.
 __global__ __launch_bounds__(kBfsBlock) void bfs_persistent_kernel(BfsArgs a, unsigned int *arrive,
                                                                    unsigned int *err) {
-  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const unsigned int nb = gridDim.x;
+  if (blockIdx.x % a.bstride) return;
+  const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
+  const unsigned int nb = (unsigned int)work_grid(a);
   unsigned int bar = 0;
   bfs_init(a);
   grid_barrier(arrive, nb * ++bar, err, a.backoff);
@@ -494,6 +501,10 @@ int bfs_setup(phde_ctx *c) {
     per_cu = 2;
   c->bar_backoff = 1;
   if (const char *e = getenv("PHDE_BAR_BACKOFF")) c->bar_backoff = std::max(0, atoi(e));
+  // work on every bstride-th block only (experiments with fewer XCDs:
+  // blocks are dealt to the XCDs round-robin)
+  c->bfs_stride = 1;
+  if (const char *e = getenv("PHDE_BFS_STRIDE")) c->bfs_stride = std::max(1, std::min(64, atoi(e)));
   c->bfs_grid = std::max(1, c->num_cus) * per_cu;
   const int64_t n = c->g.n;
   const int64_t nwords = (n + 63) / 64;
@@ -541,6 +552,7 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
   a.host_flags = c->d_flags;
   a.dummy = c->d_dummy;
   a.backoff = c->bar_backoff;
+  a.bstride = c->bfs_persistent ? c->bfs_stride : 1;
   c->h_flags[kFlagDone] = -1;
   c->h_flags[kFlagProgress] = -1;
   hipEvent_t ev = nullptr;

@@ -90,6 +90,7 @@ struct phde_ctx {
   int bfs_grid = 0;
   bool bfs_persistent = true;
   int bar_backoff = 1;
+  int bfs_stride = 1;  // persistent BFS: every bfs_stride-th block works
   unsigned int *d_bar = nullptr, *h_bar = nullptr;  // grid-barrier counter + error flag
   int32_t *d_dummy = nullptr;                       // per-thread scratch of the BFS claims
   // mapped host flags polled by the level driver (levels.h) and their
