@@ -50,18 +50,6 @@ __device__ __forceinline__ void st_qent(QEnt *q, QEnt v) {
   st_dev(p + 1, (unsigned long long)v.e);
 }
 
-__device__ __forceinline__ int wave_excl_scan(int x, int *total) {
-  const int lane = __lane_id();
-  int incl = x;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(incl, o);
-    if (lane >= o) incl += y;
-  }
-  *total = __shfl(incl, 63);
-  return incl - x;
-}
-
 // Append the lanes' `found` vertices (with row ranges) to the next frontier
 // (queue, huge list, degree sum).  Must be called by all 64 lanes.
 __device__ __forceinline__ void wave_append(bool found, uint32_t v, uint32_t b, uint32_t e,

@@ -75,4 +75,17 @@ __device__ __forceinline__ void grid_barrier(unsigned int *arrive, unsigned int 
   __syncthreads();
 }
 
+// Exclusive prefix sum over the wave; *total gets the sum of all lanes.
+__device__ __forceinline__ int wave_excl_scan(int x, int *total) {
+  const int lane = __lane_id();
+  int incl = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  *total = __shfl(incl, 63);
+  return incl - x;
+}
+
 }  // namespace phde

@@ -515,6 +515,9 @@ __global__ void net_bf16_copy_kernel(const float *p, uint16_t *wb) {
 
 // ------------------------------------------------------------------ selection
 
+constexpr int kSelGroup = 8;  // neighbours a lane claims per round trip
+constexpr int kSelRound = 8;  // picks per round of the selection
+
 struct SelArgs {
   DevGraph g;
   const float *score;
@@ -523,6 +526,8 @@ struct SelArgs {
   unsigned int *cnt;          // three rotating frontier counters
   unsigned long long *best;   // one argmax slot per pick (zero at launch)
   int64_t *picks;
+  int32_t *dummy;  // one scratch word per thread: target of masked-off claims
+  const int32_t *d12;  // the two feature traversals' levels (n each)
   int first, K, r, backoff;
 };
 
@@ -558,12 +563,17 @@ __device__ __forceinline__ void sel_push(bool push, uint32_t v, uint32_t *qn, un
   if (push) st_dev(&qn[base + __popcll(mask & lanemask_lt())], v);
 }
 
-// All picks in one cooperative launch.  Per pick: a grid-wide argmax of
-// (min(sd, r+1), score, -index) — vertices beyond r hops of every pick
-// first, then by score, ties to the lower index — then a BFS from the pick
-// bounded to r levels that only enters vertices whose sd it lowers (a path
-// through a vertex already at least as close to another pick cannot lower
-// anything behind it).
+// All picks in one cooperative launch, in rounds of up to kSelRound picks.
+// Within a round each pick maximises (min(sd, r+1), score, -index) over the
+// vertices that are provably more than r hops from the round's earlier
+// picks — max(|d1_v - d1_c|, |d2_v - d2_c|) > r, a lower bound of the hop
+// distance by the triangle inequality — where sd is the distance to the
+// picks of earlier rounds, exact within r hops.  A round ends early when no
+// such vertex is more than r hops from every earlier pick (a round's first
+// pick is always taken: then it is the vertex farthest from them).  After
+// each round one BFS from all its picks, bounded to r levels, lowers sd; it
+// only enters vertices whose sd it lowers (a path through a vertex already
+// at least as close to another pick cannot lower anything behind it).
 __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *arrive, unsigned int *err) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
@@ -574,79 +584,116 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *ar
   const int64_t n = a.g.n;
   const uint32_t *rowptr = a.g.rowptr, *adj = a.g.adj;
   const uint32_t rcap = (uint32_t)a.r + 1u;
+  const int32_t *d1 = a.d12, *d2 = a.d12 + n;
   unsigned int G = 0;  // running level counter: queue / counter rotation
-  for (int i = a.first; i < a.K; i++) {
-    if (gtid == 0) st_dev(&a.cnt[(G + 1) % 3], 0u);
-    unsigned long long best = 0;
-    for (int64_t v = gtid; v < n; v += gsize) {
-      const uint32_t sdc = min((uint32_t)ld_dev(&a.sd[v]), rcap);
-      const uint32_t hi = (sdc << 21) | (orderable(a.score[v]) >> 11);
-      const unsigned long long key = ((unsigned long long)hi << 32) | (0xffffffffull - (uint64_t)v);
-      best = key > best ? key : best;
+  int i = a.first;
+  while (i < a.K) {
+    // ---- picks of one round
+    int rp[kSelRound], r1[kSelRound], r2[kSelRound];
+    int nr = 0;
+    while (nr < kSelRound && i < a.K) {
+      unsigned long long best = 0;
+      for (int64_t v = gtid; v < n; v += gsize) {
+        bool far = true;
+        if (nr) {
+          const int x1 = d1[v], x2 = d2[v];
+          for (int j = 0; j < nr; j++) far = far && max(abs(x1 - r1[j]), abs(x2 - r2[j])) > a.r;
+        }
+        if (!far) continue;
+        const uint32_t sdc = min((uint32_t)ld_dev(&a.sd[v]), rcap);
+        const uint32_t hi = (sdc << 21) | (orderable(a.score[v]) >> 11);
+        const unsigned long long key = ((unsigned long long)hi << 32) | (0xffffffffull - (uint64_t)v);
+        best = key > best ? key : best;
+      }
+      best = wave_max_u64(best);
+      if (lane == 0 && best) atomicMax(&a.best[i], best);
+      grid_barrier(arrive, nb * ++bar, err, a.backoff);
+      if (ld_dev(err)) return;
+      const unsigned long long bk = ld_dev(&a.best[i]);
+      const bool avail = (uint32_t)(bk >> 53) == rcap;  // more than r hops from every earlier pick
+      if (nr > 0 && !avail) {                           // (bk == 0: nothing far enough either)
+        if (gtid == 0) st_dev(&a.best[i], 0ull);        // slot i is argmaxed again next round
+        break;
+      }
+      const uint32_t p = (uint32_t)(0xffffffffull - (bk & 0xffffffffull));
+      rp[nr] = (int)p;
+      r1[nr] = d1[p];
+      r2[nr] = d2[p];
+      nr++;
+      if (gtid == 0) a.picks[i] = p;
+      i++;
     }
-    best = wave_max_u64(best);
-    if (lane == 0 && best) atomicMax(&a.best[i], best);
+    // ---- bounded BFS from the round's picks
+    uint32_t *q0 = (G & 1) ? a.q1 : a.q0;
+    if (gtid == 0) {
+      for (int j = 0; j < nr; j++) {
+        st_dev(&a.sd[rp[j]], 0);
+        st_dev(&q0[j], (uint32_t)rp[j]);
+      }
+      st_dev(&a.cnt[G % 3], (unsigned int)nr);
+      st_dev(&a.cnt[(G + 1) % 3], 0u);
+    }
     grid_barrier(arrive, nb * ++bar, err, a.backoff);
     if (ld_dev(err)) return;
-    const unsigned long long bk = ld_dev(&a.best[i]);
-    const uint32_t p = (uint32_t)(0xffffffffull - (bk & 0xffffffffull));
-    if (gtid == 0) {
-      a.picks[i] = p;
-      st_dev(&a.sd[p], 0);
-    }
     for (int l = 0; l < a.r; l++) {
       if (gtid == 0) st_dev(&a.cnt[(G + 2) % 3], 0u);
       const int nd = l + 1;
       uint32_t *qn = (G & 1) ? a.q0 : a.q1;
       unsigned int *cn = &a.cnt[(G + 1) % 3];
-      if (l == 0) {
-        const uint32_t b = rowptr[p], e = rowptr[p + 1];
-        for (int64_t k0 = (int64_t)b + gwave * kWave; k0 < (int64_t)e; k0 += nwaves * kWave) {
-          const int64_t k = k0 + lane;
-          bool push = false;
-          uint32_t w = 0;
-          if (k < (int64_t)e) {
-            w = adj[k];
-            push = sel_claim(a.sd, w, nd);
-          }
-          sel_push(push, w, qn, cn);
-        }
-      } else {
-        const uint32_t *qc = (G & 1) ? a.q1 : a.q0;
-        const unsigned int nq = ld_dev(&a.cnt[G % 3]);
-        for (int64_t base = gwave * kWave; base < (int64_t)nq; base += nwaves * kWave) {
-          const int64_t idx = base + lane;
-          const bool act = idx < (int64_t)nq;
-          const uint32_t u = act ? ld_dev(&qc[idx]) : 0u;
-          const uint32_t b = act ? rowptr[u] : 0u, e = act ? rowptr[u + 1] : 0u;
-          const int d = (int)(e - b);
-          const bool big = d >= 32;
-          unsigned long long bmask = __ballot(big);
-          while (bmask) {
-            const int l2 = __ffsll((long long)bmask) - 1;
-            bmask &= bmask - 1;
-            const uint32_t bb = __shfl(b, l2), be = __shfl(e, l2);
-            for (uint32_t k0 = bb; k0 < be; k0 += kWave) {
-              const uint32_t k = k0 + lane;
-              bool push = false;
-              uint32_t w = 0;
-              if (k < be) {
-                w = adj[k];
-                push = sel_claim(a.sd, w, nd);
-              }
-              sel_push(push, w, qn, cn);
-            }
-          }
-          const int ds = big ? 0 : d;
-          const int maxd = wave_max_i32(ds);
-          for (int t = 0; t < maxd; t++) {
+      const uint32_t *qc = (G & 1) ? a.q1 : a.q0;
+      const unsigned int nq = ld_dev(&a.cnt[G % 3]);
+      for (int64_t base = gwave * kWave; base < (int64_t)nq; base += nwaves * kWave) {
+        const int64_t idx = base + lane;
+        const bool act = idx < (int64_t)nq;
+        const uint32_t u = act ? ld_dev(&qc[idx]) : 0u;
+        const uint32_t b = act ? rowptr[u] : 0u, e = act ? rowptr[u + 1] : 0u;
+        const int d = (int)(e - b);
+        const bool big = d >= 32;
+        unsigned long long bmask = __ballot(big);
+        while (bmask) {
+          const int l2 = __ffsll((long long)bmask) - 1;
+          bmask &= bmask - 1;
+          const uint32_t bb = __shfl(b, l2), be = __shfl(e, l2);
+          for (uint32_t k0 = bb; k0 < be; k0 += kWave) {
+            const uint32_t k = k0 + lane;
             bool push = false;
             uint32_t w = 0;
-            if (t < ds) {
-              w = adj[b + t];
+            if (k < be) {
+              w = adj[k];
               push = sel_claim(a.sd, w, nd);
             }
             sel_push(push, w, qn, cn);
+          }
+        }
+        // small rows: kSelGroup neighbours per lane per round trip, loads
+        // and claims issued before use, masked-off claims to scratch
+        const int ds = big ? 0 : d;
+        const int maxd = wave_max_i32(ds);
+        int32_t *dum = a.dummy + gtid;
+        for (int t0 = 0; t0 < maxd; t0 += kSelGroup) {
+          uint32_t vv[kSelGroup];
+          int32_t cur[kSelGroup];
+#pragma unroll
+          for (int j = 0; j < kSelGroup; j++) vv[j] = (t0 + j < ds) ? adj[b + t0 + j] : 0u;
+#pragma unroll
+          for (int j = 0; j < kSelGroup; j++) cur[j] = ld_dev(&a.sd[vv[j]]);
+#pragma unroll
+          for (int j = 0; j < kSelGroup; j++) cur[j] = (t0 + j < ds && cur[j] > nd) ? 1 : 0;
+#pragma unroll
+          for (int j = 0; j < kSelGroup; j++)
+            cur[j] = cur[j] ? (atomicMin(&a.sd[vv[j]], nd) > nd ? 1 : 0) : (atomicMin(dum, nd), 0);
+          int npush = 0;
+#pragma unroll
+          for (int j = 0; j < kSelGroup; j++) npush += cur[j];
+          int total = 0;
+          const int off = wave_excl_scan(npush, &total);
+          if (total) {
+            unsigned int base2 = 0;
+            if (lane == 0) base2 = atomicAdd(cn, (unsigned int)total);
+            base2 = __shfl(base2, 0) + (unsigned int)off;
+#pragma unroll
+            for (int j = 0; j < kSelGroup; j++)
+              if (cur[j]) st_dev(&qn[base2++], vv[j]);
           }
         }
       }
@@ -655,6 +702,7 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *ar
       if (ld_dev(err)) return;
       if (ld_dev(&a.cnt[G % 3]) == 0) break;
     }
+    G++;  // the next round's picks go to a fresh queue
   }
 }
 
@@ -814,6 +862,8 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
   a.cnt = reinterpret_cast<unsigned int *>(c->d_slots);
   a.best = best.p;
   a.picks = pk.p;
+  a.dummy = c->d_dummy;
+  a.d12 = c->lp_d12;
   a.first = first;
   a.K = K;
   a.r = std::max(1, std::min(r, 2046));

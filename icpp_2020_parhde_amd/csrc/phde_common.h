@@ -110,6 +110,7 @@ struct phde_ctx {
   unsigned long long *ms_visit = nullptr, *ms_fr[3] = {nullptr, nullptr, nullptr};
   uint32_t *ms_q3 = nullptr;
   int64_t *ms_src = nullptr;
+  unsigned long long *ms_dummy = nullptr;
   // learned pivot proposal (learned.hip): features (n float4), the two
   // feature traversals (2n), distance-to-pivots field, scores, training target
   float *lp_feat = nullptr;

@@ -24,6 +24,7 @@
 #include <climits>
 #include <cstdio>
 
+#include "device_util.h"
 #include "phde_common.h"
 #include "levels.h"
 #include "traverse.h"
@@ -275,7 +276,10 @@ struct MsArgs {
   const int64_t *src;
   int32_t *D;  // n*k column-major
   int *done_dev, *host_flags;
+  unsigned long long *dummy;  // one scratch word per thread: target of masked-off atomics
 };
+
+constexpr int kMsGroup = 8;  // neighbours a lane handles per round trip
 
 __global__ __launch_bounds__(kTBlock) void msbfs_fill_kernel(MsArgs a) {
   const int64_t n = a.g.n;
@@ -294,32 +298,26 @@ __global__ __launch_bounds__(kTBlock) void msbfs_fill_kernel(MsArgs a) {
   }
 }
 
+// Level 0 frontier: each source's bit in its pushed mask (visit and the
+// level-0 distance are written when level 0 processes the queue).
 __global__ void msbfs_source_kernel(MsArgs a) {
   const int t = threadIdx.x;
   if (t < a.k) {
     const int64_t s = a.src[t];
-    atomicOr(&a.visit[s], 1ull << t);
     const unsigned long long old = atomicOr(&a.fr[0][s], 1ull << t);
-    a.D[(int64_t)t * a.g.n + s] = 0;
     if (old == 0) a.q[0][atomicAdd(&a.cnt[0], 1u)] = (uint32_t)s;
   }
 }
 
+// Offer `bits` to v for the next level: one atomic, and true for the offer
+// that made v's pushed mask non-empty (v joins the next queue).  Bits v
+// already holds may still be offered (visit is read without ordering); the
+// next level filters them.
 __device__ __forceinline__ bool ms_push(const MsArgs &a, unsigned long long bits, uint32_t v,
-                                        unsigned long long *fn, int next_level) {
+                                        unsigned long long *fn) {
   const unsigned long long nb = bits & ~a.visit[v];
   if (!nb) return false;
-  const unsigned long long old = atomicOr(&a.visit[v], nb);
-  const unsigned long long nw = nb & ~old;
-  if (!nw) return false;
-  const unsigned long long pf = atomicOr(&fn[v], nw);
-  unsigned long long t = nw;
-  while (t) {
-    const int s = __ffsll((long long)t) - 1;
-    t &= t - 1;
-    a.D[(int64_t)s * a.g.n + v] = next_level;
-  }
-  return pf == 0;
+  return atomicOr(&fn[v], nb) == 0ull;
 }
 
 __global__ __launch_bounds__(kTBlock) void msbfs_level_kernel(MsArgs a, int L) {
@@ -350,12 +348,35 @@ __global__ __launch_bounds__(kTBlock) void msbfs_level_kernel(MsArgs a, int L) {
   const unsigned long long *fc = a.fr[L % 3];
   unsigned long long *fn = a.fr[(L + 1) % 3];
   unsigned int *cnext = &a.cnt[(L + 1) % 4];
-  for (int64_t base = gwave * kWave; base < (int64_t)nq; base += nwaves * kWave) {
-    const int64_t idx = base + lane;
+  // Next-frontier appends of the small rows are staged in LDS and published
+  // with one global atomic per block and chunk (a chunk of kTBlock queue
+  // entries with degree < 32 appends at most kTBlock * 31 vertices); the
+  // wave-cooperative big rows append to the global queue directly.
+  __shared__ uint32_t stage[kTBlock * 31];
+  __shared__ unsigned int nstage, gbase;
+  if (threadIdx.x == 0) nstage = 0;
+  const int w = threadIdx.x / kWave;
+  (void)gwave;
+  (void)nwaves;
+  for (int64_t cb = (int64_t)blockIdx.x * kTBlock; cb < (int64_t)nq; cb += (int64_t)gridDim.x * kTBlock) {
+    __syncthreads();
+    const int64_t idx = cb + w * kWave + lane;
     const bool act = idx < (int64_t)nq;
     const uint32_t u = act ? qc[idx] : 0u;
-    const unsigned long long bits = act ? fc[u] : 0ull;
-    const uint32_t b = act ? rowptr[u] : 0u, e = act ? rowptr[u + 1] : 0u;
+    // the bits that reach u first at this level: its distance for them is L
+    const unsigned long long pushed = act ? fc[u] : 0ull;
+    const unsigned long long vis = act ? a.visit[u] : ~0ull;
+    const unsigned long long bits = pushed & ~vis;
+    if (bits) {
+      a.visit[u] = vis | bits;  // u occurs once in the queue: sole writer
+      unsigned long long t = bits;
+      while (t) {
+        const int s = __ffsll((long long)t) - 1;
+        t &= t - 1;
+        a.D[(int64_t)s * a.g.n + u] = L;
+      }
+    }
+    const uint32_t b = bits ? rowptr[u] : 0u, e = bits ? rowptr[u + 1] : 0u;
     const int d = (int)(e - b);
     const bool big = d >= 32;
     unsigned long long bmask = __ballot(big);
@@ -370,23 +391,52 @@ __global__ __launch_bounds__(kTBlock) void msbfs_level_kernel(MsArgs a, int L) {
         uint32_t v = 0;
         if (k < be) {
           v = adj[k];
-          push = ms_push(a, bbits, v, fn, L + 1);
+          push = ms_push(a, bbits, v, fn);
         }
         wave_push(push, v, qn, cnext);
       }
     }
+    // Small rows: kMsGroup neighbours per lane per round trip, every load
+    // and atomic of the group issued before any result is used; masked-off
+    // atomics go to the lane's own scratch word instead of being predicated.
     const int ds = big ? 0 : d;
     int maxd = ds;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) maxd = max(maxd, __shfl_xor(maxd, o));
-    for (int t = 0; t < maxd; t++) {
-      bool push = false;
-      uint32_t v = 0;
-      if (t < ds) {
-        v = adj[b + t];
-        push = ms_push(a, bits, v, fn, L + 1);
+    unsigned long long *dummy = a.dummy + gtid;
+    for (int t0 = 0; t0 < maxd; t0 += kMsGroup) {
+      uint32_t vv[kMsGroup];
+      unsigned long long nb[kMsGroup], old[kMsGroup];
+#pragma unroll
+      for (int j = 0; j < kMsGroup; j++) vv[j] = (t0 + j < ds) ? adj[b + t0 + j] : 0u;
+#pragma unroll
+      for (int j = 0; j < kMsGroup; j++) nb[j] = a.visit[vv[j]];
+#pragma unroll
+      for (int j = 0; j < kMsGroup; j++) nb[j] = (t0 + j < ds) ? (bits & ~nb[j]) : 0ull;
+#pragma unroll
+      for (int j = 0; j < kMsGroup; j++) old[j] = atomicOr(nb[j] ? &fn[vv[j]] : dummy, nb[j]);
+      int npush = 0;
+#pragma unroll
+      for (int j = 0; j < kMsGroup; j++) npush += (nb[j] != 0ull && old[j] == 0ull) ? 1 : 0;
+      int total = 0;
+      const int off = wave_excl_scan(npush, &total);
+      if (total) {
+        unsigned int base = 0;
+        if (lane == 0) base = atomicAdd(&nstage, (unsigned int)total);
+        base = __shfl(base, 0) + (unsigned int)off;
+#pragma unroll
+        for (int j = 0; j < kMsGroup; j++)
+          if (nb[j] != 0ull && old[j] == 0ull) stage[base++] = vv[j];
       }
-      wave_push(push, v, qn, cnext);
+    }
+    __syncthreads();
+    const unsigned int ns = nstage;
+    if (ns) {
+      if (threadIdx.x == 0) gbase = atomicAdd(cnext, ns);
+      __syncthreads();
+      for (unsigned int i = threadIdx.x; i < ns; i += kTBlock) qn[gbase + i] = stage[i];
+      __syncthreads();
+      if (threadIdx.x == 0) nstage = 0;
     }
   }
 }
@@ -449,6 +499,17 @@ int sssp_run(phde_ctx *c, int64_t src, int32_t delta, int32_t *dist_out, int32_t
   return PHDE_OK;
 }
 
+// Grid of the multi-source sweep: two 256-thread blocks per CU measured
+// best on RGG n=16.7M with 62 sources (455 ms vs 480 / 466 / 517 ms at one,
+// four and eight blocks per CU).
+static int ms_grid(const phde_ctx *c) {
+  static const int per_cu = [] {
+    const char *e = getenv("PHDE_MS_BLOCKS_PER_CU");
+    return e ? std::max(1, std::min(32, atoi(e))) : 2;
+  }();
+  return std::max(1, c->num_cus) * per_cu;
+}
+
 int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
   int rc = bfs_setup(c);
   if (rc) return rc;
@@ -458,6 +519,7 @@ int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
     for (int i = 0; i < 3; i++) PHDE_HIP(hipMalloc(&c->ms_fr[i], sizeof(unsigned long long) * n));
     PHDE_HIP(hipMalloc(&c->ms_q3, sizeof(uint32_t) * (n + 1)));
     PHDE_HIP(hipMalloc(&c->ms_src, sizeof(int64_t) * 64));
+    PHDE_HIP(hipMalloc(&c->ms_dummy, sizeof(unsigned long long) * (size_t)ms_grid(c) * kTBlock));
     if (!c->ts_cnt) PHDE_HIP(hipMalloc(&c->ts_cnt, 256));
   }
   PHDE_HIP(hipMemcpyAsync(c->ms_src, sources, sizeof(int64_t) * k, hipMemcpyHostToDevice, c->stream));
@@ -474,11 +536,12 @@ int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
   a.D = D;
   a.done_dev = c->d_done;
   a.host_flags = c->d_flags;
+  a.dummy = c->ms_dummy;
   c->h_flags[kFlagDone] = -1;
   c->h_flags[kFlagProgress] = -1;
   hipEvent_t ev;
   c->timer.begin(c->stream, PHDE_KERNEL_MSBFS, &ev);
-  const int grid = c->bfs_grid;
+  const int grid = ms_grid(c);
   msbfs_fill_kernel<<<grid, kTBlock, 0, c->stream>>>(a);
   msbfs_source_kernel<<<1, 64, 0, c->stream>>>(a);
   PHDE_HIP(hipGetLastError());
@@ -495,7 +558,7 @@ int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
 void traverse_release(phde_ctx *c) {
   void *p[] = {c->ts_far[0], c->ts_far[1], c->ts_nstamp, c->ts_fstamp, c->ts_cnt,
                c->ms_visit, c->ms_fr[0], c->ms_fr[1], c->ms_fr[2], c->ms_q3, c->ms_src,
-               c->d_done};
+               c->d_done, c->ms_dummy};
   if (c->h_flags) (void)hipHostFree(c->h_flags);
   for (void *x : p)
     if (x) (void)hipFree(x);

@@ -13,8 +13,9 @@ module restates the GPU design so that the GPU kernels can be checked:
                              rounded to bfloat16 at the points where the GPU
                              stores BF16 (activations, neighbour means, MFMA
                              operands), everything else in float64
-  greedy_select              the greedy graph non-maximum suppression, exact
-                             integer semantics (bit-exact target for the GPU)
+  greedy_select              the greedy graph non-maximum suppression in
+                             rounds, exact integer semantics (bit-exact target
+                             for the GPU)
   coverage_target            the training target 1 - md / max(md) from the
                              k-centers distance-to-nearest-pivot field
 
@@ -130,23 +131,45 @@ def orderable(score):
     return np.where(u & 0x80000000, (~u) & 0xFFFFFFFF, u | 0x80000000)
 
 
+ROUND = 8  # picks per round (learned.hip kSelRound)
+
+
 def greedy_select(g, score, d1, d2, first, second, k, r):
-    """Pivots of learned.hip's select_kernel: picks 0 and 1 are given (the
-    feature traversals' sources); each further pick maximises
-    (min(sd, r+1), top 21 bits of the score's order key, -index), where sd
-    is the distance to the nearest pick so far, exact within r hops."""
+    """Pivots of learned.hip's select_kernel.  Picks 0 and 1 are given (the
+    feature traversals' sources).  The rest come in rounds of up to ROUND:
+    each pick maximises (min(sd, r+1), top 21 bits of the score's order key,
+    -index) over the vertices v with max(|d1_v - d1_c|, |d2_v - d2_c|) > r
+    for every earlier pick c of the same round, where sd is the distance to
+    the picks of earlier rounds (exact within r hops; the two given picks
+    exactly).  A round ends when the best such vertex has sd <= r (its first
+    pick is always taken).  After a round, sd = min(sd, distance to the
+    round's picks where that is <= r)."""
     A, _, _ = adjacency(g)
     r = max(1, min(int(r), 2046))
-    sd = np.minimum(d1.astype(np.int64), d2.astype(np.int64))
+    d1 = np.asarray(d1, dtype=np.int64)
+    d2 = np.asarray(d2, dtype=np.int64)
+    sd = np.minimum(d1, d2)
     hi = orderable(score) >> np.uint64(11)
     idx = np.arange(g.n, dtype=np.uint64)
     picks = [int(first), int(second)]
-    for _ in range(2, k):
-        sdc = np.minimum(sd, r + 1).astype(np.uint64)
-        key = (((sdc << np.uint64(21)) | hi) << np.uint64(32)) | (np.uint64(0xFFFFFFFF) - idx)
-        p = int(np.uint64(0xFFFFFFFF) - (key.max() & np.uint64(0xFFFFFFFF)))
-        picks.append(p)
-        dp = dijkstra(A, unweighted=True, indices=p, limit=r + 0.5)
+    while len(picks) < k:
+        rnd = []
+        while len(rnd) < ROUND and len(picks) < k:
+            far = np.ones(g.n, dtype=bool)
+            for c in rnd:
+                far &= np.maximum(np.abs(d1 - d1[c]), np.abs(d2 - d2[c])) > r
+            if not far.any():
+                break
+            sdc = np.minimum(sd, r + 1).astype(np.uint64)
+            key = (((sdc << np.uint64(21)) | hi) << np.uint64(32)) | (np.uint64(0xFFFFFFFF) - idx)
+            key = np.where(far, key, np.uint64(0))
+            best = key.max()
+            if rnd and int(best >> np.uint64(53)) != r + 1:
+                break
+            p = int(np.uint64(0xFFFFFFFF) - (best & np.uint64(0xFFFFFFFF)))
+            rnd.append(p)
+            picks.append(p)
+        dp = dijkstra(A, unweighted=True, indices=rnd, limit=r + 0.5, min_only=True)
         reach = np.isfinite(dp)
         sd[reach] = np.minimum(sd[reach], dp[reach].astype(np.int64))
     return np.array(picks, dtype=np.int64)

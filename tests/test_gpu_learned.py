@@ -221,3 +221,18 @@ def test_union_trains_like_parts(gpu):
             ctx.set_training_data(np.concatenate([ys[0][0], ys[1][0]]), np.zeros(u.n, np.float32))
             yu = net.scores(ctx)
     np.testing.assert_allclose(yu, np.concatenate([ys[0][1], ys[1][1]]), rtol=1e-6, atol=1e-6)
+
+
+def test_trained_network_gains_over_no_network(gpu):
+    """What the trained network adds: on held-out graphs (seeds outside the
+    training range), the layout from the trained network's scores has a
+    lower Hall energy, relative to k-centers, than the same selection on
+    constant scores (no network); both stay within the 2% bar."""
+    held = [Graph.rgg(60000, 10.0, seed=6001), Graph.holed_grid(220, 220, 10, 6002),
+            Graph.rgg(90000, 10.0, seed=6003)]
+    with L.PivotNet(path=str(LP.DEFAULT_WEIGHTS)) as net:
+        gaps = [(LP.layout_gap(g, net)["gap"], LP.layout_gap(g, None)["gap"]) for g in held]
+    for (gt, g0), g in zip(gaps, held):
+        print(f"n={g.n}: trained {100 * gt:+.3f}%, no network {100 * g0:+.3f}%")
+    assert all(gt <= 0.02 for gt, _ in gaps)
+    assert np.mean([gt for gt, _ in gaps]) < np.mean([g0 for _, g0 in gaps])

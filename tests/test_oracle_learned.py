@@ -67,20 +67,29 @@ def test_emulated_gradient_close_to_exact():
 
 
 def _brute_select(g, score, first, second, k, r):
+    """The round-wise selection restated from all-pairs distances."""
     D = shortest_path(OL.adjacency(g)[0], unweighted=True).astype(np.int64)
     r = max(1, min(r, 2046))
     hi = OL.orderable(score) >> np.uint64(11)
-    picks = [first, second]
-    for _ in range(2, k):
+    picks, rounds = [first, second], []
+    while len(picks) < k:
         sd = np.minimum(D[first], D[second])
-        for q in picks[2:]:
+        for q in rounds:
             sd = np.where(D[q] <= r, np.minimum(sd, D[q]), sd)
-        best = None
-        for v in range(g.n):
-            key = (min(int(sd[v]), r + 1), int(hi[v]), -v)
-            if best is None or key > best[0]:
-                best = (key, v)
-        picks.append(best[1])
+        rnd = []
+        while len(rnd) < OL.ROUND and len(picks) < k:
+            best = None
+            for v in range(g.n):
+                if any(max(abs(D[first, v] - D[first, c]), abs(D[second, v] - D[second, c])) <= r for c in rnd):
+                    continue
+                key = (min(int(sd[v]), r + 1), int(hi[v]), -v)
+                if best is None or key > best[0]:
+                    best = (key, v)
+            if best is None or (rnd and best[0][0] != r + 1):
+                break
+            rnd.append(best[1])
+            picks.append(best[1])
+        rounds += rnd
     return np.array(picks)
 
 
