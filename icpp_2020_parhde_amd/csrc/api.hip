@@ -896,8 +896,7 @@ int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats
       }
     }
   } else if (hde) {
-    rc = random ? layout_hde_random(c, &o, coords, out, stats != nullptr)
-                : layout_hde(c, &o, coords, out, stats != nullptr);
+    rc = random ? layout_hde_random(c, &o, coords, out, false) : layout_hde(c, &o, coords, out, false);
   } else {
     rc = layout_mds(c, &o, coords, out);
   }

@@ -198,13 +198,37 @@ __global__ void scale_cols_kernel(double *X, int64_t n, int c, const double *inv
 __global__ __launch_bounds__(kBlk) void hall_kernel(DevGraph g, const double *Y, int c, double *part) {
   __shared__ double sh[kBlk / kWave][2];
   double num = 0, den = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kBlk + threadIdx.x; i < g.n; i += (int64_t)gridDim.x * kBlk) {
+  // one vertex per lane; rows of 64 or more neighbours (hubs) are summed by
+  // the whole wave so that no lane walks a long row alone
+  const int lane0 = threadIdx.x % kWave;
+  const int64_t stride = (int64_t)gridDim.x * kBlk;
+  for (int64_t i0 = (int64_t)blockIdx.x * kBlk + (threadIdx.x - lane0); i0 < g.n; i0 += stride) {
+    const int64_t i = i0 + lane0;
+    const bool act = i < g.n;
+    const uint32_t b = act ? g.rowptr[i] : 0u, e = act ? g.rowptr[i + 1] : 0u;
+    const bool big = e - b >= (uint32_t)kWave;
+    unsigned long long bmask = __ballot(big);
+    while (bmask) {
+      const int l = __ffsll((long long)bmask) - 1;
+      bmask &= bmask - 1;
+      const int64_t v = __shfl(i, l);
+      const uint32_t bb = __shfl(b, l), be = __shfl(e, l);
+      for (int k = 0; k < c; k++) {
+        const double yv = Y[v + (int64_t)k * g.n];
+        for (uint32_t x = bb + lane0; x < be; x += kWave) {
+          const double d = yv - Y[g.adj[x] + (int64_t)k * g.n];
+          num += 0.5 * (g.w ? (double)g.w[x] : 1.0) * d * d;
+        }
+      }
+    }
+    if (!act) continue;
     for (int k = 0; k < c; k++) {
       const double yi = Y[i + (int64_t)k * g.n];
-      for (uint32_t e = g.rowptr[i]; e < g.rowptr[i + 1]; e++) {
-        const double d = yi - Y[g.adj[e] + (int64_t)k * g.n];
-        num += 0.5 * (g.w ? (double)g.w[e] : 1.0) * d * d;
-      }
+      if (!big)
+        for (uint32_t x = b; x < e; x++) {
+          const double d = yi - Y[g.adj[x] + (int64_t)k * g.n];
+          num += 0.5 * (g.w ? (double)g.w[x] : 1.0) * d * d;
+        }
       den += g.degw[i] * yi * yi;
     }
   }

@@ -92,7 +92,7 @@ typedef struct phde_stats {
   int64_t kernel_launches[16];
   /* learned strategy */
   double ms_select;            /* features, network scores and pivot selection */
-  double hall_energy;          /* tr(Y^T L Y) / tr(Y^T D Y) of the returned layout */
+  double hall_energy;          /* tr(Y^T L Y) / tr(Y^T D Y) of the returned layout (learned strategy) */
   double hall_energy_kcenters; /* of the k-centers layout when judged, else 0 */
   int32_t fallback;            /* 1 when the k-centers layout was returned instead */
   int32_t n_candidates;        /* pivots proposed by the selection */
