@@ -131,11 +131,16 @@ DrawingBase::DrawingBase(unsigned int *rowPtrs, unsigned int *adj, int32_t *weig
   phde_default_options(&opts);
   opts.s = numSources;
   opts.num_coord = numCoord;
-  opts.strategy = doRand == 2 ? PHDE_PIVOTS_KCENTERS
-                              : (doRand == 1 ? PHDE_PIVOTS_RANDFINE : PHDE_PIVOTS_RANDCOARSE);
+  // doRand as spectralDrawing.cpp:379-391 maps -p (0 RandCoarse, 1 RandFine,
+  // 2 Kcenters); 3 selects the learned proposal (setPivotNet)
+  opts.strategy = doRand == 3   ? PHDE_PIVOTS_LEARNED
+                  : doRand == 2 ? PHDE_PIVOTS_KCENTERS
+                                : (doRand == 1 ? PHDE_PIVOTS_RANDFINE : PHDE_PIVOTS_RANDCOARSE);
   int rc = phde_create(rowPtrs, adj, weights, n, m, &ctx);
   if (rc) die("graph upload", rc, 1);
 }
+
+int DrawingBase::setPivotNet(phde_mpnn *net) { return phde_set_pivot_net(ctx, net); }
 
 DrawingBase::~DrawingBase() {
   free(eigenVectors);

@@ -8,6 +8,7 @@
 // output written with default ostream formatting (:41-65), same progress
 // lines.  The layout runs on the GPU through the DrawingBase classes.
 #include <getopt.h>
+#include <unistd.h>
 
 #include <chrono>
 #include <cstdio>
@@ -45,7 +46,9 @@ int main(int argc, char **argv) {
   int weighted = 0, numCoord = 2, doRand = 2, drawingAlgorithm = 3, numSources = 10;
   int precision = PHDE_PREC_FP64, eig = PHDE_EIG_POWER, eager = 0;
   int opt;
-  while ((opt = getopt(argc, argv, "w:hc:a:p:r:f:e:E")) != -1) {
+  int judge = 0;
+  const char *netPath = nullptr;
+  while ((opt = getopt(argc, argv, "w:hc:a:p:r:f:e:EJn:")) != -1) {
     switch (opt) {
       case 'w':
         weighted = 1;
@@ -60,6 +63,8 @@ int main(int argc, char **argv) {
           doRand = 0;
         else if (!strcmp(optarg, "Kcenters"))
           doRand = 2;
+        else if (!strcmp(optarg, "Learned"))
+          doRand = 3;
         else {
           std::cerr << "Unrecognized Strategy" << std::endl;
           abort();
@@ -93,17 +98,27 @@ int main(int argc, char **argv) {
       case 'E':
         eager = 1;
         break;
+      case 'J':
+        judge = 1;
+        break;
+      case 'n':
+        netPath = optarg;
+        break;
       case 'h':
         std::cout << "Usage: " << argv[0]
                   << " [csr filename]\n"
                      "-w         : Indicates if CSR file is weighted or unweighted (default)\n"
                      "-c <value> : number of coordinates to generate( default=2)\n"
                      "-a <algo>  : Indicate the algorithm to run (PivotMDS,PHDE, wParHDE, ParHDE(default))\n"
-                     "-p <strat> : Pivot picking strategy to use (RandFine,RandCoarse,Kcenters(default))\n"
+                     "-p <strat> : Pivot picking strategy to use (RandFine,RandCoarse,Kcenters(default),Learned)\n"
                      "-r <value> : Number of pivots (default=10)"
                      "-f <prec>  : fp64 (default) or fp32 orthogonalisation / SpMM\n"
                      "-e <eig>   : power (default), jacobi or generalized eigensolver\n"
                      "-E         : eager traversals (3s, as the reference)\n"
+                     "-n <file>  : pivot network weights for -p Learned (default: weights/pivot_net.bin\n"
+                     "             next to the library)\n"
+                     "-J         : with -p Learned, also run k-centers and keep it when its Hall\n"
+                     "             energy is more than 2% lower\n"
                      "-h         : help"
                   << std::endl;
         exit(1);
@@ -159,6 +174,28 @@ int main(int argc, char **argv) {
   draw->options().eig_mode = eig;
   draw->options().eager = eager;
   draw->options().verbose = 1;
+  phde_mpnn *net = nullptr;
+  if (doRand == 3) {
+    std::string path;
+    if (netPath) {
+      path = netPath;
+    } else {
+      // bin/embed -> ../weights/pivot_net.bin
+      char exe[4096];
+      const ssize_t len = readlink("/proc/self/exe", exe, sizeof(exe) - 1);
+      exe[len > 0 ? len : 0] = 0;
+      path = std::string(exe);
+      path = path.substr(0, path.find_last_of('/')) + "/../weights/pivot_net.bin";
+    }
+    int rc = phde_net_create(0, 1, &net);
+    if (rc == PHDE_OK) rc = phde_net_load(net, path.c_str());
+    if (rc != PHDE_OK) {
+      std::cerr << "Cannot load the pivot network (" << path << "): " << phde_last_error() << std::endl;
+      return 1;
+    }
+    draw->setPivotNet(net);
+    draw->options().judge = judge;
+  }
   draw->run(1);
   double *coordinates = draw->getCoordinates();
   writeCoords(n, coordinates, numCoord, inputFilename);
@@ -167,5 +204,9 @@ int main(int argc, char **argv) {
   if (weighted) free(weights);
   elt = std::chrono::high_resolution_clock::now() - startTimer;
   std::cout << "Overall time: " << elt.count() << " s." << std::endl;
+  if (net) {
+    delete draw;  // the context that borrows the network goes first
+    phde_net_destroy(net);
+  }
   return 0;
 }

@@ -10,7 +10,8 @@
 //    reference's fatal paths keep their exit codes (exit(0) for weight and
 //    pivot errors, exit(1) for eigensolver failure);
 //  - options() exposes the GPU choices (precision, eigen mode, eager/lazy
-//    traversals); stats() the per-phase times of the last try.
+//    traversals, learned pivots); stats() the per-phase times of the last
+//    try; setPivotNet() attaches the learned-pivot network.
 #ifndef _DRAWING_BASE_H_
 #define _DRAWING_BASE_H_
 #include <limits.h>
@@ -67,6 +68,9 @@ class DrawingBase {
   virtual int run(int tries = 11, int alpha = 15, int beta = 18, int32_t delta = 1) = 0;
 
   phde_options &options() { return opts; }
+  // Network used by the learned pivot strategy (doRand 3 or
+  // options().strategy = PHDE_PIVOTS_LEARNED); borrowed.
+  int setPivotNet(phde_mpnn *net);
   const phde_stats &stats() const { return lastStats; }
 };
 
