@@ -876,7 +876,7 @@ int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats
     ok.strategy = PHDE_PIVOTS_KCENTERS;
     const double tol = o.judge_tol;
     rc = layout_learned(c, &o, coords, out);
-    if (rc == PHDE_ERR_PIVOTS) {
+    if (rc == PHDE_ERR_PIVOTS || rc == kSelectUnavailable) {
       fallback = 1;
       LayoutOut ko;
       ko.st = stats;

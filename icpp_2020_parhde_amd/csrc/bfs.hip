@@ -85,7 +85,8 @@ __device__ __forceinline__ bool discover(int32_t *dist, const uint32_t *rowptr, 
   return true;
 }
 
-constexpr int kGroup = 16;  // neighbours a lane handles per round trip
+constexpr int kGroup = 16;   // neighbours a lane handles per round trip (top-down)
+constexpr int kBuGroup = 8;  // neighbours a lane tests per round trip (bottom-up)
 
 // Direction-rule state carried from level to level (bfs.cc:159-183).
 struct DirState {
@@ -383,15 +384,25 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
       uint32_t ub = 0, ue = 0;
       if (u < n && ld_dev(&a.dist[u]) == kUnvisited) {
         const uint32_t b = rowptr[u], e = rowptr[u + 1];
-        for (uint32_t k = b; k < e; k++) {
-          const uint32_t v = adj[k];
-          const bool in_front = prev_bu
-                                    ? ((ld_dev((const unsigned long long *)&bm_cur[v >> 6]) >> (v & 63)) & 1ull) != 0
-                                    : ld_dev(&a.dist[v]) == L;
-          if (in_front) {
-            f = true;
-            break;
+        // kBuGroup neighbours per round trip: all their ids, then all their
+        // frontier tests, are in flight together; stop at the first group
+        // that holds a frontier neighbour (the reference stops at the first
+        // neighbour; the level written is the same)
+        for (uint32_t k0 = b; k0 < e && !f; k0 += kBuGroup) {
+          uint32_t vv[kBuGroup];
+#pragma unroll
+          for (int j = 0; j < kBuGroup; j++) vv[j] = adj[(k0 + j < e) ? k0 + j : b];
+          bool hit[kBuGroup];
+          if (prev_bu) {
+#pragma unroll
+            for (int j = 0; j < kBuGroup; j++)
+              hit[j] = ((ld_dev((const unsigned long long *)&bm_cur[vv[j] >> 6]) >> (vv[j] & 63)) & 1ull) != 0;
+          } else {
+#pragma unroll
+            for (int j = 0; j < kBuGroup; j++) hit[j] = ld_dev(&a.dist[vv[j]]) == L;
           }
+#pragma unroll
+          for (int j = 0; j < kBuGroup; j++) f = f || (k0 + j < e && hit[j]);
         }
         if (f) st_dev(&a.dist[u], next_level);
         ub = b;
@@ -550,9 +561,18 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
     PHDE_TRY_RC(fill_u32(c, c->d_bar, 2, 0u));
     unsigned int *arrive = c->d_bar, *err = c->d_bar + 1;
     void *args[] = {&a, &arrive, &err};
-    PHDE_HIP(hipLaunchCooperativeKernel((const void *)bfs_persistent_kernel, dim3(grid),
-                                        dim3(kBfsBlock), args, 0, c->stream));
-  } else {
+    const hipError_t e = hipLaunchCooperativeKernel((const void *)bfs_persistent_kernel, dim3(grid),
+                                                    dim3(kBfsBlock), args, 0, c->stream);
+    if (e != hipSuccess) {
+      // cooperative launches can be unavailable (measured: in a second
+      // process sharing the GPU); this context then launches one kernel
+      // per level, which needs no co-residency
+      (void)hipGetLastError();
+      c->bfs_persistent = false;
+      a.bstride = 1;
+    }
+  }
+  if (!c->bfs_persistent) {
     bfs_init_kernel<<<grid, kBfsBlock, 0, c->stream>>>(a);
     PHDE_HIP(hipGetLastError());
     rc = run_levels(c, [&](int L) -> int {

@@ -27,22 +27,58 @@ namespace phde {
 constexpr int kEigThreads = 1024;
 constexpr int kEigWaves = kEigThreads / kWave;
 
-// Sum of p over lanes 0..dim-1 in lane order, starting from 0.
-__device__ __forceinline__ double seq_sum(double p, int dim) {
-  double s = 0;
-  for (int k = 0; k < dim; k++) s += __shfl(p, k);
-  return s;
+// Sum of p over lanes 0..dim-1 in lane order, starting from 0 (the
+// one-thread order of the reference).  The wave's values go through its own
+// LDS row (65 doubles): lane 0 reads them 16 at a time, all 16 loads in
+// flight together, adds them in order and publishes the result.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+__device__ __forceinline__ double seq_sum(double p, int dim, double *wbuf) {
+  const int lane = __lane_id();
+  wbuf[lane] = p;
+  wave_lds_sync();
+  if (lane == 0) {
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < 64; c += 16) {
+      if (c >= dim) break;
+      double v[16];
+#pragma unroll
+      for (int k = 0; k < 16; k += 2) {
+        const double2 t = *reinterpret_cast<const double2 *>(wbuf + c + k);
+        v[k] = t.x;
+        v[k + 1] = t.y;
+      }
+#pragma unroll
+      for (int k = 0; k < 16; k++)
+        if (c + k < dim) s += v[k];
+    }
+    wbuf[64] = s;
+  }
+  wave_lds_sync();
+  return wbuf[64];
+}
+
+constexpr int kSumRow = 72;  // doubles per wave in the summation area
 
 __global__ __launch_bounds__(kEigThreads) void power_eig_kernel(double *K, const double *init,
                                                                double *evals, int dim,
                                                                int max_iter, double *Tglob,
                                                                int *status) {
-  extern __shared__ double lds[];
+  extern __shared__ __attribute__((aligned(16))) double lds[];
   const int dd = dim * dim;
+  // LDS: E, T (or T in global memory when dim is large), a copy of K, the
+  // per-wave summation rows (16-byte aligned), eigenvalue / denominator /
+  // convergence slots
   double *E = lds;
   double *T = Tglob ? Tglob : lds + dd;
-  double *small = Tglob ? lds + dd : lds + 2 * dd;
+  double *Kl = Tglob ? lds + dd : lds + 2 * dd;
+  double *sums = lds + (((Tglob ? 2 : 3) * dd + 1) & ~1);
+  double *small = sums + kEigWaves * kSumRow;
   double *ev = small;
   double *djj = small + 64;
   double *conv = small + 128;
@@ -50,13 +86,17 @@ __global__ __launch_bounds__(kEigThreads) void power_eig_kernel(double *K, const
   __shared__ double r_sh;
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
   const bool on = lane < dim;
+  double *wsum = sums + wid * kSumRow;
 
-  for (int e = tid; e < dd; e += kEigThreads) E[e] = init[e];
+  for (int e = tid; e < dd; e += kEigThreads) {
+    E[e] = init[e];
+    Kl[e] = K[e];
+  }
   __syncthreads();
   // ParallelComponents.cpp:67-72
   for (int i = wid; i < dim; i += kEigWaves) {
     const double x = on ? E[lane + i * dim] : 0.0;
-    const double nrm = sqrt(seq_sum(x * x, dim));
+    const double nrm = sqrt(seq_sum(x * x, dim, wsum));
     if (lane == 0) ev[i] = nrm;
     if (on) E[lane + i * dim] = x / nrm;
   }
@@ -83,7 +123,7 @@ __global__ __launch_bounds__(kEigThreads) void power_eig_kernel(double *K, const
     for (int e = tid; e < dd; e += kEigThreads) {
       const int k = e % dim, i = e / dim;
       double acc = 0;
-      for (int j = 0; j < dim; j++) acc += K[k + j * dim] * T[j + i * dim];
+      for (int j = 0; j < dim; j++) acc += Kl[k + j * dim] * T[j + i * dim];
       E[e] = acc;
     }
     __syncthreads();
@@ -94,11 +134,11 @@ __global__ __launch_bounds__(kEigThreads) void power_eig_kernel(double *K, const
         while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
           __builtin_amdgcn_s_sleep(1);
         const double ej = on ? E[lane + j * dim] : 0.0;
-        const double fac = seq_sum(ej * x, dim) / djj[j];
+        const double fac = seq_sum(ej * x, dim, wsum) / djj[j];
         x = x - fac * ej;
       }
       if (on) E[lane + i * dim] = x;
-      const double d = seq_sum(x * x, dim);
+      const double d = seq_sum(x * x, dim, wsum);
       if (lane == 0) {
         djj[i] = d;
         __hip_atomic_store(&flags[i], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -108,7 +148,7 @@ __global__ __launch_bounds__(kEigThreads) void power_eig_kernel(double *K, const
     // normalisation (:113-120)
     for (int i = wid; i < dim; i += kEigWaves) {
       const double x = on ? E[lane + i * dim] : 0.0;
-      const double nrm = sqrt(seq_sum(x * x, dim));
+      const double nrm = sqrt(seq_sum(x * x, dim, wsum));
       if (lane == 0) ev[i] = nrm;
       if (on) E[lane + i * dim] = x / nrm;
     }
@@ -117,7 +157,7 @@ __global__ __launch_bounds__(kEigThreads) void power_eig_kernel(double *K, const
     for (int i = wid; i < dim; i += kEigWaves) {
       const double x = on ? E[lane + i * dim] : 0.0;
       const double y = on ? T[lane + i * dim] : 0.0;
-      const double d = seq_sum(x * y, dim);
+      const double d = seq_sum(x * y, dim, wsum);
       if (lane == 0) conv[i] = d;
     }
     __syncthreads();
@@ -369,11 +409,11 @@ int power_eig_dev(phde_ctx *c, double *K_dev, const double *init_dev, double *ev
                   int max_iter, double *work_dev, int *status_dev) {
   int maxsh = 0;
   PHDE_HIP(hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-  const size_t small = 192 * sizeof(double) + 64 * sizeof(int);
-  size_t shmem = 2 * sizeof(double) * dim * dim + small;
+  const size_t small = (193 + kEigWaves * kSumRow) * sizeof(double) + 64 * sizeof(int);
+  size_t shmem = 3 * sizeof(double) * dim * dim + small;
   double *Tglob = nullptr;
   if (shmem > (size_t)maxsh) {
-    shmem = sizeof(double) * dim * dim + small;
+    shmem = 2 * sizeof(double) * dim * dim + small;
     Tglob = work_dev;
   }
   if (shmem > 65536)

@@ -874,8 +874,13 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
   void *args[] = {&a, &arrive, &err};
   hipEvent_t ev;
   c->timer.begin(c->stream, PHDE_KERNEL_OTHER, &ev);
-  PHDE_HIP(hipLaunchCooperativeKernel((const void *)select_kernel, dim3(c->bfs_grid), dim3(256), args, 0,
-                                      c->stream));
+  const hipError_t e = hipLaunchCooperativeKernel((const void *)select_kernel, dim3(c->bfs_grid), dim3(256),
+                                                  args, 0, c->stream);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    set_error(std::string("pivot selection needs a cooperative launch: ") + hipGetErrorString(e));
+    return kSelectUnavailable;
+  }
   c->timer.end(c->stream, PHDE_KERNEL_OTHER, ev);
   std::vector<int64_t> h(K);
   unsigned int herr = 0;
@@ -1160,7 +1165,8 @@ int phde_select_pivots(phde_ctx *c, const float *score, int64_t first, int k, in
   PHDE_HIP(hipMemcpyAsync(c->lp_score, score, sizeof(float) * c->g.n, hipMemcpyHostToDevice, c->stream));
   pivots[0] = first;
   pivots[1] = p2;
-  return greedy_select(c, c->lp_score, 2, k, radius, pivots);
+  const int rc = greedy_select(c, c->lp_score, 2, k, radius, pivots);
+  return rc == kSelectUnavailable ? PHDE_ERR_UNSUPPORTED : rc;
 }
 
 }  // extern "C"
