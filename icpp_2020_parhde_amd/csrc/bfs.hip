@@ -53,7 +53,8 @@ __device__ __forceinline__ void st_qent(QEnt *q, QEnt v) {
 // Append the lanes' `found` vertices (with row ranges) to the next frontier
 // (queue, huge list, degree sum).  Must be called by all 64 lanes.
 __device__ __forceinline__ void wave_append(bool found, uint32_t v, uint32_t b, uint32_t e,
-                                            QEnt *q_next, uint32_t *h_next, BfsSlot *slot) {
+                                            QEnt *q_next, uint32_t *h_next, BfsSlot *slot,
+                                            unsigned long long *blk_scout) {
   const unsigned long long mask = __ballot(found);
   if (mask == 0) return;
   const int lane = __lane_id();
@@ -65,7 +66,7 @@ __device__ __forceinline__ void wave_append(bool found, uint32_t v, uint32_t b, 
   unsigned int base = 0, hbase = 0;
   if (lane == leader) {
     base = atomicAdd(&slot->q, (unsigned int)__popcll(mask));
-    atomicAdd(&slot->scout, dsum);
+    atomicAdd(blk_scout, dsum);  // the level's device slot, or LDS on wide levels
     if (hmask) hbase = atomicAdd(&slot->huge, (unsigned int)__popcll(hmask));
   }
   base = __shfl(base, leader);
@@ -87,6 +88,7 @@ __device__ __forceinline__ bool discover(int32_t *dist, const uint32_t *rowptr, 
 
 constexpr int kGroup = 16;   // neighbours a lane handles per round trip (top-down)
 constexpr int kBuGroup = 8;  // neighbours a lane tests per round trip (bottom-up)
+constexpr int kWideFrontier = 65536;  // frontier size from which a level reads through the caches
 
 // Direction-rule state carried from level to level (bfs.cc:159-183).
 struct DirState {
@@ -256,6 +258,30 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
   }
   const bool prev_bu = st.prev_bu != 0;
   st.prev_bu = bu ? 1 : 0;
+  // Wide levels read the level's data through the caches.  Every write that
+  // another block reads (levels, claims, queue entries, bitmap words) is a
+  // device-scope store or atomic that reached memory before the barrier;
+  // an agent-scope acquire here invalidates this CU's L1 and the XCD's L2
+  // copies, so plain loads below see the state at the start of the level
+  // and repeated reads (bitmap words, levels of hub neighbours) hit L2.  The
+  // invalidation costs too much for the thousands of narrow levels of a
+  // high-diameter graph, which keep their device-scope loads.
+  const bool wide = bu || prev_bu || nq > (unsigned int)kWideFrontier;
+  if (wide) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  // per-block totals of the level (degree sum of the next frontier, vertices
+  // found bottom-up), published with one device atomic per block at the end
+  // (wide levels only: on narrow ones the extra block barrier and the
+  // serial publish cost more than the contention they avoid)
+  __shared__ unsigned long long blk_scout;
+  __shared__ unsigned int blk_found;
+  unsigned long long *scout_sum = wide ? &blk_scout : &nxt->scout;
+  if (wide) {
+    if (threadIdx.x == 0) {
+      blk_scout = 0;
+      blk_found = 0;
+    }
+    __syncthreads();
+  }
   if (gtid == 0) {
     st_dev(&old->q, 0u);
     st_dev(&old->huge, 0u);
@@ -280,15 +306,23 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
         const uint32_t v = ok ? adj[k] : 0u;
         uint32_t vb = 0, ve = 0;
         const bool f = ok && discover(a.dist, rowptr, v, next_level, &vb, &ve);
-        wave_append(f, v, vb, ve, q_next, h_next, nxt);
+        wave_append(f, v, vb, ve, q_next, h_next, nxt, scout_sum);
       }
     }
-    for (int64_t base = gwave * kWave; base < (int64_t)nq; base += nwaves * kWave) {
+    // The frontier: the queue, or after a bottom-up level its bitmap (a
+    // bottom-up level builds no queue; bfs.cc:109-132 converts instead).
+    const int64_t nitems = prev_bu ? ((n + 63) / 64) * 64 : (int64_t)nq;
+    for (int64_t base = gwave * kWave; base < nitems; base += nwaves * kWave) {
       const int64_t idx = base + lane;
-      const bool act = idx < (int64_t)nq;
-      const QEnt q = act ? ld_qent(&q_cur[idx]) : QEnt{0u, 0u, 0u, 0u};
+      QEnt q{0u, 0u, 0u, 0u};
+      if (prev_bu) {
+        const uint64_t word = bm_cur[base / 64];
+        if (idx < n && ((word >> lane) & 1ull)) q = QEnt{(uint32_t)idx, rowptr[idx], rowptr[idx + 1], 0u};
+      } else if (idx < (int64_t)nq) {
+        q = wide ? q_cur[idx] : ld_qent(&q_cur[idx]);
+      }
       int d = (int)(q.e - q.b);
-      if (d >= kHugeDegree) d = 0;  // expanded above
+      if (d >= kHugeDegree && !prev_bu) d = 0;  // expanded above
       const bool big = d >= kBigDegree;
       unsigned long long bmask = __ballot(big);
       while (bmask) {
@@ -301,7 +335,7 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
           const uint32_t v = ok ? adj[k] : 0u;
           uint32_t vb = 0, ve = 0;
           const bool f = ok && discover(a.dist, rowptr, v, next_level, &vb, &ve);
-          wave_append(f, v, vb, ve, q_next, h_next, nxt);
+          wave_append(f, v, vb, ve, q_next, h_next, nxt, scout_sum);
         }
       }
       // Small lists: each lane issues the loads, checks and claims of up to
@@ -360,7 +394,7 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
         const unsigned long long tsum = wave_sum_u64(dsum);
         if (lane == 0) {
           qb = atomicAdd(&nxt->q, (unsigned int)total);
-          atomicAdd(&nxt->scout, tsum);
+          atomicAdd(scout_sum, tsum);
         }
         qb = __shfl(qb, 0);
         unsigned int pos = qb + (unsigned int)off;
@@ -382,7 +416,7 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
       const int64_t u = w * 64 + lane;
       bool f = false;
       uint32_t ub = 0, ue = 0;
-      if (u < n && ld_dev(&a.dist[u]) == kUnvisited) {
+      if (u < n && a.dist[u] == kUnvisited) {
         const uint32_t b = rowptr[u], e = rowptr[u + 1];
         // kBuGroup neighbours per round trip: all their ids, then all their
         // frontier tests, are in flight together; stop at the first group
@@ -395,11 +429,10 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
           bool hit[kBuGroup];
           if (prev_bu) {
 #pragma unroll
-            for (int j = 0; j < kBuGroup; j++)
-              hit[j] = ((ld_dev((const unsigned long long *)&bm_cur[vv[j] >> 6]) >> (vv[j] & 63)) & 1ull) != 0;
+            for (int j = 0; j < kBuGroup; j++) hit[j] = ((bm_cur[vv[j] >> 6] >> (vv[j] & 63)) & 1ull) != 0;
           } else {
 #pragma unroll
-            for (int j = 0; j < kBuGroup; j++) hit[j] = ld_dev(&a.dist[vv[j]]) == L;
+            for (int j = 0; j < kBuGroup; j++) hit[j] = a.dist[vv[j]] == L;
           }
 #pragma unroll
           for (int j = 0; j < kBuGroup; j++) f = f || (k0 + j < e && hit[j]);
@@ -409,8 +442,19 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
         ue = e;
       }
       const unsigned long long mask = __ballot(f);
-      if (lane == 0) st_dev((unsigned long long *)&bm_next[w], mask);
-      wave_append(f, (uint32_t)u, ub, ue, q_next, h_next, nxt);
+      if (lane == 0) {
+        st_dev((unsigned long long *)&bm_next[w], mask);
+        if (mask) atomicAdd(&blk_found, (unsigned int)__popcll(mask));
+      }
+      (void)ub;
+      (void)ue;
+    }
+  }
+  if (wide) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      if (blk_scout) atomicAdd(&nxt->scout, blk_scout);
+      if (blk_found) atomicAdd(&nxt->q, blk_found);
     }
   }
   return true;
