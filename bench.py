@@ -91,6 +91,8 @@ def main():
                     help="also time the learned-pivot layout against k-centers (needs the weights file)")
     ap.add_argument("--learned-n", type=int, default=1 << 24,
                     help="points of the second, held-out RGG for the learned comparison (0: skip)")
+    ap.add_argument("--configs", type=int, default=1,
+                    help="also time RMAT-22 (ParHDE FP32) and the weighted 160^3 grid (WParHDE) with CPU samples")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the layouts, run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -157,6 +159,8 @@ def main():
                 log(f"[bench] rgg{g5.n} k-centers layout {ms5:.1f} ms")
                 learned[f"rgg{args.learned_n >> 20}m"] = learned_vs_kcenters(c5, X5, ms5, args.s, prec)
 
+    other = other_configs(args) if args.configs else None
+
     kernel = bfs_kernel_name()
     workload = f"rgg{args.n >> 20}m_s{args.s}" if args.n >= 1 << 20 else f"rgg{args.n}_s{args.s}"
     record = {
@@ -189,6 +193,7 @@ def main():
         "h2d_ms": round(h2d_s * 1e3, 1),
         **extras,
         "learned_pivots": learned,
+        "other_configs": other,
         "cpu_baseline": None,
     }
     if args.cpu_baseline:
@@ -227,6 +232,59 @@ def learned_vs_kcenters(ctx, X_kc, ms_kc, s, prec):
          "within_2pct": bool(e_l <= 1.02 * e_k), "n_candidates": st.n_candidates}
     log(f"[bench] learned pivots: {r}")
     return r
+
+
+def aligned_rel_err(X, Y):
+    err = 0.0
+    for k in range(Y.shape[1]):
+        x, y = X[:, k], Y[:, k]
+        x = -x if np.dot(x, y) < 0 else x
+        err = max(err, float(np.max(np.abs(x - y)) / np.max(np.abs(y))))
+    return err
+
+
+def other_configs(args):
+    """RMAT scale 22 (a=.57 b=.19 c=.19, seed 3, LCC), ParHDE s=50 with FP32
+    orthogonalisation / SpMM and FP64 eigensolve, and the weighted 160^3 grid
+    (weights 1..255, seed 4), WParHDE s=20 c=3 FP32: GPU ms per layout against
+    the oracle's CPU sample on the host cores, and the FP32 coordinates'
+    error against the GPU FP64 path (itself within 1e-8 of the oracle)."""
+    from icpp_2020_parhde_amd import layout as L
+    from icpp_2020_parhde_amd.graphs import Graph
+    from oracle import pyoracle as O
+    out = {}
+    for name, make, algo, s, c, nbfs in [
+            ("rmat22_s50", lambda: Graph.rmat(22, 16, seed=3), L.ALGO_PARHDE, 50, 2, 40),
+            ("grid160_wparhde_s20", lambda: Graph.grid3d(160, 160, 160, seed=4, wmax=255), L.ALGO_WPARHDE, 20, 3, 4)]:
+        t0 = time.time()
+        g = make()
+        log(f"[bench] {name}: n={g.n} nnz={g.m} ({time.time() - t0:.1f}s)")
+        with L.Context(g) as ctx:
+            opts = dict(s=s, num_coord=c, algorithm=algo)
+            ctx.layout(precision=L.PREC_FP32, **opts)
+            ts = []
+            for _ in range(2):
+                t = time.perf_counter()
+                X32, st = ctx.layout(precision=L.PREC_FP32, **opts)
+                ts.append((time.perf_counter() - t) * 1e3)
+            X64, st64 = ctx.layout(precision=L.PREC_FP64, **opts)
+        trav = st.kernel_ms["bfs" if algo == L.ALGO_PARHDE else "sssp"] / st.n_traversals
+        t0 = time.time()
+        r = O.parhde_sample(g, s, nbfs=nbfs, c=c)
+        rest = sum(v for k, v in r["phases_ms"].items() if k != "bfs_each")
+        # ParHDE: the reference's eager 3s traversals; WParHDE traverses s
+        cpu = r["value"] if algo == L.ALGO_PARHDE else s * r["phases_ms"]["bfs_each"] + rest
+        ms = float(np.mean(ts))
+        out[name] = {"n": g.n, "nnz": g.m, "gpu_ms_per_layout": round(ms, 2),
+                     "gpu_ms_per_traversal": round(trav, 3), "traversals": st.n_traversals,
+                     "gpu_fp64_ms_per_layout": round(st64.ms_total, 2),
+                     "fp32_vs_fp64_rel_err": aligned_rel_err(X32, X64),
+                     "cpu_ms_per_layout": round(cpu, 1), "cpu_cores": r["cores"],
+                     "cpu_traversal_ms": r["phases_ms"]["bfs_each"], "cpu_sample_s": round(time.time() - t0, 1),
+                     "speedup_vs_cpu": round(cpu / ms, 2)}
+        log(f"[bench] {name}: {out[name]}")
+        del g
+    return out
 
 
 def cpu_baseline(g, s):

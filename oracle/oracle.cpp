@@ -883,7 +883,10 @@ int or_phde(const or_graph *g, int s, int numCoord, or_result *r) {
 }
 
 int or_parhde_sample(const or_graph *g, int s, int numCoord, int nbfs, or_sample *out) {
-  if (!g || !out || s < 2 || s > 64 || nbfs < 1 || g->w) return OR_ERR_ARG;
+  // weighted graphs: WParHDE's traversals (delta-stepping, the reference's
+  // delta 10000, WParHDE.cpp:104); its MGS step is costed as a CGS step (the
+  // same dot products and updates per candidate)
+  if (!g || !out || s < 2 || s > 64 || nbfs < 1) return OR_ERR_ARG;
   const int64_t n = g->n;
   auto t_begin = clk::now();
   std::vector<double> degrees(n);
@@ -898,7 +901,7 @@ int or_parhde_sample(const or_graph *g, int s, int numCoord, int nbfs, or_sample
   std::set<int64_t> sources;
   auto t0 = clk::now();
   for (int k = 0; k < nbfs; k++) {
-    distance_column(g, src, col.data(), 0);
+    distance_column(g, src, col.data(), g->w ? 10000 : 0);
     double maxval;
     int64_t idx;
     kcenter_argmax(n, col.data(), min_dists.data(), &maxval, &idx);
