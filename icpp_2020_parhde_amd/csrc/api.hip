@@ -519,7 +519,7 @@ static int layout_learned(phde_ctx *c, const phde_options *o, double *coords, La
   const int64_t n = c->g.n;
   const int s = o->s;
   const bool fp32 = o->precision == PHDE_PREC_FP32;
-  const int K = (int)std::min<int64_t>(64, n);
+  const int kmax = (int)std::min<int64_t>(64, n);
   auto t0 = clk::now();
   int64_t p2 = 0;
   int e1 = 0, e2 = 0;
@@ -527,34 +527,49 @@ static int layout_learned(phde_ctx *c, const phde_options *o, double *coords, La
   PHDE_TRY(net_score(c, c->net));
   const double rho = o->nms_rho > 0 ? o->nms_rho : 1.0;
   const int r = (int)std::max(1L, std::min(2046L, std::lround(rho * e1 / std::sqrt((double)s))));
-  std::vector<int64_t> picks(K);
+  std::vector<int64_t> picks(kmax);
   picks[0] = o->first_pivot;
   picks[1] = p2;
-  PHDE_TRY(greedy_select(c, c->lp_score, 2, K, r, picks.data()));
   out.t_select += ms_since(t0);
-  auto t1 = clk::now();
   out.pivot(picks[0]);
   out.pivot(picks[1]);
   out.levels += e1 + e2;
-  std::vector<unsigned long long> sumsq(K + 1);
+  std::vector<unsigned long long> sumsq(kmax + 1);
   PHDE_TRY(column_sumsq(c, c->lp_d12, 2, sumsq.data()));
   DBuf<int32_t> D;
-  PHDE_TRY(D.alloc((size_t)n * std::max(1, K - 2)));
-  if (K > 2) PHDE_TRY(columns_of(c, o, picks.data() + 2, K - 2, D.p, sumsq.data() + 2, out));
-  out.t_dist += ms_since(t1);
-  auto t2 = clk::now();
+  PHDE_TRY(D.alloc((size_t)n * std::max(1, kmax - 2)));
   Orth orth;
   PHDE_TRY(orth.setup(c, n, s, fp32));
   PHDE_TRY(orth.init_ones(c->d_degw, 1.0 * (1 / std::sqrt((double)n))));
-  int accepted = 0;
-  for (int i = 0; i < K && accepted < s; i++) {
-    const int32_t *col = i < 2 ? c->lp_d12 + (size_t)i * n : D.p + (size_t)(i - 2) * n;
-    PHDE_TRY(orth_candidate(orth, o, false, fp32, col, sumsq[i], accepted, out));
+  // Proposals in batches: s + 2 first (two spares for discarded columns),
+  // then as many more as columns are still missing (+2), up to 64.  Each
+  // batch continues the selection (the distance-to-pivots field persists)
+  // and traverses only its new pivots.
+  int have = 2, accepted = 0, next = 0;
+  int target = std::min(kmax, s + 2);
+  for (;;) {
+    if (target > have) {
+      auto ts = clk::now();
+      PHDE_TRY(greedy_select(c, c->lp_score, have, target, r, picks.data()));
+      out.t_select += ms_since(ts);
+      auto td = clk::now();
+      PHDE_TRY(columns_of(c, o, picks.data() + have, target - have, D.p + (size_t)(have - 2) * n,
+                          sumsq.data() + have, out));
+      out.t_dist += ms_since(td);
+      have = target;
+    }
+    auto to = clk::now();
+    for (; next < have && accepted < s; next++) {
+      const int32_t *col = next < 2 ? c->lp_d12 + (size_t)next * n : D.p + (size_t)(next - 2) * n;
+      PHDE_TRY(orth_candidate(orth, o, false, fp32, col, sumsq[next], accepted, out));
+    }
+    out.t_orth += ms_since(to);
+    if (accepted >= s || have >= kmax) break;
+    target = std::min(kmax, have + (s - accepted) + 2);
   }
-  out.t_orth += ms_since(t2);
   out.nacc = accepted;
-  if (c->timer.enabled && out.st) out.st->n_candidates = K;
-  if (accepted < s) return not_enough_pivots(o, accepted, K);
+  if (c->timer.enabled && out.st) out.st->n_candidates = have;
+  if (accepted < s) return not_enough_pivots(o, accepted, have);
   return hde_tail(c, o, orth, fp32, coords, out, true);
 }
 

@@ -134,7 +134,7 @@ def orderable(score):
 ROUND = 8  # picks per round (learned.hip kSelRound)
 
 
-def greedy_select(g, score, d1, d2, first, second, k, r):
+def greedy_select(g, score, d1, d2, first, second, k, r, batches=None):
     """Pivots of learned.hip's select_kernel.  Picks 0 and 1 are given (the
     feature traversals' sources).  The rest come in rounds of up to ROUND:
     each pick maximises (min(sd, r+1), top 21 bits of the score's order key,
@@ -142,8 +142,9 @@ def greedy_select(g, score, d1, d2, first, second, k, r):
     for every earlier pick c of the same round, where sd is the distance to
     the picks of earlier rounds (exact within r hops; the two given picks
     exactly).  A round ends when the best such vertex has sd <= r (its first
-    pick is always taken).  After a round, sd = min(sd, distance to the
-    round's picks where that is <= r)."""
+    pick is always taken), or at the end of a batch (`batches`: increasing
+    pick counts at which the selection was resumed, default [k]).  After a
+    round, sd = min(sd, distance to the round's picks where that is <= r)."""
     A, _, _ = adjacency(g)
     r = max(1, min(int(r), 2046))
     d1 = np.asarray(d1, dtype=np.int64)
@@ -152,26 +153,27 @@ def greedy_select(g, score, d1, d2, first, second, k, r):
     hi = orderable(score) >> np.uint64(11)
     idx = np.arange(g.n, dtype=np.uint64)
     picks = [int(first), int(second)]
-    while len(picks) < k:
-        rnd = []
-        while len(rnd) < ROUND and len(picks) < k:
-            far = np.ones(g.n, dtype=bool)
-            for c in rnd:
-                far &= np.maximum(np.abs(d1 - d1[c]), np.abs(d2 - d2[c])) > r
-            if not far.any():
-                break
-            sdc = np.minimum(sd, r + 1).astype(np.uint64)
-            key = (((sdc << np.uint64(21)) | hi) << np.uint64(32)) | (np.uint64(0xFFFFFFFF) - idx)
-            key = np.where(far, key, np.uint64(0))
-            best = key.max()
-            if rnd and int(best >> np.uint64(53)) != r + 1:
-                break
-            p = int(np.uint64(0xFFFFFFFF) - (best & np.uint64(0xFFFFFFFF)))
-            rnd.append(p)
-            picks.append(p)
-        dp = dijkstra(A, unweighted=True, indices=rnd, limit=r + 0.5, min_only=True)
-        reach = np.isfinite(dp)
-        sd[reach] = np.minimum(sd[reach], dp[reach].astype(np.int64))
+    for end in (batches or [k]):
+        while len(picks) < end:
+            rnd = []
+            while len(rnd) < ROUND and len(picks) < end:
+                far = np.ones(g.n, dtype=bool)
+                for c in rnd:
+                    far &= np.maximum(np.abs(d1 - d1[c]), np.abs(d2 - d2[c])) > r
+                if not far.any():
+                    break
+                sdc = np.minimum(sd, r + 1).astype(np.uint64)
+                key = (((sdc << np.uint64(21)) | hi) << np.uint64(32)) | (np.uint64(0xFFFFFFFF) - idx)
+                key = np.where(far, key, np.uint64(0))
+                best = key.max()
+                if rnd and int(best >> np.uint64(53)) != r + 1:
+                    break
+                p = int(np.uint64(0xFFFFFFFF) - (best & np.uint64(0xFFFFFFFF)))
+                rnd.append(p)
+                picks.append(p)
+            dp = dijkstra(A, unweighted=True, indices=rnd, limit=r + 0.5, min_only=True)
+            reach = np.isfinite(dp)
+            sd[reach] = np.minimum(sd[reach], dp[reach].astype(np.int64))
     return np.array(picks, dtype=np.int64)
 
 

@@ -66,18 +66,20 @@ def test_emulated_gradient_close_to_exact():
     assert np.linalg.norm(ge - gx) <= 0.05 * np.linalg.norm(gx)
 
 
-def _brute_select(g, score, first, second, k, r):
+def _brute_select(g, score, first, second, k, r, batches=None):
     """The round-wise selection restated from all-pairs distances."""
     D = shortest_path(OL.adjacency(g)[0], unweighted=True).astype(np.int64)
     r = max(1, min(r, 2046))
     hi = OL.orderable(score) >> np.uint64(11)
     picks, rounds = [first, second], []
+    ends = list(batches or [k])
     while len(picks) < k:
+        end = next(e for e in ends if e > len(picks))
         sd = np.minimum(D[first], D[second])
         for q in rounds:
             sd = np.where(D[q] <= r, np.minimum(sd, D[q]), sd)
         rnd = []
-        while len(rnd) < OL.ROUND and len(picks) < k:
+        while len(rnd) < OL.ROUND and len(picks) < end:
             best = None
             for v in range(g.n):
                 if any(max(abs(D[first, v] - D[first, c]), abs(D[second, v] - D[second, c])) <= r for c in rnd):
@@ -106,6 +108,20 @@ def test_greedy_select_matches_brute_force(name, r):
         k = min(20, g.n)
         got = OL.greedy_select(g, score, d1, d2, 0, second, k, r)
         assert got.tolist() == _brute_select(g, score, 0, second, k, r).tolist()
+
+
+def test_greedy_select_resumed_in_batches():
+    # the layout resumes the selection when columns are still missing: a
+    # batch boundary closes the current round
+    g = Graph.rgg(400, 8.0, seed=12)
+    rng = np.random.default_rng(2)
+    score = rng.random(g.n).astype(np.float32)
+    d1 = O.seq_bfs(g, 0)
+    second = int(np.argmax(d1))
+    d2 = O.seq_bfs(g, second)
+    for batches in ([7, 13, 24], [24], [5, 6, 24]):
+        got = OL.greedy_select(g, score, d1, d2, 0, second, 24, 2, batches=batches)
+        assert got.tolist() == _brute_select(g, score, 0, second, 24, 2, batches=batches).tolist()
 
 
 def test_greedy_select_path_by_hand():
