@@ -74,7 +74,10 @@ struct SsspArgs {
   int32_t *min_dist;
   KcResult *kc;
   int *done_dev, *host_flags;
+  int32_t *dummy;  // one scratch word per thread: target of masked-off atomics
 };
+
+constexpr int kSsspGroup = 8;  // edges a lane relaxes per round trip
 
 __global__ __launch_bounds__(kTBlock) void sssp_init_kernel(SsspArgs a) {
   const int64_t n = a.g.n;
@@ -192,8 +195,23 @@ __global__ __launch_bounds__(kTBlock) void sssp_round_kernel(SsspArgs a, int R) 
       a.fmin[(s.epoch + 1) & 1] = INT32_MAX;
     }
     unsigned int *fcnt = &a.fcnt[s.epoch & 1];
-    for (int64_t base = gwave * kWave; base < (int64_t)nq; base += nwaves * kWave) {
-      const int64_t idx = base + lane;
+    // Small rows: kSsspGroup edges per lane per round trip, every load and
+    // atomic of the group issued before any result is used (masked-off
+    // atomics go to the lane's scratch word); near / far appends staged in
+    // LDS and published with one global atomic per block and chunk; the far
+    // pile's minimum reduced per wave.  Rows of 32 or more edges are relaxed
+    // by the whole wave.
+    // One stage holds both: near appends grow up from 0, far appends down
+    // from the top (a lane appends at most 31 of either kind per chunk).
+    constexpr unsigned int kStage = kTBlock * 31;
+    __shared__ uint32_t stage[kStage];
+    __shared__ unsigned int nstn, nstf, gbn, gbf;
+    if (threadIdx.x == 0) nstn = nstf = 0;
+    int32_t *dum = a.dummy + gtid;
+    const int w = threadIdx.x / kWave;
+    for (int64_t cb = (int64_t)blockIdx.x * kTBlock; cb < (int64_t)nq; cb += (int64_t)gridDim.x * kTBlock) {
+      __syncthreads();
+      const int64_t idx = cb + w * kWave + lane;
       const bool act = idx < (int64_t)nq;
       const uint32_t u = act ? ncur[idx] : 0u;
       const uint32_t b = act ? rowptr[u] : 0u, e = act ? rowptr[u + 1] : 0u;
@@ -222,15 +240,76 @@ __global__ __launch_bounds__(kTBlock) void sssp_round_kernel(SsspArgs a, int R) 
       int maxd = ds;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) maxd = max(maxd, __shfl_xor(maxd, o));
-      for (int t = 0; t < maxd; t++) {
-        bool pn = false, pf = false;
-        uint32_t v = 0;
-        if (t < ds) {
-          v = adj[b + t];
-          relax(a, v, du + wt[b + t], s.bound, R, s.epoch, &pn, &pf);
+      for (int t0 = 0; t0 < maxd; t0 += kSsspGroup) {
+        uint32_t vv[kSsspGroup];
+        int32_t nd[kSsspGroup], cur[kSsspGroup];
+        bool ok[kSsspGroup];
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) {
+          ok[j] = t0 + j < ds;
+          vv[j] = ok[j] ? adj[b + t0 + j] : 0u;
+          nd[j] = ok[j] ? du + wt[b + t0 + j] : INT32_MAX;
         }
-        wave_push(pn, v, nnext, nnext_cnt);
-        wave_push(pf, v, fcur, fcnt);
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) cur[j] = a.dist[vv[j]];
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) ok[j] = ok[j] && nd[j] < cur[j];
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) cur[j] = atomicMin(ok[j] ? &a.dist[vv[j]] : dum, nd[j]);
+        bool nr[kSsspGroup], fr[kSsspGroup];
+        int fm = INT32_MAX;
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) {
+          const bool imp = ok[j] && nd[j] < cur[j];
+          nr[j] = imp && nd[j] < s.bound;
+          fr[j] = imp && !(nd[j] < s.bound);
+          if (fr[j]) fm = min(fm, nd[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) cur[j] = atomicExch(nr[j] ? &a.nstamp[vv[j]] : dum, R + 1);
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) nr[j] = nr[j] && cur[j] != R + 1;
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) cur[j] = atomicExch(fr[j] ? &a.fstamp[vv[j]] : dum, s.epoch);
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) fr[j] = fr[j] && cur[j] != s.epoch;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) fm = min(fm, __shfl_xor(fm, o));
+        if (lane == 0 && fm != INT32_MAX) atomicMin(&a.fmin[s.epoch & 1], fm);
+        int cn = 0, cf = 0;
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) {
+          cn += nr[j] ? 1 : 0;
+          cf += fr[j] ? 1 : 0;
+        }
+        int totn = 0, totf = 0;
+        const int offn = wave_excl_scan(cn, &totn);
+        const int offf = wave_excl_scan(cf, &totf);
+        unsigned int bn = 0, bf = 0;
+        if (lane == 0) {
+          if (totn) bn = atomicAdd(&nstn, (unsigned int)totn);
+          if (totf) bf = atomicAdd(&nstf, (unsigned int)totf);
+        }
+        bn = __shfl(bn, 0) + (unsigned int)offn;
+        bf = __shfl(bf, 0) + (unsigned int)offf;
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) {
+          if (nr[j]) stage[bn++] = vv[j];
+          if (fr[j]) stage[kStage - 1 - bf++] = vv[j];
+        }
+      }
+      __syncthreads();
+      const unsigned int nn = nstn, nf2 = nstf;
+      if (nn | nf2) {
+        if (threadIdx.x == 0) {
+          gbn = nn ? atomicAdd(nnext_cnt, nn) : 0u;
+          gbf = nf2 ? atomicAdd(fcnt, nf2) : 0u;
+        }
+        __syncthreads();
+        for (unsigned int i = threadIdx.x; i < nn; i += kTBlock) nnext[gbn + i] = stage[i];
+        for (unsigned int i = threadIdx.x; i < nf2; i += kTBlock) fcur[gbf + i] = stage[kStage - 1 - i];
+        __syncthreads();
+        if (threadIdx.x == 0) nstn = nstf = 0;
       }
     }
   } else {
@@ -443,6 +522,36 @@ __global__ __launch_bounds__(kTBlock) void msbfs_level_kernel(MsArgs a, int L) {
 
 // ------------------------------------------------------------------ host side
 
+// Grid of the multi-source sweep: two 256-thread blocks per CU measured
+// best on RGG n=16.7M with 62 sources (455 ms vs 480 / 466 / 517 ms at one,
+// four and eight blocks per CU).
+static int ms_grid(const phde_ctx *c) {
+  static const int per_cu = [] {
+    const char *e = getenv("PHDE_MS_BLOCKS_PER_CU");
+    return e ? std::max(1, std::min(32, atoi(e))) : 2;
+  }();
+  return std::max(1, c->num_cus) * per_cu;
+}
+
+// Grid of the delta-stepping rounds (blocks per CU: PHDE_SSSP_BLOCKS_PER_CU).
+static int sssp_grid(const phde_ctx *c) {
+  static const int per_cu = [] {
+    const char *e = getenv("PHDE_SSSP_BLOCKS_PER_CU");
+    return e ? std::max(1, std::min(32, atoi(e))) : 2;
+  }();
+  return std::max(1, c->num_cus) * per_cu;
+}
+
+// Per-thread scratch words of the masked-off atomics, sized for the larger
+// of the two grids above.
+static int ensure_dummy(phde_ctx *c) {
+  if (!c->ms_dummy) {
+    const size_t words = (size_t)std::max(ms_grid(c), sssp_grid(c)) * kTBlock;
+    PHDE_HIP(hipMalloc(&c->ms_dummy, sizeof(unsigned long long) * words));
+  }
+  return PHDE_OK;
+}
+
 int sssp_run(phde_ctx *c, int64_t src, int32_t delta, int32_t *dist_out, int32_t *min_dist,
              bool sync_result) {
   if (!c->g.w) {
@@ -452,12 +561,13 @@ int sssp_run(phde_ctx *c, int64_t src, int32_t delta, int32_t *dist_out, int32_t
   int rc = bfs_setup(c);
   if (rc) return rc;
   const int64_t n = c->g.n;
+  PHDE_TRY_RC(ensure_dummy(c));
   if (!c->ts_far[0]) {
     for (int i = 0; i < 2; i++) PHDE_HIP(hipMalloc(&c->ts_far[i], sizeof(uint32_t) * (n + 1)));
     PHDE_HIP(hipMalloc(&c->ts_nstamp, sizeof(int32_t) * n));
     PHDE_HIP(hipMalloc(&c->ts_fstamp, sizeof(int32_t) * n));
-    PHDE_HIP(hipMalloc(&c->ts_cnt, 256));
   }
+  if (!c->ts_cnt) PHDE_HIP(hipMalloc(&c->ts_cnt, 256));
   if (delta <= 0) delta = c->default_delta;
   SsspArgs a;
   a.g = c->g;
@@ -478,11 +588,12 @@ int sssp_run(phde_ctx *c, int64_t src, int32_t delta, int32_t *dist_out, int32_t
   a.kc = c->d_kc;
   a.done_dev = c->d_done;
   a.host_flags = c->d_flags;
+  a.dummy = (int32_t *)c->ms_dummy;
   c->h_flags[kFlagDone] = -1;
   c->h_flags[kFlagProgress] = -1;
   hipEvent_t ev;
   c->timer.begin(c->stream, PHDE_KERNEL_SSSP, &ev);
-  const int grid = c->bfs_grid;
+  const int grid = sssp_grid(c);
   sssp_init_kernel<<<grid, kTBlock, 0, c->stream>>>(a);
   PHDE_HIP(hipGetLastError());
   rc = run_levels(c, [&](int R) -> int {
@@ -499,17 +610,6 @@ int sssp_run(phde_ctx *c, int64_t src, int32_t delta, int32_t *dist_out, int32_t
   return PHDE_OK;
 }
 
-// Grid of the multi-source sweep: two 256-thread blocks per CU measured
-// best on RGG n=16.7M with 62 sources (455 ms vs 480 / 466 / 517 ms at one,
-// four and eight blocks per CU).
-static int ms_grid(const phde_ctx *c) {
-  static const int per_cu = [] {
-    const char *e = getenv("PHDE_MS_BLOCKS_PER_CU");
-    return e ? std::max(1, std::min(32, atoi(e))) : 2;
-  }();
-  return std::max(1, c->num_cus) * per_cu;
-}
-
 int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
   int rc = bfs_setup(c);
   if (rc) return rc;
@@ -519,7 +619,7 @@ int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
     for (int i = 0; i < 3; i++) PHDE_HIP(hipMalloc(&c->ms_fr[i], sizeof(unsigned long long) * n));
     PHDE_HIP(hipMalloc(&c->ms_q3, sizeof(uint32_t) * (n + 1)));
     PHDE_HIP(hipMalloc(&c->ms_src, sizeof(int64_t) * 64));
-    PHDE_HIP(hipMalloc(&c->ms_dummy, sizeof(unsigned long long) * (size_t)ms_grid(c) * kTBlock));
+    PHDE_TRY_RC(ensure_dummy(c));
     if (!c->ts_cnt) PHDE_HIP(hipMalloc(&c->ts_cnt, 256));
   }
   PHDE_HIP(hipMemcpyAsync(c->ms_src, sources, sizeof(int64_t) * k, hipMemcpyHostToDevice, c->stream));
