@@ -89,6 +89,7 @@ PARHDE_SYMBOLS = {
                                           ctypes.POINTER(ctypes.c_void_p)]),
     "phde_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "phde_num_vertices": (ctypes.c_int64, [ctypes.c_void_p]),
+    "phde_vertex_order": (ctypes.c_int, [ctypes.c_void_p, c_u32p, ctypes.POINTER(ctypes.c_int)]),
     "phde_bfs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, c_i32p]),
     "phde_bfs_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_void_p]),

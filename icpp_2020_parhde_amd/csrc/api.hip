@@ -141,6 +141,8 @@ static int ctx_init(phde_ctx *c, int64_t n, int64_t m, bool weighted) {
   c->g.w = c->d_w;
   c->g.deg32 = c->d_deg32;
   c->g.degw = c->d_degw;
+  c->g.orig = nullptr;
+  c->g.perm = nullptr;
   return PHDE_OK;
 }
 
@@ -153,6 +155,13 @@ static int ctx_finish(phde_ctx *c) {
   return PHDE_OK;
 }
 
+// Internal numbering of the graph copy (relabel.hip); degrees follow it.
+static int ctx_renumber(phde_ctx *c) {
+  bool done = false;
+  PHDE_TRY_RC(relabel_graph(c, &done));
+  return done ? ctx_finish(c) : PHDE_OK;
+}
+
 static void ctx_free(phde_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -162,6 +171,8 @@ static void ctx_free(phde_ctx *c) {
   for (void *p : dev)
     if (p) (void)hipFree(p);
   if (c->h_kc) (void)hipHostFree(c->h_kc);
+  if (c->d_orig) (void)hipFree(c->d_orig);
+  if (c->d_perm) (void)hipFree(c->d_perm);
   if (c->d_bar) (void)hipFree(c->d_bar);
   if (c->d_dummy) (void)hipFree(c->d_dummy);
   if (c->h_bar) (void)hipHostFree(c->h_bar);
@@ -200,8 +211,13 @@ struct KCenters {
       PHDE_HIP(hipMemcpyAsync(md.data(), min_dist_dev, sizeof(int32_t) * c->g.n,
                               hipMemcpyDeviceToHost, c->stream));
       PHDE_HIP(hipStreamSynchronize(c->stream));
-      for (int64_t i = 0; i < c->g.n; i++)
-        if (md[i] == maxval && !sources.count(i)) idx = i;
+      // the last such vertex in the caller's numbering
+      int64_t last = -1;
+      for (int64_t i = 0; i < c->g.n; i++) {
+        const int64_t v = to_original(c, i);
+        if (md[i] == maxval && !sources.count(v)) last = std::max(last, v);
+      }
+      if (last >= 0) idx = last;
     }
     sources.insert(old_idx);
     next = idx;
@@ -270,7 +286,12 @@ static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, 
   PHDE_TRY(part.alloc((size_t)nc * c->num_cus * 8 + 64));
   PHDE_TRY(inv.alloc(nc));
   PHDE_TRY(coords_compute(c, S_dev, fp32, s, nc, V.p, X.p, part.p, inv.p));
-  PHDE_HIP(hipMemcpyAsync(coords, X.p, sizeof(double) * c->g.n * nc, hipMemcpyDeviceToHost,
+  DBuf<double> Xo;  // rows in the caller's numbering
+  if (renumbered(c)) {
+    PHDE_TRY(Xo.alloc((size_t)c->g.n * nc));
+    PHDE_TRY(rows_to_original(c, X.p, Xo.p, nc, sizeof(double)));
+  }
+  PHDE_HIP(hipMemcpyAsync(coords, Xo.p ? Xo.p : X.p, sizeof(double) * c->g.n * nc, hipMemcpyDeviceToHost,
                           c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
   out.t_coords += ms_since(t1);
@@ -381,9 +402,9 @@ static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, Layout
     const int64_t src = kc.next;
     out.pivot(src);
     if (weighted)
-      PHDE_TRY(sssp_run(c, src, o->delta, col, min_dist.p, true));
+      PHDE_TRY(sssp_run(c, to_internal(c, src), o->delta, col, min_dist.p, true));
     else
-      PHDE_TRY(bfs_run(c, src, o->alpha, o->beta, col, min_dist.p, true));
+      PHDE_TRY(bfs_run(c, to_internal(c, src), o->alpha, o->beta, col, min_dist.p, true));
     const KcResult r = *c->h_kc;
     if (r.unreached) {
       set_error("graph is not connected: " + std::to_string(r.unreached) +
@@ -440,10 +461,14 @@ static std::vector<int64_t> random_sources(const phde_options *o, int64_t n, int
 static int columns_of(phde_ctx *c, const phde_options *o, const int64_t *src, int k, int32_t *D,
                       unsigned long long *sumsq, LayoutOut &out) {
   const int64_t n = c->g.n;
-  for (int i = 0; i < k; i++) out.pivot(src[i]);
+  std::vector<int64_t> isrc(k);
+  for (int i = 0; i < k; i++) {
+    out.pivot(src[i]);
+    isrc[i] = to_internal(c, src[i]);
+  }
   if (c->g.w) {
     for (int i = 0; i < k; i++) {
-      PHDE_TRY(sssp_run(c, src[i], o->delta, D + (size_t)i * n, nullptr, false));
+      PHDE_TRY(sssp_run(c, isrc[i], o->delta, D + (size_t)i * n, nullptr, false));
       std::vector<unsigned long long> sq(2);
       PHDE_TRY(column_sumsq(c, D + (size_t)i * n, 1, sq.data()));
       if (sq[1]) {
@@ -456,7 +481,7 @@ static int columns_of(phde_ctx *c, const phde_options *o, const int64_t *src, in
   }
   for (int b = 0; b < k; b += 64) {
     const int kb = std::min(64, k - b);
-    PHDE_TRY(msbfs_run(c, src + b, kb, D + (size_t)b * n));
+    PHDE_TRY(msbfs_run(c, isrc.data() + b, kb, D + (size_t)b * n));
     std::vector<unsigned long long> sq(kb + 1);
     PHDE_TRY(column_sumsq(c, D + (size_t)b * n, kb, sq.data()));
     if (sq[kb]) {
@@ -646,9 +671,9 @@ static int layout_mds(phde_ctx *c, const phde_options *o, double *coords, Layout
     out.pivot(src);
     int32_t *col = D.p + (size_t)rc * n;
     if (c->g.w)
-      PHDE_TRY(sssp_run(c, src, o->delta, col, min_dist.p, true));
+      PHDE_TRY(sssp_run(c, to_internal(c, src), o->delta, col, min_dist.p, true));
     else
-      PHDE_TRY(bfs_run(c, src, o->alpha, o->beta, col, min_dist.p, true));
+      PHDE_TRY(bfs_run(c, to_internal(c, src), o->alpha, o->beta, col, min_dist.p, true));
     const KcResult r = *c->h_kc;
     if (r.unreached) {
       set_error("graph is not connected");
@@ -769,6 +794,12 @@ int phde_create(const uint32_t *rowptr, const uint32_t *adj, const int32_t *weig
       }
     c->default_delta = pick_delta(weights, m);
   }
+  rc = ctx_renumber(c);
+  if (rc != PHDE_OK) {
+    ctx_free(c);
+    delete c;
+    return rc;
+  }
   *out = c;
   return PHDE_OK;
 }
@@ -783,6 +814,17 @@ int phde_create_device(const uint32_t *rowptr_dev, const uint32_t *adj_dev,
       (m > 0 && hipMemcpy(ad.data(), adj_dev, sizeof(uint32_t) * m, hipMemcpyDeviceToHost) != hipSuccess))
     return PHDE_ERR_DEVICE;
   if (check_graph_host(rp.data(), ad.data(), n, m)) return PHDE_ERR_ARG;
+  std::vector<int32_t> wh;
+  if (weights_dev && m > 0) {
+    wh.resize(m);
+    if (hipMemcpy(wh.data(), weights_dev, sizeof(int32_t) * m, hipMemcpyDeviceToHost) != hipSuccess)
+      return PHDE_ERR_DEVICE;
+    for (int64_t i = 0; i < m; i++)
+      if (wh[i] < 0) {
+        set_error("negative edge weight");
+        return PHDE_ERR_ARG;
+      }
+  }
   phde_ctx *c = new phde_ctx();
   int rc = ctx_init(c, n, m, weights_dev != nullptr);
   if (rc == PHDE_OK) {
@@ -794,6 +836,8 @@ int phde_create_device(const uint32_t *rowptr_dev, const uint32_t *adj_dev,
       rc = PHDE_ERR_DEVICE;
   }
   if (rc == PHDE_OK) rc = ctx_finish(c);
+  if (rc == PHDE_OK && !wh.empty()) c->default_delta = pick_delta(wh.data(), m);
+  if (rc == PHDE_OK) rc = ctx_renumber(c);
   if (rc != PHDE_OK) {
     ctx_free(c);
     delete c;
@@ -812,46 +856,68 @@ int phde_destroy(phde_ctx *ctx) {
 
 int64_t phde_num_vertices(const phde_ctx *ctx) { return ctx ? ctx->g.n : -1; }
 
+int phde_vertex_order(const phde_ctx *c, uint32_t *order, int *is_renumbered) {
+  if (!c) return PHDE_ERR_ARG;
+  if (is_renumbered) *is_renumbered = renumbered(c) ? 1 : 0;
+  if (order)
+    for (int64_t k = 0; k < c->g.n; k++) order[k] = (uint32_t)to_original(c, k);
+  return PHDE_OK;
+}
+
 int phde_bfs_dev(phde_ctx *c, int64_t source, int alpha, int beta, int32_t *dist_dev) {
   if (!c || !dist_dev || source < 0 || source >= c->g.n) return PHDE_ERR_ARG;
-  PHDE_TRY(bfs_run(c, source, alpha, beta, c->d_dist, nullptr, false));
-  PHDE_TRY(dist_finalize(c, c->d_dist, dist_dev, kUnvisited, -1));
+  PHDE_TRY(bfs_run(c, to_internal(c, source), alpha, beta, c->d_dist, nullptr, false));
+  if (renumbered(c)) {
+    PHDE_TRY(dist_finalize(c, c->d_dist, c->d_dist, kUnvisited, -1));
+    PHDE_TRY(rows_to_original(c, c->d_dist, dist_dev, 1, sizeof(int32_t)));
+  } else {
+    PHDE_TRY(dist_finalize(c, c->d_dist, dist_dev, kUnvisited, -1));
+  }
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  return PHDE_OK;
+}
+
+// Copies n x k int32 columns of internal rows to the caller's host buffer
+// in the caller's numbering.
+static int columns_to_host(phde_ctx *c, const int32_t *D, int k, int32_t *host) {
+  DBuf<int32_t> T;
+  const int32_t *src = D;
+  if (renumbered(c)) {
+    PHDE_TRY(T.alloc((size_t)c->g.n * k));
+    PHDE_TRY(rows_to_original(c, D, T.p, k, sizeof(int32_t)));
+    src = T.p;
+  }
+  PHDE_HIP(hipMemcpyAsync(host, src, sizeof(int32_t) * c->g.n * k, hipMemcpyDeviceToHost, c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
   return PHDE_OK;
 }
 
 int phde_bfs(phde_ctx *c, int64_t source, int alpha, int beta, int32_t *dist) {
   if (!c || !dist || source < 0 || source >= c->g.n) return PHDE_ERR_ARG;
-  PHDE_TRY(bfs_run(c, source, alpha, beta, c->d_dist, nullptr, false));
+  PHDE_TRY(bfs_run(c, to_internal(c, source), alpha, beta, c->d_dist, nullptr, false));
   PHDE_TRY(dist_finalize(c, c->d_dist, c->d_dist, kUnvisited, -1));
-  PHDE_HIP(hipMemcpyAsync(dist, c->d_dist, sizeof(int32_t) * c->g.n, hipMemcpyDeviceToHost,
-                          c->stream));
-  PHDE_HIP(hipStreamSynchronize(c->stream));
-  return PHDE_OK;
+  return columns_to_host(c, c->d_dist, 1, dist);
 }
 
 int phde_sssp(phde_ctx *c, int64_t source, int32_t delta, int32_t *dist) {
   if (!c || !dist || source < 0 || source >= c->g.n) return PHDE_ERR_ARG;
   if (!c->g.w) return PHDE_ERR_WEIGHTS;
   PHDE_TRY(bfs_setup(c));
-  PHDE_TRY(sssp_run(c, source, delta, c->d_dist, nullptr, false));
-  PHDE_HIP(hipMemcpyAsync(dist, c->d_dist, sizeof(int32_t) * c->g.n, hipMemcpyDeviceToHost,
-                          c->stream));
-  PHDE_HIP(hipStreamSynchronize(c->stream));
-  return PHDE_OK;
+  PHDE_TRY(sssp_run(c, to_internal(c, source), delta, c->d_dist, nullptr, false));
+  return columns_to_host(c, c->d_dist, 1, dist);
 }
 
 int phde_msbfs(phde_ctx *c, const int64_t *sources, int k, int32_t *dist) {
   if (!c || !sources || !dist || k < 1 || k > 64) return PHDE_ERR_ARG;
-  for (int i = 0; i < k; i++)
+  std::vector<int64_t> isrc(k);
+  for (int i = 0; i < k; i++) {
     if (sources[i] < 0 || sources[i] >= c->g.n) return PHDE_ERR_ARG;
+    isrc[i] = to_internal(c, sources[i]);
+  }
   DBuf<int32_t> D;
   PHDE_TRY(D.alloc((size_t)c->g.n * k));
-  PHDE_TRY(msbfs_run(c, sources, k, D.p));
-  PHDE_HIP(hipMemcpyAsync(dist, D.p, sizeof(int32_t) * c->g.n * k, hipMemcpyDeviceToHost,
-                          c->stream));
-  PHDE_HIP(hipStreamSynchronize(c->stream));
-  return PHDE_OK;
+  PHDE_TRY(msbfs_run(c, isrc.data(), k, D.p));
+  return columns_to_host(c, D.p, k, dist);
 }
 
 int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats *stats) {
@@ -955,9 +1021,9 @@ int phde_kcenters(phde_ctx *c, int s, int64_t first, int64_t *sources, int32_t *
     const int64_t src = kc.next;
     if (sources) sources[i] = src;
     if (c->g.w)
-      PHDE_TRY(sssp_run(c, src, 0, c->d_dist, md.p, true));
+      PHDE_TRY(sssp_run(c, to_internal(c, src), 0, c->d_dist, md.p, true));
     else
-      PHDE_TRY(bfs_run(c, src, 15, 18, c->d_dist, md.p, true));
+      PHDE_TRY(bfs_run(c, to_internal(c, src), 15, 18, c->d_dist, md.p, true));
     const KcResult r = *c->h_kc;
     if (r.unreached) {
       set_error("graph is not connected");
@@ -965,8 +1031,7 @@ int phde_kcenters(phde_ctx *c, int s, int64_t first, int64_t *sources, int32_t *
     }
     PHDE_TRY(kc.step(c, md.p, r, i != 0, true, src));
   }
-  if (min_dist)
-    PHDE_HIP(hipMemcpyAsync(min_dist, md.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  if (min_dist) return columns_to_host(c, md.p, 1, min_dist);
   PHDE_HIP(hipStreamSynchronize(c->stream));
   return PHDE_OK;
 }
@@ -1027,8 +1092,13 @@ int phde_hall_energy(phde_ctx *c, const double *Y, int nc, double *energy) {
   PHDE_TRY(Yd.alloc((size_t)c->g.n * nc));
   PHDE_TRY(part.alloc(2 * (size_t)c->num_cus * 4 + 16));
   PHDE_HIP(hipMemcpyAsync(Yd.p, Y, sizeof(double) * c->g.n * nc, hipMemcpyHostToDevice, c->stream));
+  DBuf<double> Yi;  // rows in the internal numbering
+  if (renumbered(c)) {
+    PHDE_TRY(Yi.alloc((size_t)c->g.n * nc));
+    PHDE_TRY(rows_to_internal(c, Yd.p, Yi.p, nc, sizeof(double)));
+  }
   double h[2];
-  PHDE_TRY(hall_energy_dev(c, Yd.p, nc, part.p, h));
+  PHDE_TRY(hall_energy_dev(c, Yi.p ? Yi.p : Yd.p, nc, part.p, h));
   *energy = h[0] / h[1];
   return PHDE_OK;
 }

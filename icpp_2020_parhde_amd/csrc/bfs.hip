@@ -169,7 +169,8 @@ __device__ void bfs_epilogue(const BfsArgs &a) {
         a.min_dist[i] = md;
       }
       const unsigned long long kk =
-          ((unsigned long long)(uint32_t)md << 32) | (0xffffffffull - (unsigned long long)i);
+          ((unsigned long long)(uint32_t)md << 32) |
+          (0xffffffffull - (unsigned long long)original_id(a.g, (uint32_t)i));
       key = kk > key ? kk : key;
     }
   }

@@ -602,7 +602,8 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *ar
         if (!far) continue;
         const uint32_t sdc = min((uint32_t)ld_dev(&a.sd[v]), rcap);
         const uint32_t hi = (sdc << 21) | (orderable(a.score[v]) >> 11);
-        const unsigned long long key = ((unsigned long long)hi << 32) | (0xffffffffull - (uint64_t)v);
+        const unsigned long long key =
+            ((unsigned long long)hi << 32) | (0xffffffffull - (uint64_t)original_id(a.g, (uint32_t)v));
         best = key > best ? key : best;
       }
       best = wave_max_u64(best);
@@ -615,12 +616,13 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *ar
         if (gtid == 0) st_dev(&a.best[i], 0ull);        // slot i is argmaxed again next round
         break;
       }
-      const uint32_t p = (uint32_t)(0xffffffffull - (bk & 0xffffffffull));
+      const uint32_t po = (uint32_t)(0xffffffffull - (bk & 0xffffffffull));  // caller's id
+      const uint32_t p = internal_id(a.g, po);
       rp[nr] = (int)p;
       r1[nr] = d1[p];
       r2[nr] = d2[p];
       nr++;
-      if (gtid == 0) a.picks[i] = p;
+      if (gtid == 0) a.picks[i] = po;
       i++;
     }
     // ---- bounded BFS from the round's picks
@@ -749,15 +751,15 @@ int pivot_features(phde_ctx *c, int64_t first, int64_t *second, int *ecc1, int *
   }
   int32_t *d1 = c->lp_d12, *d2 = c->lp_d12 + n;
   PHDE_TRY_RC(fill_i32(c, c->lp_sd, n, INT32_MAX));
-  PHDE_TRY_RC(bfs_run(c, first, 15, 18, d1, c->lp_sd, true));
+  PHDE_TRY_RC(bfs_run(c, to_internal(c, first), 15, 18, d1, c->lp_sd, true));
   KcResult r1 = *c->h_kc;
   if (r1.unreached) {
     set_error("graph is not connected: " + std::to_string(r1.unreached) + " vertices unreached from " +
               std::to_string(first));
     return PHDE_ERR_UNREACHED;
   }
-  const int64_t p2 = (int64_t)(0xffffffffull - (r1.key & 0xffffffffull));
-  PHDE_TRY_RC(bfs_run(c, p2, 15, 18, d2, c->lp_sd, true));
+  const int64_t p2 = (int64_t)(0xffffffffull - (r1.key & 0xffffffffull));  // caller's id
+  PHDE_TRY_RC(bfs_run(c, to_internal(c, p2), 15, 18, d2, c->lp_sd, true));
   const KcResult r2 = *c->h_kc;
   *second = p2;
   *ecc1 = (int)r1.depth;
@@ -1082,9 +1084,14 @@ int phde_pivot_features(phde_ctx *c, int64_t first, float *feat, int64_t *source
   int64_t p2 = 0;
   int e1 = 0, e2 = 0;
   PHDE_TRY_RC(pivot_features(c, first, &p2, &e1, &e2));
+  DBuf<float4> fo;  // rows in the caller's numbering
+  if (feat && renumbered(c)) {
+    PHDE_TRY_RC(fo.alloc(c->g.n));
+    PHDE_TRY_RC(rows_to_original(c, c->lp_feat, fo.p, 1, sizeof(float4)));
+  }
   if (feat)
-    PHDE_HIP(hipMemcpyAsync(feat, c->lp_feat, sizeof(float) * 4 * c->g.n, hipMemcpyDeviceToHost,
-                            c->stream));
+    PHDE_HIP(hipMemcpyAsync(feat, fo.p ? (const void *)fo.p : (const void *)c->lp_feat,
+                            sizeof(float) * 4 * c->g.n, hipMemcpyDeviceToHost, c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
   if (sources) {
     sources[0] = first;
@@ -1103,6 +1110,18 @@ int phde_set_training_data(phde_ctx *c, const float *feat, const float *target) 
     PHDE_HIP(hipMalloc(&c->lp_score, sizeof(float) * n));
   }
   if (!c->lp_target) PHDE_HIP(hipMalloc(&c->lp_target, sizeof(float) * n));
+  if (renumbered(c)) {
+    DBuf<float4> fi;
+    DBuf<float> ti;
+    PHDE_TRY_RC(fi.alloc(n));
+    PHDE_TRY_RC(ti.alloc(n));
+    PHDE_HIP(hipMemcpyAsync(fi.p, feat, sizeof(float) * 4 * n, hipMemcpyHostToDevice, c->stream));
+    PHDE_HIP(hipMemcpyAsync(ti.p, target, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    PHDE_TRY_RC(rows_to_internal(c, fi.p, c->lp_feat, 1, sizeof(float4)));
+    PHDE_TRY_RC(rows_to_internal(c, ti.p, c->lp_target, 1, sizeof(float)));
+    PHDE_HIP(hipStreamSynchronize(c->stream));
+    return PHDE_OK;
+  }
   PHDE_HIP(hipMemcpyAsync(c->lp_feat, feat, sizeof(float) * 4 * n, hipMemcpyHostToDevice, c->stream));
   PHDE_HIP(hipMemcpyAsync(c->lp_target, target, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
@@ -1112,7 +1131,13 @@ int phde_set_training_data(phde_ctx *c, const float *feat, const float *target) 
 int phde_net_scores(phde_mpnn *net, phde_ctx *c, float *score) {
   if (!net || !c || !score) return PHDE_ERR_ARG;
   PHDE_TRY_RC(net_score(c, net));
-  PHDE_HIP(hipMemcpyAsync(score, c->lp_score, sizeof(float) * c->g.n, hipMemcpyDeviceToHost, c->stream));
+  DBuf<float> so;  // rows in the caller's numbering
+  if (renumbered(c)) {
+    PHDE_TRY_RC(so.alloc(c->g.n));
+    PHDE_TRY_RC(rows_to_original(c, c->lp_score, so.p, 1, sizeof(float)));
+  }
+  PHDE_HIP(hipMemcpyAsync(score, so.p ? so.p : c->lp_score, sizeof(float) * c->g.n, hipMemcpyDeviceToHost,
+                          c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
   return PHDE_OK;
 }
@@ -1162,7 +1187,15 @@ int phde_select_pivots(phde_ctx *c, const float *score, int64_t first, int k, in
   int64_t p2 = 0;
   int e1 = 0, e2 = 0;
   PHDE_TRY_RC(pivot_features(c, first, &p2, &e1, &e2));
-  PHDE_HIP(hipMemcpyAsync(c->lp_score, score, sizeof(float) * c->g.n, hipMemcpyHostToDevice, c->stream));
+  if (renumbered(c)) {
+    DBuf<float> si;
+    PHDE_TRY_RC(si.alloc(c->g.n));
+    PHDE_HIP(hipMemcpyAsync(si.p, score, sizeof(float) * c->g.n, hipMemcpyHostToDevice, c->stream));
+    PHDE_TRY_RC(rows_to_internal(c, si.p, c->lp_score, 1, sizeof(float)));
+    PHDE_HIP(hipStreamSynchronize(c->stream));
+  } else {
+    PHDE_HIP(hipMemcpyAsync(c->lp_score, score, sizeof(float) * c->g.n, hipMemcpyHostToDevice, c->stream));
+  }
   pivots[0] = first;
   pivots[1] = p2;
   const int rc = greedy_select(c, c->lp_score, 2, k, radius, pivots);

@@ -48,7 +48,20 @@ struct DevGraph {
   const int32_t *w;      // nullptr when unweighted
   const int32_t *deg32;  // vertex degree (count of neighbours)
   const double *degw;    // degree as used by the Laplacian (count or weight sum)
+  // internal numbering (relabel.hip): caller's id of internal vertex k and
+  // internal id of caller's vertex v; both nullptr when the numbers agree
+  const uint32_t *orig;
+  const uint32_t *perm;
 };
+
+// Caller's id of internal vertex v (tie-breaks use the caller's ids).
+__device__ __forceinline__ uint32_t original_id(const DevGraph &g, uint32_t v) {
+  return g.orig ? g.orig[v] : v;
+}
+// Internal id of caller's vertex v.
+__device__ __forceinline__ uint32_t internal_id(const DevGraph &g, uint32_t v) {
+  return g.perm ? g.perm[v] : v;
+}
 
 // Timing of kernel classes with HIP events on the context stream.
 struct Timer {
@@ -77,6 +90,9 @@ struct phde_ctx {
   hipStream_t stream = nullptr;
   phde::DevGraph g{};
   uint32_t *d_rowptr = nullptr, *d_adj = nullptr;
+  // internal numbering (relabel.hip); empty / nullptr when not renumbered
+  uint32_t *d_orig = nullptr, *d_perm = nullptr;
+  std::vector<uint32_t> h_orig, h_perm;
   int32_t *d_w = nullptr, *d_deg32 = nullptr;
   double *d_degw = nullptr;
   // traversal scratch
@@ -140,6 +156,18 @@ struct DBuf {
     if (p) (void)hipFree(p);
   }
 };
+
+// relabel.hip — the context's internal numbering.  relabel_graph renumbers
+// the context's graph copy (at creation; sets *relabeled when it did).
+// rows_to_original: out[v] = in[perm[v]]; rows_to_internal: out[k] =
+// in[orig[k]]; `cols` columns of n rows, `esize`-byte elements (4, 8, 16),
+// out != in (a plain copy when the numbers agree).
+int relabel_graph(phde_ctx *c, bool *relabeled);
+int rows_to_original(phde_ctx *c, const void *in, void *out, int cols, int esize);
+int rows_to_internal(phde_ctx *c, const void *in, void *out, int cols, int esize);
+inline int64_t to_internal(const phde_ctx *c, int64_t v) { return c->h_perm.empty() ? v : c->h_perm[v]; }
+inline int64_t to_original(const phde_ctx *c, int64_t v) { return c->h_orig.empty() ? v : c->h_orig[v]; }
+inline bool renumbered(const phde_ctx *c) { return c->d_orig != nullptr; }
 
 // bfs.hip
 int bfs_setup(phde_ctx *c);

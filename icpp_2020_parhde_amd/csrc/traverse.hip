@@ -139,7 +139,8 @@ __device__ void sssp_epilogue(const SsspArgs &a) {
         a.min_dist[i] = md;
       }
       const unsigned long long kk =
-          ((unsigned long long)(uint32_t)md << 32) | (0xffffffffull - (unsigned long long)i);
+          ((unsigned long long)(uint32_t)md << 32) |
+          (0xffffffffull - (unsigned long long)original_id(a.g, (uint32_t)i));
       key = kk > key ? kk : key;
     }
   }
@@ -158,7 +159,21 @@ __device__ void sssp_epilogue(const SsspArgs &a) {
 }
 
 __global__ __launch_bounds__(kTBlock) void sssp_round_kernel(SsspArgs a, int R) {
-  if (level_finished(a.done_dev, R)) return;
+  // the round's control words are independent: load them in one batch
+  // before the early exit instead of one dependent load after another
+  // (relaxed atomic loads, so that the compiler keeps them ahead of the
+  // branch without waiting on each)
+  SsspState *sp = &a.st[R & 1];
+  const int done = __hip_atomic_load(a.done_dev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  SsspState s;
+  s.bound = __hip_atomic_load(&sp->bound, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s.epoch = __hip_atomic_load(&sp->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s.nsel = __hip_atomic_load(&sp->nsel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s.fsel = __hip_atomic_load(&sp->fsel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s.pad = 0;
+  const unsigned int nq = aload(&a.ncnt[R % 3]);
+  const unsigned int fc0 = aload(&a.fcnt[0]), fc1 = aload(&a.fcnt[1]);
+  if (done >= 0 && done + 1 < R) return;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
   const int lane = __lane_id();
@@ -166,9 +181,7 @@ __global__ __launch_bounds__(kTBlock) void sssp_round_kernel(SsspArgs a, int R) 
   const uint32_t *rowptr = a.g.rowptr, *adj = a.g.adj;
   const int32_t *wt = a.g.w;
 
-  SsspState s = a.st[R & 1];
-  const unsigned int nq = a.ncnt[R % 3];
-  const unsigned int nf = a.fcnt[s.epoch & 1];
+  const unsigned int nf = (s.epoch & 1) ? fc1 : fc0;
   if (nq == 0 && nf == 0) {
     if (gtid == 0) {
       level_done(a.done_dev, a.host_flags, R - 1);
@@ -400,13 +413,15 @@ __device__ __forceinline__ bool ms_push(const MsArgs &a, unsigned long long bits
 }
 
 __global__ __launch_bounds__(kTBlock) void msbfs_level_kernel(MsArgs a, int L) {
-  if (level_finished(a.done_dev, L)) return;
+  // control words in one batch of relaxed atomic loads ahead of the exit
+  const int done = __hip_atomic_load(a.done_dev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned int nq = aload(&a.cnt[L % 4]);
+  const unsigned int np = aload(&a.cnt[(L + 3) % 4]);
+  if (done >= 0 && done + 1 < L) return;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
   const int lane = __lane_id();
-  const int64_t gwave = gtid / kWave, nwaves = gsize / kWave;
   const uint32_t *rowptr = a.g.rowptr, *adj = a.g.adj;
-  const unsigned int nq = a.cnt[L % 4];
   if (nq == 0) {
     if (gtid == 0) level_done(a.done_dev, a.host_flags, L - 1);
     return;
@@ -417,7 +432,6 @@ __global__ __launch_bounds__(kTBlock) void msbfs_level_kernel(MsArgs a, int L) {
   }
   // retire the previous level's fresh masks (read by the previous kernel)
   if (L > 0) {
-    const unsigned int np = a.cnt[(L + 3) % 4];
     const uint32_t *qp = a.q[(L + 2) % 3];
     unsigned long long *fp = a.fr[(L + 2) % 3];
     for (int64_t i = gtid; i < (int64_t)np; i += gsize) fp[qp[i]] = 0;
@@ -435,8 +449,6 @@ __global__ __launch_bounds__(kTBlock) void msbfs_level_kernel(MsArgs a, int L) {
   __shared__ unsigned int nstage, gbase;
   if (threadIdx.x == 0) nstage = 0;
   const int w = threadIdx.x / kWave;
-  (void)gwave;
-  (void)nwaves;
   for (int64_t cb = (int64_t)blockIdx.x * kTBlock; cb < (int64_t)nq; cb += (int64_t)gridDim.x * kTBlock) {
     __syncthreads();
     const int64_t idx = cb + w * kWave + lane;

@@ -138,6 +138,14 @@ class Context:
         _check(self.lib.phde_bfs(self.h, source, alpha, beta, as_ptr(d, ctypes.c_int32)), "phde_bfs")
         return d
 
+    def vertex_order(self):
+        """(caller's ids in the context's internal order, renumbered?)."""
+        order = np.empty(self.n, dtype=np.uint32)
+        r = ctypes.c_int()
+        _check(self.lib.phde_vertex_order(self.h, as_ptr(order, ctypes.c_uint32), ctypes.byref(r)),
+               "phde_vertex_order")
+        return order, bool(r.value)
+
     def sssp(self, source: int, delta: int = 0) -> np.ndarray:
         """Exact distances (DeltaStep, gapbs/sssp.cc:53); 2^30-1 when unreached."""
         d = np.empty(self.n, dtype=np.int32)

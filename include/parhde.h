@@ -129,6 +129,14 @@ int phde_create_device(const uint32_t *rowptr_dev, const uint32_t *adj_dev,
                        const int32_t *weights_dev, int64_t n, int64_t m, phde_ctx **out);
 int phde_destroy(phde_ctx *ctx);
 int64_t phde_num_vertices(const phde_ctx *ctx);
+/* The context's internal vertex order (no reference counterpart).  A
+ * context renumbers its copy of a high-diameter graph (BFS depth from
+ * vertex 0 >= 64, n >= 65,536; environment PHDE_RELABEL=0/1 forces off/on)
+ * in the order of (level from vertex 0, level from the vertex farthest
+ * from 0, id); every argument and result of this API stays in the
+ * caller's numbering.  order (n entries, may be NULL) gets the caller's
+ * ids in internal order; *renumbered (may be NULL) 1 or 0. */
+int phde_vertex_order(const phde_ctx *ctx, uint32_t *order, int *renumbered);
 
 /* Breadth-first levels from one source: DOBFS (gapbs/bfs.cc:144-185).
  * dist[v] = hop count, -1 when unreached.  alpha/beta <= 0 use 15/18. */
