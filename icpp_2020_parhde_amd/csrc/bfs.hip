@@ -490,7 +490,8 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_level_kernel(BfsArgs a, int L) 
 
 // The whole traversal in one cooperative launch: levels separated by the
 // grid barrier above.  *arrive must be 0 at launch; *err receives 1 if a
-// barrier wait timed out.
+// barrier wait timed out (a block whose wait timed out leaves the loop, the
+// others then time out too, so the kernel always ends).
 __global__ __launch_bounds__(kBfsBlock) void bfs_persistent_kernel(BfsArgs a, unsigned int *arrive,
                                                                    unsigned int *err) {
   if (blockIdx.x % a.bstride) return;
@@ -505,8 +506,7 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_persistent_kernel(BfsArgs a, un
   const int64_t maxL = a.g.n + 1;
   for (; L <= maxL; L++) {
     if (!bfs_step(a, L, st)) break;
-    grid_barrier(arrive, nb * ++bar, err, a.backoff);
-    if (ld_dev(err)) break;
+    if (grid_barrier(arrive, nb * ++bar, err, a.backoff)) break;
   }
   if (gtid == 0) a.kc->depth = L - 1;
   bfs_epilogue(a);

@@ -54,25 +54,33 @@ __device__ __forceinline__ void st_dev(unsigned long long *p, unsigned long long
 // operations complete before the arrival (s_waitcnt), and every
 // cross-block datum is accessed with device-scope operations, so no cache
 // maintenance is needed.  The wait is bounded: on timeout the block flags
-// an error and continues, so the kernel always ends.
-__device__ __forceinline__ void grid_barrier(unsigned int *arrive, unsigned int target,
+// an error and continues, so the kernel always ends.  Returns true in the
+// blocks whose own wait timed out (the others then time out at their next
+// barrier), so callers can stop without loading the flag after every
+// barrier.
+__device__ __forceinline__ bool grid_barrier(unsigned int *arrive, unsigned int target,
                                              unsigned int *err, int backoff) {
+  __shared__ int timed_out;
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     __hip_atomic_fetch_add(arrive, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     long spins = 0;
+    int to = 0;
     while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       // polling every block on one word congests its memory channel and
       // delays unrelated loads: back off between polls
       for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(2);
       if (++spins > (1L << 25)) {
         atomicOr(err, 1u);
+        to = 1;
         break;
       }
     }
+    timed_out = to;
   }
   __syncthreads();
+  return timed_out != 0;
 }
 
 // Exclusive prefix sum over the wave; *total gets the sum of all lanes.
