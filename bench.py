@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--learned-n", type=int, default=1 << 24,
                     help="points of the second, held-out RGG for the learned comparison (0: skip)")
     ap.add_argument("--configs", type=int, default=1,
-                    help="also time RMAT-22 (ParHDE FP32) and the weighted 160^3 grid (WParHDE) with CPU samples")
+                    help="also time RMAT-22 (ParHDE FP32) and the weighted 160^3 grid (WParHDE), with CPU samples unless --cpu-baseline 0")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the layouts, run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -269,19 +269,23 @@ def other_configs(args):
                 ts.append((time.perf_counter() - t) * 1e3)
             X64, st64 = ctx.layout(precision=L.PREC_FP64, **opts)
         trav = st.kernel_ms["bfs" if algo == L.ALGO_PARHDE else "sssp"] / st.n_traversals
+        ms = float(np.mean(ts))
+        out[name] = {"n": g.n, "nnz": g.m, "gpu_ms_per_layout": round(ms, 2),
+                     "gpu_ms_per_traversal": round(trav, 3), "traversals": st.n_traversals,
+                     "gpu_fp64_ms_per_layout": round(st64.ms_total, 2),
+                     "fp32_vs_fp64_rel_err": aligned_rel_err(X32, X64)}
+        if not args.cpu_baseline:  # GPU side only (profiling runs)
+            log(f"[bench] {name}: {out[name]}")
+            del g
+            continue
         t0 = time.time()
         r = O.parhde_sample(g, s, nbfs=nbfs, c=c)
         rest = sum(v for k, v in r["phases_ms"].items() if k != "bfs_each")
         # ParHDE: the reference's eager 3s traversals; WParHDE traverses s
         cpu = r["value"] if algo == L.ALGO_PARHDE else s * r["phases_ms"]["bfs_each"] + rest
-        ms = float(np.mean(ts))
-        out[name] = {"n": g.n, "nnz": g.m, "gpu_ms_per_layout": round(ms, 2),
-                     "gpu_ms_per_traversal": round(trav, 3), "traversals": st.n_traversals,
-                     "gpu_fp64_ms_per_layout": round(st64.ms_total, 2),
-                     "fp32_vs_fp64_rel_err": aligned_rel_err(X32, X64),
-                     "cpu_ms_per_layout": round(cpu, 1), "cpu_cores": r["cores"],
-                     "cpu_traversal_ms": r["phases_ms"]["bfs_each"], "cpu_sample_s": round(time.time() - t0, 1),
-                     "speedup_vs_cpu": round(cpu / ms, 2)}
+        out[name].update({"cpu_ms_per_layout": round(cpu, 1), "cpu_cores": r["cores"],
+                          "cpu_traversal_ms": r["phases_ms"]["bfs_each"],
+                          "cpu_sample_s": round(time.time() - t0, 1), "speedup_vs_cpu": round(cpu / ms, 2)})
         log(f"[bench] {name}: {out[name]}")
         del g
     return out

@@ -155,11 +155,13 @@ static int ctx_finish(phde_ctx *c) {
   return PHDE_OK;
 }
 
-// Internal numbering of the graph copy (relabel.hip); degrees follow it.
+// Internal numbering of the graph copy (relabel.hip); degrees follow it;
+// then the SpMM's list of long rows (spmm.hip).
 static int ctx_renumber(phde_ctx *c) {
   bool done = false;
   PHDE_TRY_RC(relabel_graph(c, &done));
-  return done ? ctx_finish(c) : PHDE_OK;
+  if (done) PHDE_TRY_RC(ctx_finish(c));
+  return spmm_hubs(c);
 }
 
 static void ctx_free(phde_ctx *c) {
@@ -171,6 +173,8 @@ static void ctx_free(phde_ctx *c) {
   for (void *p : dev)
     if (p) (void)hipFree(p);
   if (c->h_kc) (void)hipHostFree(c->h_kc);
+  for (void *p : {(void *)c->sp_hub_row, (void *)c->sp_hub_c0, (void *)c->sp_chunk_row, (void *)c->sp_chunk_beg})
+    if (p) (void)hipFree(p);
   if (c->d_orig) (void)hipFree(c->d_orig);
   if (c->d_perm) (void)hipFree(c->d_perm);
   if (c->d_bar) (void)hipFree(c->d_bar);

@@ -127,6 +127,10 @@ struct phde_ctx {
   uint32_t *ms_q3 = nullptr;
   int64_t *ms_src = nullptr;
   unsigned long long *ms_dummy = nullptr;
+  // SpMM hub rows (spmm.hip): rows longer than one chunk, cut into chunks;
+  // sp_nhubs < 0 until built
+  int64_t sp_nhubs = -1, sp_nchunks = 0;
+  uint32_t *sp_hub_row = nullptr, *sp_hub_c0 = nullptr, *sp_chunk_row = nullptr, *sp_chunk_beg = nullptr;
   // learned pivot proposal (learned.hip): features (n float4), the two
   // feature traversals (2n), distance-to-pivots field, scores, training target
   float *lp_feat = nullptr;

@@ -12,6 +12,7 @@ struct Lap {
   int nblk_gram = 0;
   void *Srow = nullptr, *LSrow = nullptr;
   double *part = nullptr, *SLS = nullptr;  // SLS: s*s device, reference storage order
+  double *hub_part = nullptr;                // SpMM partials of the hub rows' chunks
   int setup(phde_ctx *c, int64_t n, int s, bool fp32);
   void release();
   ~Lap() { release(); }
@@ -19,6 +20,8 @@ struct Lap {
   int gram_self(const void *Acol);  // SLS = A^T A
 };
 
+// The SpMM's rows longer than one chunk (ctx sp_* fields), once per context.
+int spmm_hubs(phde_ctx *c);
 int coords_compute(phde_ctx *c, const void *Scol, bool fp32, int s, int nc, const double *V_dev,
                    double *X_dev, double *part, double *inv);
 int hall_energy_dev(phde_ctx *c, const double *Y_dev, int nc, double *part_dev, double *h_out2);

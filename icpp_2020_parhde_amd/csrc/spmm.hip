@@ -3,7 +3,8 @@
 //
 // Reference behaviour followed:
 //  - L*S (ParHDE.cpp:288-300, WParHDE.cpp:229-241): LS_ik = deg_i S_ik minus
-//    sum over neighbours j of (w_ij) S_jk, subtracted in adjacency order.
+//    sum over neighbours j of (w_ij) S_jk, subtracted in adjacency order
+//    (rows longer than kHubRow: chunk by chunk, see below).
 //  - S^T(LS) (DrawingBase.cpp:289-312 matrixTransMultiplication):
 //    C[k + i*s] = sum_j S_ji LS_jk, the transposed storage the eigensolver
 //    consumes.
@@ -13,9 +14,13 @@
 // Layout: the SpMM gathers whole neighbour rows, so it works on a row-major
 // copy of S (n x s, row i contiguous): one wave per vertex, lane k holds
 // column k (s <= 64 = wave width), each neighbour costs one coalesced
-// s-wide row read.  The Gram product reads row tiles of S and LS through
+// s-wide row read.  Rows of more than kHubRow neighbours (the hubs of a
+// power-law graph) would leave one wave gathering for the whole kernel:
+// they are cut into chunks of kHubRow neighbours, one wave each, whose
+// in-order partial sums a second pass subtracts in chunk order.  The Gram product reads row tiles of S and LS through
 // LDS; per-block s*s partials are summed in block order (deterministic).
 #include <algorithm>
+#include <vector>
 
 #include "phde_common.h"
 #include "spmm.h"
@@ -24,6 +29,7 @@
 namespace phde {
 
 constexpr int kBlk = 256;
+constexpr uint32_t kHubRow = 256;  // SpMM: longer rows are split into chunks of this many neighbours
 
 // S column-major [s][n] -> row-major [n][s]; 64 rows per block via LDS.
 template <typename T>
@@ -45,7 +51,45 @@ __global__ __launch_bounds__(kBlk) void col_to_row_kernel(const T *Sc, T *Sr, in
   }
 }
 
-// LS = L * S, row-major in and out.  One wave per row, lane = column.
+// v -= (w_ij) S[j][lane] for the neighbours j in adj[b, e), in order; the
+// gathers of kSpmmGroup neighbours are in flight together.
+constexpr int kSpmmGroup = 8;
+template <typename T, bool W>
+__device__ __forceinline__ double subtract_neighbours(const DevGraph &g, const T *S, int s, int lane,
+                                                      uint32_t b, uint32_t e, double v) {
+  const bool on = lane < s;
+  for (uint32_t e0 = b; e0 < e; e0 += kWave) {
+    const uint32_t cnt = min((uint32_t)kWave, e - e0);
+    const uint32_t my = (lane < (int)cnt) ? g.adj[e0 + lane] : 0u;
+    int32_t myw = 0;
+    if (W) myw = (lane < (int)cnt) ? g.w[e0 + lane] : 0;
+    uint32_t t = 0;
+    for (; t + kSpmmGroup <= cnt; t += kSpmmGroup) {
+      double x[kSpmmGroup];
+#pragma unroll
+      for (int q = 0; q < kSpmmGroup; q++) {
+        const uint32_t j = __shfl(my, t + q);
+        x[q] = on ? (double)S[(int64_t)j * s + lane] : 0.0;
+      }
+      if (W) {
+#pragma unroll
+        for (int q = 0; q < kSpmmGroup; q++) x[q] *= (double)__shfl(myw, t + q);
+      }
+#pragma unroll
+      for (int q = 0; q < kSpmmGroup; q++) v -= x[q];
+    }
+    for (; t < cnt; t++) {
+      const uint32_t j = __shfl(my, t);
+      double x = on ? (double)S[(int64_t)j * s + lane] : 0.0;
+      if (W) x *= (double)__shfl(myw, t);
+      v -= x;
+    }
+  }
+  return v;
+}
+
+// LS = L * S, row-major in and out.  One wave per row, lane = column; rows
+// longer than kHubRow are left to the two hub kernels below.
 template <typename T, bool W>
 __global__ __launch_bounds__(kBlk) void laplacian_spmm_kernel(DevGraph g, const T *S, T *LS, int s) {
   const int lane = threadIdx.x % kWave;
@@ -54,42 +98,45 @@ __global__ __launch_bounds__(kBlk) void laplacian_spmm_kernel(DevGraph g, const 
   const bool on = lane < s;
   for (int64_t i = wave; i < g.n; i += nwaves) {
     const uint32_t b = g.rowptr[i], e = g.rowptr[i + 1];
+    if (e - b > kHubRow) continue;
     double v = on ? g.degw[i] * (double)S[i * s + lane] : 0.0;
-    for (uint32_t e0 = b; e0 < e; e0 += kWave) {
-      const uint32_t cnt = min((uint32_t)kWave, e - e0);
-      const uint32_t my = (lane < (int)cnt) ? g.adj[e0 + lane] : 0u;
-      int32_t myw = 0;
-      if (W) myw = (lane < (int)cnt) ? g.w[e0 + lane] : 0;
-      uint32_t t = 0;
-      for (; t + 4 <= cnt; t += 4) {
-        const uint32_t j0 = __shfl(my, t), j1 = __shfl(my, t + 1), j2 = __shfl(my, t + 2),
-                       j3 = __shfl(my, t + 3);
-        double x0 = 0, x1 = 0, x2 = 0, x3 = 0;
-        if (on) {
-          x0 = (double)S[(int64_t)j0 * s + lane];
-          x1 = (double)S[(int64_t)j1 * s + lane];
-          x2 = (double)S[(int64_t)j2 * s + lane];
-          x3 = (double)S[(int64_t)j3 * s + lane];
-        }
-        if (W) {
-          x0 *= (double)__shfl(myw, t);
-          x1 *= (double)__shfl(myw, t + 1);
-          x2 *= (double)__shfl(myw, t + 2);
-          x3 *= (double)__shfl(myw, t + 3);
-        }
-        v -= x0;
-        v -= x1;
-        v -= x2;
-        v -= x3;
-      }
-      for (; t < cnt; t++) {
-        const uint32_t j = __shfl(my, t);
-        double x = on ? (double)S[(int64_t)j * s + lane] : 0.0;
-        if (W) x *= (double)__shfl(myw, t);
-        v -= x;
-      }
-    }
+    v = subtract_neighbours<T, W>(g, S, s, lane, b, e, v);
     if (on) LS[i * s + lane] = (T)v;
+  }
+}
+
+// Hub rows, pass 1: one wave per chunk of kHubRow neighbours, its in-order
+// partial (0 minus the chunk's terms) to part[chunk][lane].
+template <typename T, bool W>
+__global__ __launch_bounds__(kBlk) void hub_partial_kernel(DevGraph g, const T *S, int s,
+                                                           const uint32_t *chunk_row,
+                                                           const uint32_t *chunk_beg, int64_t nchunks,
+                                                           double *part) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t wave = ((int64_t)blockIdx.x * kBlk + threadIdx.x) / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlk / kWave);
+  for (int64_t c = wave; c < nchunks; c += nwaves) {
+    const uint32_t r = chunk_row[c], b = chunk_beg[c];
+    const uint32_t e = min(b + kHubRow, g.rowptr[r + 1]);
+    const double p = subtract_neighbours<T, W>(g, S, s, lane, b, e, 0.0);
+    if (lane < s) part[c * s + lane] = p;
+  }
+}
+
+// Hub rows, pass 2: deg_i S_i plus the row's chunk partials in chunk order.
+template <typename T>
+__global__ __launch_bounds__(kBlk) void hub_finish_kernel(DevGraph g, const T *S, T *LS, int s,
+                                                          const uint32_t *hub_row, const uint32_t *hub_c0,
+                                                          int64_t nhubs, const double *part) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t wave = ((int64_t)blockIdx.x * kBlk + threadIdx.x) / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlk / kWave);
+  if (lane >= s) return;
+  for (int64_t h = wave; h < nhubs; h += nwaves) {
+    const int64_t i = hub_row[h];
+    double v = g.degw[i] * (double)S[i * s + lane];
+    for (uint32_t c = hub_c0[h]; c < hub_c0[h + 1]; c++) v += part[(int64_t)c * s + lane];
+    LS[i * s + lane] = (T)v;
   }
 }
 
@@ -260,9 +307,43 @@ static int blocks_for(phde_ctx *c, int64_t work, int per_cu = 8) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(work, (int64_t)c->num_cus * per_cu));
 }
 
+// Rows longer than kHubRow and their chunks (built once per context, at
+// creation, from a host copy of the row pointers).
+int spmm_hubs(phde_ctx *c) {
+  if (c->sp_nhubs >= 0) return PHDE_OK;
+  const int64_t n = c->g.n;
+  std::vector<uint32_t> rp(n + 1);
+  PHDE_HIP(hipMemcpyAsync(rp.data(), c->g.rowptr, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  std::vector<uint32_t> hr, c0, cr, cb;
+  for (int64_t i = 0; i < n; i++) {
+    if (rp[i + 1] - rp[i] <= kHubRow) continue;
+    hr.push_back((uint32_t)i);
+    c0.push_back((uint32_t)cr.size());
+    for (uint32_t b = rp[i]; b < rp[i + 1]; b += kHubRow) {
+      cr.push_back((uint32_t)i);
+      cb.push_back(b);
+    }
+  }
+  c0.push_back((uint32_t)cr.size());
+  if (!hr.empty()) {
+    struct {
+      uint32_t **dst;
+      const std::vector<uint32_t> *src;
+    } up[] = {{&c->sp_hub_row, &hr}, {&c->sp_hub_c0, &c0}, {&c->sp_chunk_row, &cr}, {&c->sp_chunk_beg, &cb}};
+    for (auto &u : up) {
+      PHDE_HIP(hipMalloc(u.dst, sizeof(uint32_t) * u.src->size()));
+      PHDE_HIP(hipMemcpy(*u.dst, u.src->data(), sizeof(uint32_t) * u.src->size(), hipMemcpyHostToDevice));
+    }
+  }
+  c->sp_nhubs = (int64_t)hr.size();
+  c->sp_nchunks = (int64_t)cr.size();
+  return PHDE_OK;
+}
+
 template <typename T>
 static int laplacian_t(phde_ctx *c, const void *Scol, void *Srow, void *LSrow, int s, double *SLS,
-                       double *part, int nblk_gram) {
+                       double *part, int nblk_gram, double *hub_part) {
   const int64_t n = c->g.n;
   hipEvent_t ev;
   c->timer.begin(c->stream, PHDE_KERNEL_OTHER, &ev);
@@ -275,6 +356,17 @@ static int laplacian_t(phde_ctx *c, const void *Scol, void *Srow, void *LSrow, i
     laplacian_spmm_kernel<T, true><<<sb, kBlk, 0, c->stream>>>(c->g, (const T *)Srow, (T *)LSrow, s);
   else
     laplacian_spmm_kernel<T, false><<<sb, kBlk, 0, c->stream>>>(c->g, (const T *)Srow, (T *)LSrow, s);
+  if (c->sp_nhubs > 0) {
+    const int hb = blocks_for(c, (c->sp_nchunks + 3) / 4, 16);
+    if (c->g.w)
+      hub_partial_kernel<T, true><<<hb, kBlk, 0, c->stream>>>(c->g, (const T *)Srow, s, c->sp_chunk_row,
+                                                               c->sp_chunk_beg, c->sp_nchunks, hub_part);
+    else
+      hub_partial_kernel<T, false><<<hb, kBlk, 0, c->stream>>>(c->g, (const T *)Srow, s, c->sp_chunk_row,
+                                                                c->sp_chunk_beg, c->sp_nchunks, hub_part);
+    hub_finish_kernel<T><<<blocks_for(c, (c->sp_nhubs + 3) / 4, 16), kBlk, 0, c->stream>>>(
+        c->g, (const T *)Srow, (T *)LSrow, s, c->sp_hub_row, c->sp_hub_c0, c->sp_nhubs, hub_part);
+  }
   c->timer.end(c->stream, PHDE_KERNEL_SPMM, ev);
   c->timer.begin(c->stream, PHDE_KERNEL_GRAM, &ev);
   gram_partial_kernel<T><<<nblk_gram, kBlk, 0, c->stream>>>((const T *)Srow, (const T *)LSrow, n, s,
@@ -295,6 +387,8 @@ int Lap::setup(phde_ctx *c, int64_t n, int s_, bool fp32_) {
   PHDE_HIP(hipMalloc(&LSrow, esz * (size_t)n * s));
   PHDE_HIP(hipMalloc(&part, sizeof(double) * (size_t)std::max(nblk_gram, 1) * s * s));
   PHDE_HIP(hipMalloc(&SLS, sizeof(double) * s * s));
+  PHDE_TRY_RC(spmm_hubs(c));
+  if (c->sp_nchunks > 0) PHDE_HIP(hipMalloc(&hub_part, sizeof(double) * (size_t)c->sp_nchunks * s));
   return PHDE_OK;
 }
 
@@ -303,13 +397,14 @@ void Lap::release() {
   if (LSrow) (void)hipFree(LSrow);
   if (part) (void)hipFree(part);
   if (SLS) (void)hipFree(SLS);
+  if (hub_part) (void)hipFree(hub_part);
   Srow = LSrow = nullptr;
-  part = SLS = nullptr;
+  part = SLS = hub_part = nullptr;
 }
 
 int Lap::run(const void *Scol) {
-  return fp32 ? laplacian_t<float>(ctx, Scol, Srow, LSrow, s, SLS, part, nblk_gram)
-              : laplacian_t<double>(ctx, Scol, Srow, LSrow, s, SLS, part, nblk_gram);
+  return fp32 ? laplacian_t<float>(ctx, Scol, Srow, LSrow, s, SLS, part, nblk_gram, hub_part)
+              : laplacian_t<double>(ctx, Scol, Srow, LSrow, s, SLS, part, nblk_gram, hub_part);
 }
 
 // C = A^T A for a column-major n x s matrix (PivotMDS / PHDE C^T C,
