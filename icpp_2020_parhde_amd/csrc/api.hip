@@ -533,8 +533,9 @@ static int layout_hde_random(phde_ctx *c, const phde_options *o, double *coords,
 
 // ParHDE with learned pivots: the first two k-centers traversals give the
 // features and the first two columns, the network scores the vertices, the
-// greedy suppression proposes up to 64 pivots and one multi-source BFS
-// traverses the rest; CGS then takes the columns in proposal order.
+// greedy suppression proposes pivots in batches (up to 64 in all) and one
+// multi-source BFS per batch traverses them; CGS takes the columns in
+// proposal order.
 static int layout_learned(phde_ctx *c, const phde_options *o, double *coords, LayoutOut &out) {
   if (o->algorithm != PHDE_ALGO_PARHDE) {
     set_error("learned pivots are available for ParHDE only");
