@@ -138,6 +138,8 @@ GRAPH_SYMBOLS = {
     "pg_write_csr": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(PgCsr)]),
     "pg_read_csr": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(PgCsr)]),
     "pg_validate": (ctypes.c_int, [ctypes.POINTER(PgCsr)]),
+    "pg_read_mtx": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(PgCsr), ctypes.POINTER(ctypes.c_int)]),
+    "pg_mtx2csr": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, c_i64p, c_i64p]),
 }
 
 _cache: dict[str, ctypes.CDLL] = {}

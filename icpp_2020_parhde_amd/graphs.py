@@ -166,6 +166,27 @@ class Graph:
         cls._check(rc, "pg_read_csr")
         return cls._take(c, str(path))
 
+    @classmethod
+    def read_mtx(cls, path: str, lcc: bool = True) -> "Graph":
+        """Matrix Market coordinate file, read as the reference's mtx2csr.cpp
+        reads it (weights kept for integer files), then its largest
+        component unless lcc=False."""
+        c = PgCsr()
+        kind = ctypes.c_int()
+        cls._check(_native.graph_lib().pg_read_mtx(str(path).encode(), ctypes.byref(c), ctypes.byref(kind)),
+                   "pg_read_mtx")
+        g = cls._take(c, str(path))
+        return g.lcc() if lcc else g
+
+    @staticmethod
+    def mtx2csr(mtx_path: str, csr_path: str):
+        """The reference converter's whole job (mtx2csr.cpp): Matrix Market in,
+        binary CSR of the largest component out; returns (n, m) written."""
+        n, m = ctypes.c_int64(), ctypes.c_int64()
+        Graph._check(_native.graph_lib().pg_mtx2csr(str(mtx_path).encode(), str(csr_path).encode(),
+                                                     ctypes.byref(n), ctypes.byref(m)), "pg_mtx2csr")
+        return n.value, m.value
+
     def write_csr(self, path: str) -> None:
         c = self._csr()
         self._check(_native.graph_lib().pg_write_csr(str(path).encode(), ctypes.byref(c)), "pg_write_csr")

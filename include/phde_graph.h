@@ -76,10 +76,23 @@ int pg_holed_grid(int64_t rows, int64_t cols, int holes, uint64_t seed,
 int pg_from_edges(int64_t n, int64_t ne, const uint32_t *u, const uint32_t *v,
                   const int32_t *wt, pg_csr *out);
 
-/* Largest connected component (ties: lowest first vertex), relabelled in
- * old-id order.  keep_out (optional, n entries) receives new id + 1 of each
- * old vertex, 0 when dropped. */
+/* Largest connected component (ties: the one found first from the lowest
+ * vertex), relabelled in old-id order; *n_components (may be NULL) gets the
+ * number of components. */
 int pg_lcc(const pg_csr *in, pg_csr *out, int64_t *n_components);
+
+/* Matrix Market coordinate file -> CSR, read as the reference converter
+ * mtx2csr.cpp:10-157 reads it: "general" square matrices as the graph of
+ * A + A^T, rectangular ones as bipartite graphs (columns first), self loops
+ * dropped, duplicates merged keeping the smallest weight; weights kept for
+ * "integer" files only.  No component extraction.  *kind (may be NULL)
+ * gets one of PG_MTX_*. */
+enum { PG_MTX_PATTERN = 0, PG_MTX_REAL = 1, PG_MTX_INTEGER = 2 };
+int pg_read_mtx(const char *path, pg_csr *out, int *kind);
+/* The whole converter (mtx2csr.cpp main): read, largest component, write the
+ * binary CSR (weights for integer files).  *n, *m (may be NULL) get the
+ * written graph's size. */
+int pg_mtx2csr(const char *mtx_path, const char *csr_path, int64_t *n, int64_t *m);
 
 /* Binary CSR file I/O in the format above. */
 int pg_write_csr(const char *path, const pg_csr *g);
