@@ -71,6 +71,38 @@ def copy_calibration(mib: int) -> None:
     log(f"[bench] copy calibration: 3 x {mib} MiB device copies")
 
 
+def copy_bandwidth(mib: int = 1024, reps: int = 5) -> float:
+    """Measured device-to-device copy rate in GB/s (bytes read + written per
+    second, HIP events around hipMemcpy of `mib` MiB; best of `reps`): the
+    practical HBM ceiling next to the 8 TB/s peak."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    size = ctypes.c_size_t(mib << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), size) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipEventCreate(ctypes.byref(a))
+    hip.hipEventCreate(ctypes.byref(b))
+    hip.hipMemset(src, 1, size)
+    hip.hipMemcpy(dst, src, size, 3)  # hipMemcpyDeviceToDevice (warm-up)
+    best = None
+    for _ in range(reps):
+        hip.hipEventRecord(a, None)
+        hip.hipMemcpy(dst, src, size, 3)
+        hip.hipEventRecord(b, None)
+        hip.hipEventSynchronize(b)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), a, b)
+        best = ms.value if best is None else min(best, ms.value)
+    hip.hipEventDestroy(a)
+    hip.hipEventDestroy(b)
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return 2.0 * (mib << 20) / (best * 1e-3) / 1e9
+
+
 def bfs_bytes(n: int, m: int) -> int:
     # adjacency once, row pointers once, int32 level written once per vertex
     return 4 * m + 4 * (n + 1) + 4 * n
@@ -143,6 +175,7 @@ def main():
         if prec == L.PREC_FP64:
             _, sf = ctx.layout(s=args.s, num_coord=2, precision=L.PREC_FP32)
             extras["fp32_ms_per_layout"] = round(sf.ms_total, 2)
+        extras["hbm_copy_gbs"] = round(copy_bandwidth(), 1)
         log(f"[bench] extras {extras}")
     learned = None
     if args.learned:
@@ -191,6 +224,7 @@ def main():
                      "traffic": measured_traffic(workload, kernel),
                      "algorithmic_bytes": bfs_bytes(g.n, g.m)},
         "h2d_ms": round(h2d_s * 1e3, 1),
+        "ms_per_layout_incl_graph_upload": round(ms_per_layout + h2d_s * 1e3, 3),
         **extras,
         "learned_pivots": learned,
         "other_configs": other,

@@ -140,6 +140,8 @@ GRAPH_SYMBOLS = {
     "pg_validate": (ctypes.c_int, [ctypes.POINTER(PgCsr)]),
     "pg_read_mtx": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(PgCsr), ctypes.POINTER(ctypes.c_int)]),
     "pg_mtx2csr": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, c_i64p, c_i64p]),
+    "pg_draw_png": (ctypes.c_int, [ctypes.POINTER(PgCsr), c_f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_uint)]),
 }
 
 _cache: dict[str, ctypes.CDLL] = {}

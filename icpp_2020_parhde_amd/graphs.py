@@ -187,6 +187,18 @@ class Graph:
                                                      ctypes.byref(n), ctypes.byref(m)), "pg_mtx2csr")
         return n.value, m.value
 
+    def draw_png(self, coords, path: str, first: int = 0, second: int = 1):
+        """PNG of the edges at `coords` (n x c), x / y from columns first /
+        second, drawn as the reference's draw_graph.c draws them; returns
+        (width, height)."""
+        X = np.ascontiguousarray(np.asarray(coords, dtype=np.float64).T)  # column-major n x c
+        w, h = ctypes.c_uint(), ctypes.c_uint()
+        c = self._csr()
+        self._check(_native.graph_lib().pg_draw_png(ctypes.byref(c), X.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                    X.shape[0], first, second, str(path).encode(),
+                                                    ctypes.byref(w), ctypes.byref(h)), "pg_draw_png")
+        return w.value, h.value
+
     def write_csr(self, path: str) -> None:
         c = self._csr()
         self._check(_native.graph_lib().pg_write_csr(str(path).encode(), ctypes.byref(c)), "pg_write_csr")

@@ -98,6 +98,14 @@ int pg_mtx2csr(const char *mtx_path, const char *csr_path, int64_t *n, int64_t *
 int pg_write_csr(const char *path, const pg_csr *g);
 int pg_read_csr(const char *path, int weighted, pg_csr *out);
 
+/* PNG picture of a layout's edges, drawn as the reference's draw_graph.c
+ * draws them (image size from the coordinate ranges, Bresenham lines
+ * coloured by length bin; edges in order, as its single-thread run does).
+ * coords: column-major n x num_coord; first / second: 0-based coordinates
+ * for x and y.  *width / *height (may be NULL) get the image size. */
+int pg_draw_png(const pg_csr *g, const double *coords, int num_coord, int first, int second,
+                const char *path, unsigned *width, unsigned *height);
+
 /* Checks symmetry, sortedness, no self loops / duplicates, in-range ids and
  * matching weights.  Returns PG_OK or PG_ERR_FORMAT. */
 int pg_validate(const pg_csr *g);
