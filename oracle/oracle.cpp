@@ -20,11 +20,12 @@
 //   or_deltastep          scipy Dijkstra on random weighted graphs; or_dijkstra;
 //                         independence from delta.
 //   or_power_eig          the reference's own ParallelComponents.cpp compiled
-//                         from source into oracle/_ref (bit-exact), fixtures
-//                         written from it into tests/golden for the GPU box.
+//                         from source into oracle/_ref (bit-exact), and
+//                         fixtures written from it into tests/golden for
+//                         machines without the reference tree.
 //   or_parhde             the reference's pivot sequence and discard on the
-//                         128x128 mesh (captured from the compiled reference
-//                         when the path was surveyed), D-orthogonality
+//                         128x128 mesh (captured from a run of the compiled
+//                         reference, tests/golden), D-orthogonality
 //                         invariants, the closed-form projection for CGS,
 //                         dense Laplacian algebra for L*S and S^T(LS),
 //                         brute-force farthest-first pivots on small graphs.

@@ -1,5 +1,5 @@
-"""Writes the golden fixtures under tests/golden/ (run in a container that
-has the reference tree; the GPU box only reads the fixtures).
+"""Writes the golden fixtures under tests/golden/ (run where the reference
+tree exists; elsewhere the tests only read the fixtures).
 
 power_eig_*.npz: inputs and outputs of the reference's own
 eigenValuePowerMethod (variants/ParallelComponents.cpp:48-158), compiled from
