@@ -350,8 +350,10 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
 #pragma unroll
         for (int j = 0; j < kGroup; j++) {
           c[j] = g0 + j < ds;
-          vv[j] = c[j] ? adj[q.b + g0 + j] : 0u;
+          vv[j] = adj[c[j] ? q.b + g0 + j : 0u];  // unconditional: no branch per neighbour
         }
+#pragma unroll
+        for (int j = 0; j < kGroup; j++) vv[j] = c[j] ? vv[j] : 0u;
         // Visited pre-check with plain (cacheable) loads, all issued before
         // any comparison: a stale line can only still read "unvisited"
         // (levels go from kUnvisited to a value once per traversal, and
@@ -377,6 +379,13 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
           vb[j] = rowptr[r];
           ve[j] = rowptr[r + 1];
         }
+        // Keep the row-pointer loads in flight with the claims: the
+        // compiler otherwise sinks them below the claims' results (waits on
+        // them, the wave scan and the early exit first), one more round
+        // trip per group; issued here the RGG 4.2M traversal went from 23.0
+        // to 21.3 ms.  The degree sum below is taken before the early exit
+        // for the same reason.
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < kGroup; j++) c[j] = c[j] && won[j] == kUnvisited;
         int cnt = 0, hcnt = 0;
@@ -390,9 +399,9 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
         }
         int total = 0;
         const int off = wave_excl_scan(cnt, &total);
+        const unsigned long long tsum = wave_sum_u64(dsum);
         if (total == 0) continue;
         unsigned int qb = 0;
-        const unsigned long long tsum = wave_sum_u64(dsum);
         if (lane == 0) {
           qb = atomicAdd(&nxt->q, (unsigned int)total);
           atomicAdd(scout_sum, tsum);

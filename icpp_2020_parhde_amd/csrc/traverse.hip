@@ -257,11 +257,21 @@ __global__ __launch_bounds__(kTBlock) void sssp_round_kernel(SsspArgs a, int R) 
         uint32_t vv[kSsspGroup];
         int32_t nd[kSsspGroup], cur[kSsspGroup];
         bool ok[kSsspGroup];
+        // unconditional loads (index 0 for lanes past their row): a load
+        // under a lane condition whose value is used in the same branch
+        // makes the compiler wait for it there, one round trip per edge
+        int32_t wv[kSsspGroup];
 #pragma unroll
         for (int j = 0; j < kSsspGroup; j++) {
           ok[j] = t0 + j < ds;
-          vv[j] = ok[j] ? adj[b + t0 + j] : 0u;
-          nd[j] = ok[j] ? du + wt[b + t0 + j] : INT32_MAX;
+          const uint32_t k = ok[j] ? b + t0 + j : 0u;
+          vv[j] = adj[k];
+          wv[j] = wt[k];
+        }
+#pragma unroll
+        for (int j = 0; j < kSsspGroup; j++) {
+          vv[j] = ok[j] ? vv[j] : 0u;
+          nd[j] = ok[j] ? du + wv[j] : INT32_MAX;
         }
 #pragma unroll
         for (int j = 0; j < kSsspGroup; j++) cur[j] = a.dist[vv[j]];
@@ -499,7 +509,9 @@ __global__ __launch_bounds__(kTBlock) void msbfs_level_kernel(MsArgs a, int L) {
       uint32_t vv[kMsGroup];
       unsigned long long nb[kMsGroup], old[kMsGroup];
 #pragma unroll
-      for (int j = 0; j < kMsGroup; j++) vv[j] = (t0 + j < ds) ? adj[b + t0 + j] : 0u;
+      for (int j = 0; j < kMsGroup; j++) vv[j] = adj[(t0 + j < ds) ? b + t0 + j : 0u];
+#pragma unroll
+      for (int j = 0; j < kMsGroup; j++) vv[j] = (t0 + j < ds) ? vv[j] : 0u;
 #pragma unroll
       for (int j = 0; j < kMsGroup; j++) nb[j] = a.visit[vv[j]];
 #pragma unroll
