@@ -197,10 +197,14 @@ struct LevelCounts {
   unsigned long long sc;
 };
 __device__ __forceinline__ LevelCounts read_counts(const BfsSlot *cur) {
+  // two waves load at once: (q, huge) as one 64-bit word, and the degree
+  // sum (one wave's device-scope loads would each wait for the previous)
   __shared__ LevelCounts sh;
   if (threadIdx.x == 0) {
-    sh.nq = ld_dev(&cur->q);
-    sh.nh = ld_dev(&cur->huge);
+    const unsigned long long qh = ld_dev(reinterpret_cast<const unsigned long long *>(&cur->q));
+    sh.nq = (unsigned int)qh;
+    sh.nh = (unsigned int)(qh >> 32);
+  } else if (threadIdx.x == kWave) {
     sh.sc = ld_dev(&cur->scout);
   }
   __syncthreads();

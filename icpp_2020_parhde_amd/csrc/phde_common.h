@@ -27,8 +27,9 @@ constexpr int kWave = 64;
 
 // Per-level frontier counters of the persistent BFS (three rotating slots).
 struct BfsSlot {
-  unsigned int q;            // frontier vertices
-  unsigned int huge;         // of which degree >= kHugeDegree
+  // q and huge share one 8-byte word (read together)
+  alignas(8) unsigned int q;  // frontier vertices
+  unsigned int huge;          // of which degree >= kHugeDegree
   unsigned long long scout;  // sum of their degrees
   unsigned int pad[4];
 };
