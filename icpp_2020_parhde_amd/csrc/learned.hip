@@ -676,14 +676,20 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *ar
           uint32_t vv[kSelGroup];
           int32_t cur[kSelGroup];
 #pragma unroll
-          for (int j = 0; j < kSelGroup; j++) vv[j] = (t0 + j < ds) ? adj[b + t0 + j] : 0u;
+          for (int j = 0; j < kSelGroup; j++) vv[j] = adj[(t0 + j < ds) ? b + t0 + j : 0u];
+#pragma unroll
+          for (int j = 0; j < kSelGroup; j++) vv[j] = (t0 + j < ds) ? vv[j] : 0u;
 #pragma unroll
           for (int j = 0; j < kSelGroup; j++) cur[j] = ld_dev(&a.sd[vv[j]]);
 #pragma unroll
           for (int j = 0; j < kSelGroup; j++) cur[j] = (t0 + j < ds && cur[j] > nd) ? 1 : 0;
+          // every claim issued before any result is used (a claim under a
+          // lane condition would wait for its result in that branch)
+          int32_t old[kSelGroup];
 #pragma unroll
-          for (int j = 0; j < kSelGroup; j++)
-            cur[j] = cur[j] ? (atomicMin(&a.sd[vv[j]], nd) > nd ? 1 : 0) : (atomicMin(dum, nd), 0);
+          for (int j = 0; j < kSelGroup; j++) old[j] = atomicMin(cur[j] ? &a.sd[vv[j]] : dum, nd);
+#pragma unroll
+          for (int j = 0; j < kSelGroup; j++) cur[j] = (cur[j] && old[j] > nd) ? 1 : 0;
           int npush = 0;
 #pragma unroll
           for (int j = 0; j < kSelGroup; j++) npush += cur[j];
