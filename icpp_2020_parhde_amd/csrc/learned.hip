@@ -515,6 +515,7 @@ __global__ void net_bf16_copy_kernel(const float *p, uint16_t *wb) {
 
 // ------------------------------------------------------------------ selection
 
+constexpr int kSelUnroll = 4;  // vertices per thread and round trip in the argmax pass
 constexpr int kSelGroup = 8;  // neighbours a lane claims per round trip
 constexpr int kSelRound = 8;  // picks per round of the selection
 
@@ -592,19 +593,32 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *ar
     int rp[kSelRound], r1[kSelRound], r2[kSelRound];
     int nr = 0;
     while (nr < kSelRound && i < a.K) {
+      // kSelUnroll vertices per thread and round trip: all their loads
+      // issued before any is used
       unsigned long long best = 0;
-      for (int64_t v = gtid; v < n; v += gsize) {
-        bool far = true;
-        if (nr) {
-          const int x1 = d1[v], x2 = d2[v];
-          for (int j = 0; j < nr; j++) far = far && max(abs(x1 - r1[j]), abs(x2 - r2[j])) > a.r;
+      for (int64_t v0 = gtid; v0 < n; v0 += gsize * kSelUnroll) {
+        int x1[kSelUnroll], x2[kSelUnroll];
+        uint32_t sdv[kSelUnroll], oid[kSelUnroll];
+        float scv[kSelUnroll];
+#pragma unroll
+        for (int u = 0; u < kSelUnroll; u++) {
+          const int64_t v = v0 + (int64_t)u * gsize;
+          const uint32_t w = v < n ? (uint32_t)v : 0u;
+          x1[u] = d1[w];
+          x2[u] = d2[w];
+          sdv[u] = (uint32_t)ld_dev(&a.sd[w]);
+          scv[u] = a.score[w];
+          oid[u] = original_id(a.g, w);
         }
-        if (!far) continue;
-        const uint32_t sdc = min((uint32_t)ld_dev(&a.sd[v]), rcap);
-        const uint32_t hi = (sdc << 21) | (orderable(a.score[v]) >> 11);
-        const unsigned long long key =
-            ((unsigned long long)hi << 32) | (0xffffffffull - (uint64_t)original_id(a.g, (uint32_t)v));
-        best = key > best ? key : best;
+#pragma unroll
+        for (int u = 0; u < kSelUnroll; u++) {
+          bool far = v0 + (int64_t)u * gsize < n;
+          for (int j = 0; j < nr; j++) far = far && max(abs(x1[u] - r1[j]), abs(x2[u] - r2[j])) > a.r;
+          if (!far) continue;
+          const uint32_t hi = (min(sdv[u], rcap) << 21) | (orderable(scv[u]) >> 11);
+          const unsigned long long key = ((unsigned long long)hi << 32) | (0xffffffffull - (uint64_t)oid[u]);
+          best = key > best ? key : best;
+        }
       }
       best = wave_max_u64(best);
       if (lane == 0 && best) atomicMax(&a.best[i], best);
