@@ -112,6 +112,8 @@ struct BfsArgs {
   int32_t *dummy;  // one scratch word per thread of the grid
   int backoff;     // barrier poll back-off (units of s_sleep 2)
   int bstride;     // persistent kernel: only blocks with blockIdx % bstride == 0 work
+  int vpw_min;     // narrow top-down levels: fewest queue entries per wave
+  int epw;         // narrow top-down levels: edges per wave aimed at
 };
 
 // Block index and grid size of the working blocks (all of them unless the
@@ -316,14 +318,29 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
     }
     // The frontier: the queue, or after a bottom-up level its bitmap (a
     // bottom-up level builds no queue; bfs.cc:109-132 converts instead).
+    // It is cut into batches of vpw entries, one wave each, dealt to the
+    // blocks round-robin (consecutive batches on one CU queue up behind its
+    // memory pipeline: each wave gathers 16 x 64 scattered lines per phase).
+    // A narrow queue gets batches of about epw edges (the frontier's mean
+    // degree is sc / nq): high-degree frontiers are spread over more waves,
+    // while low-degree ones keep 64 entries per wave, since every wave that
+    // appends costs an atomic on the shared queue counter.
+    int vpw = kWave;
+    if (!prev_bu && !wide) {
+      const int64_t want = (int64_t)a.epw * (int64_t)nq / (int64_t)(sc > 0 ? sc : 1);
+      vpw = want < a.vpw_min ? a.vpw_min : want > kWave ? kWave : (int)want;
+    }
     const int64_t nitems = prev_bu ? ((n + 63) / 64) * 64 : (int64_t)nq;
-    for (int64_t base = gwave * kWave; base < nitems; base += nwaves * kWave) {
+    const int64_t nbatch = (nitems + vpw - 1) / vpw;
+    const int64_t wrank = (int64_t)(threadIdx.x / kWave) * work_grid(a) + work_block(a);
+    for (int64_t k = wrank; k < nbatch; k += nwaves) {
+      const int64_t base = k * vpw;
       const int64_t idx = base + lane;
       QEnt q{0u, 0u, 0u, 0u};
       if (prev_bu) {
-        const uint64_t word = bm_cur[base / 64];
+        const uint64_t word = bm_cur[k];  // vpw == 64 here
         if (idx < n && ((word >> lane) & 1ull)) q = QEnt{(uint32_t)idx, rowptr[idx], rowptr[idx + 1], 0u};
-      } else if (idx < (int64_t)nq) {
+      } else if (lane < vpw && idx < (int64_t)nq) {
         q = wide ? q_cur[idx] : ld_qent(&q_cur[idx]);
       }
       int d = (int)(q.e - q.b);
@@ -562,6 +579,10 @@ int bfs_setup(phde_ctx *c) {
   // blocks are dealt to the XCDs round-robin)
   c->bfs_stride = 1;
   if (const char *e = getenv("PHDE_BFS_STRIDE")) c->bfs_stride = std::max(1, std::min(64, atoi(e)));
+  c->bfs_vpw = 1;
+  if (const char *e = getenv("PHDE_BFS_VPW")) c->bfs_vpw = std::max(1, std::min(kWave, atoi(e)));
+  c->bfs_epw = 640;
+  if (const char *e = getenv("PHDE_BFS_EPW")) c->bfs_epw = std::max(1, std::min(1 << 20, atoi(e)));
   c->bfs_grid = std::max(1, c->num_cus) * per_cu;
   const int64_t n = c->g.n;
   const int64_t nwords = (n + 63) / 64;
@@ -608,6 +629,8 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
   a.done_dev = c->d_done;
   a.host_flags = c->d_flags;
   a.dummy = c->d_dummy;
+  a.vpw_min = c->bfs_vpw;
+  a.epw = c->bfs_epw;
   a.backoff = c->bar_backoff;
   a.bstride = c->bfs_persistent ? c->bfs_stride : 1;
   c->h_flags[kFlagDone] = -1;
