@@ -53,8 +53,7 @@ __device__ __forceinline__ void st_qent(QEnt *q, QEnt v) {
 // Append the lanes' `found` vertices (with row ranges) to the next frontier
 // (queue, huge list, degree sum).  Must be called by all 64 lanes.
 __device__ __forceinline__ void wave_append(bool found, uint32_t v, uint32_t b, uint32_t e,
-                                            QEnt *q_next, uint32_t *h_next, BfsSlot *slot,
-                                            unsigned long long *blk_scout) {
+                                            QEnt *q_next, uint32_t *h_next, BfsSlot *slot) {
   const unsigned long long mask = __ballot(found);
   if (mask == 0) return;
   const int lane = __lane_id();
@@ -65,8 +64,7 @@ __device__ __forceinline__ void wave_append(bool found, uint32_t v, uint32_t b, 
   const unsigned long long dsum = wave_sum_u64((unsigned long long)dv);
   unsigned int base = 0, hbase = 0;
   if (lane == leader) {
-    base = atomicAdd(&slot->q, (unsigned int)__popcll(mask));
-    atomicAdd(blk_scout, dsum);  // the level's device slot, or LDS on wide levels
+    base = (unsigned int)(atomicAdd(&slot->qs, ((unsigned long long)__popcll(mask) << 32) | dsum) >> 32);
     if (hmask) hbase = atomicAdd(&slot->huge, (unsigned int)__popcll(hmask));
   }
   base = __shfl(base, leader);
@@ -129,15 +127,13 @@ __device__ void bfs_init(const BfsArgs &a) {
   for (int64_t i = gtid; i < n; i += gsize) st_dev(&a.dist[i], (i == a.src) ? 0 : kUnvisited);
   if (gtid == 0) {
     const int dsrc = a.g.deg32[a.src];
-    st_dev(&a.slots[0].q, 1u);
-    st_dev(&a.slots[0].scout, (unsigned long long)dsrc);
+    st_dev(&a.slots[0].qs, (1ull << 32) | (unsigned long long)dsrc);
     st_dev(&a.slots[0].huge, dsrc >= kHugeDegree ? 1u : 0u);
     st_qent(&a.q0[0], QEnt{(uint32_t)a.src, a.g.rowptr[a.src], a.g.rowptr[a.src + 1], 0u});
     st_dev(&a.h0[0], (unsigned int)a.src);
     for (int s = 1; s < 3; s++) {
-      st_dev(&a.slots[s].q, 0u);
+      st_dev(&a.slots[s].qs, 0ull);
       st_dev(&a.slots[s].huge, 0u);
-      st_dev(&a.slots[s].scout, 0ull);
     }
     DirState d0{};
     d0.edges_to_check = a.g.m;
@@ -199,15 +195,15 @@ struct LevelCounts {
   unsigned long long sc;
 };
 __device__ __forceinline__ LevelCounts read_counts(const BfsSlot *cur) {
-  // two waves load at once: (q, huge) as one 64-bit word, and the degree
-  // sum (one wave's device-scope loads would each wait for the previous)
+  // two waves load at once: (count, degree sum), and the huge count (one
+  // wave's device-scope loads would each wait for the previous)
   __shared__ LevelCounts sh;
   if (threadIdx.x == 0) {
-    const unsigned long long qh = ld_dev(reinterpret_cast<const unsigned long long *>(&cur->q));
-    sh.nq = (unsigned int)qh;
-    sh.nh = (unsigned int)(qh >> 32);
+    const unsigned long long qs = ld_dev(&cur->qs);
+    sh.nq = (unsigned int)(qs >> 32);
+    sh.sc = qs & 0xffffffffull;
   } else if (threadIdx.x == kWave) {
-    sh.sc = ld_dev(&cur->scout);
+    sh.nh = ld_dev(&cur->huge);
   }
   __syncthreads();
   const LevelCounts r = sh;
@@ -275,24 +271,16 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
   // high-diameter graph, which keep their device-scope loads.
   const bool wide = bu || prev_bu || nq > (unsigned int)kWideFrontier;
   if (wide) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  // per-block totals of the level (degree sum of the next frontier, vertices
-  // found bottom-up), published with one device atomic per block at the end
-  // (wide levels only: on narrow ones the extra block barrier and the
-  // serial publish cost more than the contention they avoid)
-  __shared__ unsigned long long blk_scout;
+  // vertices found bottom-up: per-block total, published with one device
+  // atomic per block at the end
   __shared__ unsigned int blk_found;
-  unsigned long long *scout_sum = wide ? &blk_scout : &nxt->scout;
-  if (wide) {
-    if (threadIdx.x == 0) {
-      blk_scout = 0;
-      blk_found = 0;
-    }
+  if (bu) {
+    if (threadIdx.x == 0) blk_found = 0;
     __syncthreads();
   }
   if (gtid == 0) {
-    st_dev(&old->q, 0u);
+    st_dev(&old->qs, 0ull);
     st_dev(&old->huge, 0u);
-    st_dev(&old->scout, 0ull);
   }
   const QEnt *q_cur = (L & 1) ? a.q1 : a.q0;
   QEnt *q_next = (L & 1) ? a.q0 : a.q1;
@@ -313,7 +301,7 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
         const uint32_t v = ok ? adj[k] : 0u;
         uint32_t vb = 0, ve = 0;
         const bool f = ok && discover(a.dist, rowptr, v, next_level, &vb, &ve);
-        wave_append(f, v, vb, ve, q_next, h_next, nxt, scout_sum);
+        wave_append(f, v, vb, ve, q_next, h_next, nxt);
       }
     }
     // The frontier: the queue, or after a bottom-up level its bitmap (a
@@ -357,7 +345,7 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
           const uint32_t v = ok ? adj[k] : 0u;
           uint32_t vb = 0, ve = 0;
           const bool f = ok && discover(a.dist, rowptr, v, next_level, &vb, &ve);
-          wave_append(f, v, vb, ve, q_next, h_next, nxt, scout_sum);
+          wave_append(f, v, vb, ve, q_next, h_next, nxt);
         }
       }
       // Small lists: each lane issues the loads, checks and claims of up to
@@ -423,10 +411,8 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
         const unsigned long long tsum = wave_sum_u64(dsum);
         if (total == 0) continue;
         unsigned int qb = 0;
-        if (lane == 0) {
-          qb = atomicAdd(&nxt->q, (unsigned int)total);
-          atomicAdd(scout_sum, tsum);
-        }
+        if (lane == 0)
+          qb = (unsigned int)(atomicAdd(&nxt->qs, ((unsigned long long)total << 32) | tsum) >> 32);
         qb = __shfl(qb, 0);
         unsigned int pos = qb + (unsigned int)off;
 #pragma unroll
@@ -481,12 +467,9 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
       (void)ue;
     }
   }
-  if (wide) {
+  if (bu) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-      if (blk_scout) atomicAdd(&nxt->scout, blk_scout);
-      if (blk_found) atomicAdd(&nxt->q, blk_found);
-    }
+    if (threadIdx.x == 0 && blk_found) atomicAdd(&nxt->qs, (unsigned long long)blk_found << 32);
   }
   return true;
 }

@@ -27,11 +27,12 @@ constexpr int kWave = 64;
 
 // Per-level frontier counters of the persistent BFS (three rotating slots).
 struct BfsSlot {
-  // q and huge share one 8-byte word (read together)
-  alignas(8) unsigned int q;  // frontier vertices
-  unsigned int huge;          // of which degree >= kHugeDegree
-  unsigned long long scout;  // sum of their degrees
-  unsigned int pad[4];
+  // (frontier vertices << 32) | the sum of their degrees: one atomic adds
+  // both (a frontier is a set of vertices, so the sum is at most m < 2^32
+  // and never carries into the count)
+  unsigned long long qs;
+  unsigned int huge;  // of which degree >= kHugeDegree
+  unsigned int pad[5];
 };
 
 // Result of the fused k-centers epilogue of one traversal.
