@@ -211,10 +211,17 @@ __device__ __forceinline__ LevelCounts read_counts(const BfsSlot *cur) {
   return r;
 }
 
+// What the previous level left for guessing this one's batches: its
+// frontier count (0: no guess) and batch size.
+struct LevelGuess {
+  unsigned int nq;
+  int vpw;
+};
+
 // One BFS level.  st holds the direction-rule state on entry (the same in
-// every thread) and is advanced.  Returns false when the frontier of level
-// L is empty (the traversal is complete).
-__device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
+// every thread) and is advanced, as is gs.  Returns false when the frontier
+// of level L is empty (the traversal is complete).
+__device__ bool bfs_step(const BfsArgs &a, int L, DirState &st, LevelGuess &gs) {
   const int64_t n = a.g.n;
   const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
   const int64_t gsize = work_grid(a) * blockDim.x;
@@ -227,8 +234,25 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
   BfsSlot *cur = &a.slots[L % 3];
   BfsSlot *nxt = &a.slots[(L + 1) % 3];
   BfsSlot *old = &a.slots[(L + 2) % 3];
+  const QEnt *q_cur = (L & 1) ? a.q1 : a.q0;
+  // The wave's first batch of a top-down level's queue, loaded before the
+  // level's counts arrive so that the two round trips overlap, on the guess
+  // that the batch size stays 64 and the count below 1.5 x the previous
+  // one (+ one batch).  Used only if the batch size is 64 after all;
+  // entries below the count are final once the barrier has passed, the
+  // others are ignored.  Every lane of a guessing wave loads: indices up to
+  // n are in bounds (the queue holds n + 1 entries) and the count is at
+  // most n.
+  const int64_t wrank = (int64_t)(threadIdx.x / kWave) * work_grid(a) + work_block(a);
+  const int64_t first = wrank * kWave;
+  const bool spec = gs.nq != 0 && gs.vpw == kWave && !st.prev_bu &&
+                    first < (int64_t)gs.nq + (gs.nq >> 1) + kWave;
+  QEnt q_first{0u, 0u, 0u, 0u};
+  if (spec && first + lane <= n) q_first = ld_qent(&q_cur[first + lane]);
   const LevelCounts cnt = read_counts(cur);
   const unsigned int nq = cnt.nq;
+  gs.nq = nq;
+  gs.vpw = 0;
   if (nq == 0) return false;
   const unsigned int nh = cnt.nh;
   const unsigned long long sc = cnt.sc;
@@ -282,7 +306,6 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
     st_dev(&old->qs, 0ull);
     st_dev(&old->huge, 0u);
   }
-  const QEnt *q_cur = (L & 1) ? a.q1 : a.q0;
   QEnt *q_next = (L & 1) ? a.q0 : a.q1;
   const uint32_t *h_cur = (L & 1) ? a.h1 : a.h0;
   uint32_t *h_next = (L & 1) ? a.h0 : a.h1;
@@ -318,9 +341,9 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
       const int64_t want = (int64_t)a.epw * (int64_t)nq / (int64_t)(sc > 0 ? sc : 1);
       vpw = want < a.vpw_min ? a.vpw_min : want > kWave ? kWave : (int)want;
     }
+    gs.vpw = prev_bu ? 0 : vpw;
     const int64_t nitems = prev_bu ? ((n + 63) / 64) * 64 : (int64_t)nq;
     const int64_t nbatch = (nitems + vpw - 1) / vpw;
-    const int64_t wrank = (int64_t)(threadIdx.x / kWave) * work_grid(a) + work_block(a);
     for (int64_t k = wrank; k < nbatch; k += nwaves) {
       const int64_t base = k * vpw;
       const int64_t idx = base + lane;
@@ -329,7 +352,7 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st) {
         const uint64_t word = bm_cur[k];  // vpw == 64 here
         if (idx < n && ((word >> lane) & 1ull)) q = QEnt{(uint32_t)idx, rowptr[idx], rowptr[idx + 1], 0u};
       } else if (lane < vpw && idx < (int64_t)nq) {
-        q = wide ? q_cur[idx] : ld_qent(&q_cur[idx]);
+        q = (spec && k == wrank && vpw == kWave) ? q_first : wide ? q_cur[idx] : ld_qent(&q_cur[idx]);
       }
       int d = (int)(q.e - q.b);
       if (d >= kHugeDegree && !prev_bu) d = 0;  // expanded above
@@ -487,7 +510,8 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_level_kernel(BfsArgs a, int L) 
   if (finished) return;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   DirState st = st_sh;
-  if (!bfs_step(a, L, st)) {
+  LevelGuess gs{0u, 0};  // no guess across launches
+  if (!bfs_step(a, L, st, gs)) {
     if (gtid == 0) {
       level_done(a.done_dev, a.host_flags, L - 1);
       a.kc->depth = L - 1;
@@ -516,9 +540,10 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_persistent_kernel(BfsArgs a, un
   DirState st{};
   st.edges_to_check = a.g.m;
   int L = 0;
+  LevelGuess gs{1u, kWave};  // level 0: the source
   const int64_t maxL = a.g.n + 1;
   for (; L <= maxL; L++) {
-    if (!bfs_step(a, L, st)) break;
+    if (!bfs_step(a, L, st, gs)) break;
     if (grid_barrier(arrive, nb * ++bar, err, a.backoff)) break;
   }
   if (gtid == 0) a.kc->depth = L - 1;
@@ -564,7 +589,7 @@ int bfs_setup(phde_ctx *c) {
   if (const char *e = getenv("PHDE_BFS_STRIDE")) c->bfs_stride = std::max(1, std::min(64, atoi(e)));
   c->bfs_vpw = 1;
   if (const char *e = getenv("PHDE_BFS_VPW")) c->bfs_vpw = std::max(1, std::min(kWave, atoi(e)));
-  c->bfs_epw = 640;
+  c->bfs_epw = 1024;
   if (const char *e = getenv("PHDE_BFS_EPW")) c->bfs_epw = std::max(1, std::min(1 << 20, atoi(e)));
   c->bfs_grid = std::max(1, c->num_cus) * per_cu;
   const int64_t n = c->g.n;

@@ -110,7 +110,7 @@ struct phde_ctx {
   int bar_backoff = 1;
   int bfs_stride = 1;  // persistent BFS: every bfs_stride-th block works
   int bfs_vpw = 1;     // narrow top-down levels: fewest queue entries per wave
-  int bfs_epw = 640;   // narrow top-down levels: edges per wave aimed at
+  int bfs_epw = 1024;   // narrow top-down levels: edges per wave aimed at
   unsigned int *d_bar = nullptr, *h_bar = nullptr;  // grid-barrier counter + error flag
   int32_t *d_dummy = nullptr;                       // per-thread scratch of the BFS claims
   // mapped host flags polled by the level driver (levels.h) and their
