@@ -84,7 +84,6 @@ __device__ __forceinline__ bool discover(int32_t *dist, const uint32_t *rowptr, 
   return true;
 }
 
-constexpr int kGroup = 16;   // neighbours a lane handles per round trip (top-down)
 constexpr int kBuGroup = 8;  // neighbours a lane tests per round trip (bottom-up)
 constexpr int kWideFrontier = 65536;  // frontier size from which a level reads through the caches
 
@@ -209,6 +208,87 @@ __device__ __forceinline__ LevelCounts read_counts(const BfsSlot *cur) {
   const LevelCounts r = sh;
   __syncthreads();
   return r;
+}
+
+// Top-down expansion of the lanes' short adjacency lists (ds neighbours
+// from adj[b], fewer than kBigDegree; maxd their wave maximum), G
+// neighbours per lane per round: all loads, level checks and claims of a
+// round in flight together, the winners appended with one atomic per wave.
+template <int G>
+__device__ __forceinline__ void small_lists(const BfsArgs &a, uint32_t b, int ds, int maxd, int32_t next_level,
+                                            QEnt *q_next, uint32_t *h_next, BfsSlot *nxt) {
+  const int lane = __lane_id();
+  const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
+  const uint32_t *rowptr = a.g.rowptr;
+  const uint32_t *adj = a.g.adj;
+  for (int g0 = 0; g0 < maxd; g0 += G) {
+    uint32_t vv[G], vb[G], ve[G];
+    bool c[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      c[j] = g0 + j < ds;
+      vv[j] = adj[c[j] ? b + g0 + j : 0u];  // unconditional: no branch per neighbour
+    }
+#pragma unroll
+    for (int j = 0; j < G; j++) vv[j] = c[j] ? vv[j] : 0u;
+    // Visited pre-check with plain (cacheable) loads, all issued before
+    // any comparison: a stale line can only still read "unvisited"
+    // (levels go from kUnvisited to a value once per traversal, and
+    // each traversal starts in a new kernel), and the claim below decides.
+    int32_t dd[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) dd[j] = a.dist[c[j] ? vv[j] : 0u];
+#pragma unroll
+    for (int j = 0; j < G; j++) c[j] = c[j] && dd[j] == kUnvisited;
+    // Claims, all issued before any result is used: lanes without a
+    // candidate aim at a private dummy word, so the atomics need no
+    // branch (a predicated atomic makes the compiler wait for each result
+    // before issuing the next).  min(level) claims v exactly once: only
+    // the first caller sees kUnvisited.
+    int32_t won[G];
+    int32_t *dummy = a.dummy + gtid;
+#pragma unroll
+    for (int j = 0; j < G; j++) won[j] = atomicMin(c[j] ? &a.dist[vv[j]] : dummy, next_level);
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      const uint32_t r = c[j] ? vv[j] : 0u;
+      vb[j] = rowptr[r];
+      ve[j] = rowptr[r + 1];
+    }
+    // Keep the row-pointer loads in flight with the claims: the compiler
+    // otherwise sinks them below the claims' results (waits on them, the
+    // wave scan and the early exit first), one more round trip per group;
+    // issued here the RGG 4.2M traversal went from 23.0 to 21.3 ms.  The
+    // degree sum below is taken before the early exit for the same reason.
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < G; j++) c[j] = c[j] && won[j] == kUnvisited;
+    int cnt = 0, hcnt = 0;
+    unsigned long long dsum = 0;
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      cnt += c[j] ? 1 : 0;
+      const uint32_t dv = c[j] ? ve[j] - vb[j] : 0u;
+      dsum += dv;
+      hcnt += (c[j] && dv >= (uint32_t)kHugeDegree) ? 1 : 0;
+    }
+    int total = 0;
+    const int off = wave_excl_scan(cnt, &total);
+    const unsigned long long tsum = wave_sum_u64(dsum);
+    if (total == 0) continue;
+    unsigned int qb = 0;
+    if (lane == 0) qb = (unsigned int)(atomicAdd(&nxt->qs, ((unsigned long long)total << 32) | tsum) >> 32);
+    qb = __shfl(qb, 0);
+    unsigned int pos = qb + (unsigned int)off;
+#pragma unroll
+    for (int j = 0; j < G; j++)
+      if (c[j]) st_qent(&q_next[pos++], QEnt{vv[j], vb[j], ve[j], 0u});
+    if (__ballot(hcnt > 0)) {
+#pragma unroll
+      for (int j = 0; j < G; j++)
+        if (c[j] && ve[j] - vb[j] >= (uint32_t)kHugeDegree) st_dev(&h_next[atomicAdd(&nxt->huge, 1u)], vv[j]);
+    }
+  }
 }
 
 // What the previous level left for guessing this one's batches: its
@@ -372,81 +452,18 @@ __device__ bool bfs_step(const BfsArgs &a, int L, DirState &st, LevelGuess &gs) 
         }
       }
       // Small lists: each lane issues the loads, checks and claims of up to
-      // kGroup neighbours at once, then the wave appends all winners with
-      // one atomic.
+      // G neighbours at once, then the wave appends all winners with one
+      // atomic.  G is the wave's longest list rounded up to 8, 16 or 24 (one
+      // round), else 16 per round.
       const int ds = big ? 0 : d;
       const int maxd = wave_max_i32(ds);
-      for (int g0 = 0; g0 < maxd; g0 += kGroup) {
-        uint32_t vv[kGroup], vb[kGroup], ve[kGroup];
-        bool c[kGroup];
-#pragma unroll
-        for (int j = 0; j < kGroup; j++) {
-          c[j] = g0 + j < ds;
-          vv[j] = adj[c[j] ? q.b + g0 + j : 0u];  // unconditional: no branch per neighbour
-        }
-#pragma unroll
-        for (int j = 0; j < kGroup; j++) vv[j] = c[j] ? vv[j] : 0u;
-        // Visited pre-check with plain (cacheable) loads, all issued before
-        // any comparison: a stale line can only still read "unvisited"
-        // (levels go from kUnvisited to a value once per traversal, and
-        // each traversal starts in a new kernel), and the CAS below decides.
-        int32_t dd[kGroup];
-#pragma unroll
-        for (int j = 0; j < kGroup; j++) dd[j] = a.dist[c[j] ? vv[j] : 0u];
-#pragma unroll
-        for (int j = 0; j < kGroup; j++) c[j] = c[j] && dd[j] == kUnvisited;
-        // Claims, all issued before any result is used: lanes without a
-        // candidate aim at a private dummy word, so the atomics need no
-        // branch (a predicated atomic makes the compiler wait for each result
-        // before issuing the next).  min(level) claims v exactly once: only
-        // the first caller sees kUnvisited.
-        int32_t won[kGroup];
-        int32_t *dummy = a.dummy + gtid;
-#pragma unroll
-        for (int j = 0; j < kGroup; j++)
-          won[j] = atomicMin(c[j] ? &a.dist[vv[j]] : dummy, next_level);
-#pragma unroll
-        for (int j = 0; j < kGroup; j++) {
-          const uint32_t r = c[j] ? vv[j] : 0u;
-          vb[j] = rowptr[r];
-          ve[j] = rowptr[r + 1];
-        }
-        // Keep the row-pointer loads in flight with the claims: the
-        // compiler otherwise sinks them below the claims' results (waits on
-        // them, the wave scan and the early exit first), one more round
-        // trip per group; issued here the RGG 4.2M traversal went from 23.0
-        // to 21.3 ms.  The degree sum below is taken before the early exit
-        // for the same reason.
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int j = 0; j < kGroup; j++) c[j] = c[j] && won[j] == kUnvisited;
-        int cnt = 0, hcnt = 0;
-        unsigned long long dsum = 0;
-#pragma unroll
-        for (int j = 0; j < kGroup; j++) {
-          cnt += c[j] ? 1 : 0;
-          const uint32_t dv = c[j] ? ve[j] - vb[j] : 0u;
-          dsum += dv;
-          hcnt += (c[j] && dv >= (uint32_t)kHugeDegree) ? 1 : 0;
-        }
-        int total = 0;
-        const int off = wave_excl_scan(cnt, &total);
-        const unsigned long long tsum = wave_sum_u64(dsum);
-        if (total == 0) continue;
-        unsigned int qb = 0;
-        if (lane == 0)
-          qb = (unsigned int)(atomicAdd(&nxt->qs, ((unsigned long long)total << 32) | tsum) >> 32);
-        qb = __shfl(qb, 0);
-        unsigned int pos = qb + (unsigned int)off;
-#pragma unroll
-        for (int j = 0; j < kGroup; j++)
-          if (c[j]) st_qent(&q_next[pos++], QEnt{vv[j], vb[j], ve[j], 0u});
-        if (__ballot(hcnt > 0)) {
-#pragma unroll
-          for (int j = 0; j < kGroup; j++)
-            if (c[j] && ve[j] - vb[j] >= (uint32_t)kHugeDegree)
-              st_dev(&h_next[atomicAdd(&nxt->huge, 1u)], vv[j]);
-        }
+      if (maxd > 0) {
+        if (maxd <= 8)
+          small_lists<8>(a, q.b, ds, maxd, next_level, q_next, h_next, nxt);
+        else if (maxd <= 16 || maxd > 24)
+          small_lists<16>(a, q.b, ds, maxd, next_level, q_next, h_next, nxt);
+        else
+          small_lists<24>(a, q.b, ds, maxd, next_level, q_next, h_next, nxt);
       }
     }
   } else {
