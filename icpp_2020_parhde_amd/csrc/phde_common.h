@@ -179,6 +179,9 @@ inline bool renumbered(const phde_ctx *c) { return c->d_orig != nullptr; }
 
 // bfs.hip
 int bfs_setup(phde_ctx *c);
+// Called once a graph is known to have a deep BFS: the persistent traversal
+// then runs on half the CUs (see bfs.hip).
+void bfs_deep_graph(phde_ctx *c);
 // Runs one traversal from `src`, writing levels into dist_out (device,
 // kUnvisited for unreached).  When min_dist != nullptr the k-centers update
 // and argmax are fused; result in c->d_kc (device) and, after the call

@@ -157,6 +157,7 @@ int relabel_graph(phde_ctx *c, bool *relabeled) {
   PHDE_TRY_RC(fill_i32(c, md.p, n, INT32_MAX));
   PHDE_TRY_RC(bfs_run(c, 0, 15, 18, d1.p, md.p, true));
   const KcResult r1 = *c->h_kc;
+  if (r1.depth >= kMinDepth) bfs_deep_graph(c);
   if (mode < 0 && r1.depth < kMinDepth) return PHDE_OK;
   // the farthest vertex from 0 (lowest id on ties; an unreached one when
   // the graph is not connected — any order is valid, this one is still
