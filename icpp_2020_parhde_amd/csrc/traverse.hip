@@ -288,14 +288,17 @@ __global__ __launch_bounds__(kTBlock) void sssp_round_kernel(SsspArgs a, int R) 
           fr[j] = imp && !(nd[j] < s.bound);
           if (fr[j]) fm = min(fm, nd[j]);
         }
+        // One stamp exchange per edge: an improvement goes to the near
+        // queue or to the far pile, never both, so the near and far stamps
+        // are swapped in one batch (one round trip instead of two).
 #pragma unroll
-        for (int j = 0; j < kSsspGroup; j++) cur[j] = atomicExch(nr[j] ? &a.nstamp[vv[j]] : dum, R + 1);
+        for (int j = 0; j < kSsspGroup; j++)
+          cur[j] = atomicExch(nr[j] ? &a.nstamp[vv[j]] : fr[j] ? &a.fstamp[vv[j]] : dum, nr[j] ? R + 1 : s.epoch);
 #pragma unroll
-        for (int j = 0; j < kSsspGroup; j++) nr[j] = nr[j] && cur[j] != R + 1;
-#pragma unroll
-        for (int j = 0; j < kSsspGroup; j++) cur[j] = atomicExch(fr[j] ? &a.fstamp[vv[j]] : dum, s.epoch);
-#pragma unroll
-        for (int j = 0; j < kSsspGroup; j++) fr[j] = fr[j] && cur[j] != s.epoch;
+        for (int j = 0; j < kSsspGroup; j++) {
+          nr[j] = nr[j] && cur[j] != R + 1;
+          fr[j] = fr[j] && cur[j] != s.epoch;
+        }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) fm = min(fm, __shfl_xor(fm, o));
         if (lane == 0 && fm != INT32_MAX) atomicMin(&a.fmin[s.epoch & 1], fm);
