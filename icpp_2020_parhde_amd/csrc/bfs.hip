@@ -19,7 +19,11 @@
 // `level == L` on the distance array instead, so top-down levels never touch
 // a bitmap.  Queue appends are wave-aggregated (one atomic per wave).
 // Vertices of degree >= kHugeDegree are expanded by the whole grid, those of
-// degree >= kBigDegree by one wave, the rest one lane each.
+// degree >= kBigDegree by one wave, the rest one lane each.  The same levels
+// run in one cooperative launch with a grid barrier between them
+// (bfs_persistent_kernel), and graphs with a deep BFS take the multi-hop
+// traversal (bfs_hops_kernel: several hops per barrier, label-correcting,
+// exact levels).
 #include <algorithm>
 #include <cstdio>
 
@@ -34,20 +38,21 @@ constexpr int kHugeDegree = 8192;
 constexpr int kBigDegree = 32;
 
 // Frontier queue entry: the vertex and its adjacency range, so the next
-// level reads one 16-byte entry instead of entry + row pointers.
+// level reads one 16-byte entry instead of entry + row pointers; lvl is the
+// vertex's level when it was queued (used by the multi-hop traversal).
 struct QEnt {
-  uint32_t v, b, e, pad;
+  uint32_t v, b, e, lvl;
 };
 
 __device__ __forceinline__ QEnt ld_qent(const QEnt *q) {
   const unsigned long long *p = reinterpret_cast<const unsigned long long *>(q);
   const unsigned long long x = ld_dev(p), y = ld_dev(p + 1);
-  return QEnt{(uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, 0u};
+  return QEnt{(uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, (uint32_t)(y >> 32)};
 }
 __device__ __forceinline__ void st_qent(QEnt *q, QEnt v) {
   unsigned long long *p = reinterpret_cast<unsigned long long *>(q);
   st_dev(p, (unsigned long long)v.v | ((unsigned long long)v.b << 32));
-  st_dev(p + 1, (unsigned long long)v.e);
+  st_dev(p + 1, (unsigned long long)v.e | ((unsigned long long)v.lvl << 32));
 }
 
 // Append the lanes' `found` vertices (with row ranges) to the next frontier
@@ -111,6 +116,10 @@ struct BfsArgs {
   int bstride;     // persistent kernel: only blocks with blockIdx % bstride == 0 work
   int vpw_min;     // narrow top-down levels: fewest queue entries per wave
   int epw;         // narrow top-down levels: edges per wave aimed at
+  int hops;        // multi-hop traversal: hops a wave expands per grid barrier
+  unsigned int *overflow;  // multi-hop traversal: set when the queue would overflow
+  uint32_t qcap;           // multi-hop traversal: queue entries usable (0: n + 1)
+  int hop_batch;           // multi-hop traversal: queue entries a wave starts a round with
 };
 
 // Block index and grid size of the working blocks (all of them unless the
@@ -140,25 +149,31 @@ __device__ void bfs_init(const BfsArgs &a) {
     st_dev(&a.kc->key, 0ull);
     st_dev(&a.kc->sumsq, 0ull);
     st_dev(&a.kc->unreached, 0ull);
+    st_dev(reinterpret_cast<unsigned long long *>(&a.kc->depth), 0ull);
     *a.done_dev = -1;
   }
 }
 
 __global__ __launch_bounds__(kBfsBlock) void bfs_init_kernel(BfsArgs a) { bfs_init(a); }
 
-// Fused epilogue: k-centers update + argmax, sum of squares, unreached.
-__device__ void bfs_epilogue(const BfsArgs &a) {
+// Fused epilogue: k-centers update + argmax, sum of squares, unreached;
+// with want_depth also the largest level (the source's eccentricity) into
+// kc->depth, which must be 0 beforehand.
+__device__ void bfs_epilogue(const BfsArgs &a, bool want_depth = false) {
   const int64_t n = a.g.n;
   const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
   const int64_t gsize = work_grid(a) * blockDim.x;
   const int lane = __lane_id();
   unsigned long long key = 0, sumsq = 0, unreached = 0;
+  int32_t maxd = 0;
   for (int64_t i = gtid; i < n; i += gsize) {
     const int32_t d = ld_dev(&a.dist[i]);
-    if (d == kUnvisited)
+    if (d == kUnvisited) {
       unreached++;
-    else
+    } else {
       sumsq += (unsigned long long)d * (unsigned long long)d;
+      maxd = d > maxd ? d : maxd;
+    }
     if (a.min_dist) {
       int32_t md = a.min_dist[i];
       if (d < md) {
@@ -178,10 +193,12 @@ __device__ void bfs_epilogue(const BfsArgs &a) {
     sumsq += __shfl_xor(sumsq, o);
     unreached += __shfl_xor(unreached, o);
   }
+  if (want_depth) maxd = wave_max_i32(maxd);
   if (lane == 0) {
     if (a.min_dist && key) atomicMax(&a.kc->key, key);
     if (sumsq) atomicAdd(&a.kc->sumsq, sumsq);
     if (unreached) atomicAdd(&a.kc->unreached, unreached);
+    if (want_depth && maxd > 0) atomicMax(reinterpret_cast<unsigned long long *>(&a.kc->depth), (unsigned long long)maxd);
   }
 }
 
@@ -567,6 +584,209 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_persistent_kernel(BfsArgs a, un
   bfs_epilogue(a);
 }
 
+
+// ------------------------------------------------------------------ multi-hop traversal
+// For deep graphs (thousands of narrow levels) the grid barrier between
+// levels dominates.  This traversal lets each wave expand its batch of the
+// frontier for `hops` hops between two grid barriers, keeping the vertices
+// it discovers in a wave-local LDS queue (64 entries; the rest and the last
+// hop's discoveries go to the global queue for the next round).  A vertex u
+// reached from v (level l) is claimed with atomicMin(dist[u], l + 1) and
+// expanded again whenever that lowers its level, so a vertex first reached
+// by a longer path (another wave was slower to reach it) is corrected when
+// the shorter path arrives: the levels at the end are the exact BFS levels,
+// only the amount of work depends on timing (label-correcting search).
+// Entries carry their level.  If a round would queue more entries than the
+// queue holds, *overflow is set and the host repeats the traversal level by
+// level.
+
+// Wave-collective: place each lane's winners (c[j]: vertex vv[j], rows
+// [vb[j], ve[j]), level nl) in the wave's local queue loc while it has room
+// (only when `local`), the rest in the global queue.  loc_n is the local
+// count (wave-uniform), updated.
+template <int G>
+__device__ __forceinline__ void hop_emit(const BfsArgs &a, const bool *c, const uint32_t *vv, const uint32_t *vb,
+                                         const uint32_t *ve, uint32_t nl, bool local, QEnt *loc, int &loc_n,
+                                         QEnt *q_next, BfsSlot *nxt, uint32_t cap) {
+  int cnt = 0;
+#pragma unroll
+  for (int j = 0; j < G; j++) cnt += c[j] ? 1 : 0;
+  int total = 0;
+  const int off = wave_excl_scan(cnt, &total);
+  if (total == 0) return;
+  const int avail = local ? kWave - loc_n : 0;
+  const int nglob = total > avail ? total - avail : 0;
+  unsigned int base = 0;
+  if (__lane_id() == 0 && nglob)
+    base = (unsigned int)(atomicAdd(&nxt->qs, (unsigned long long)nglob << 32) >> 32);
+  base = __shfl(base, 0);
+  int r = off;
+#pragma unroll
+  for (int j = 0; j < G; j++) {
+    if (c[j]) {
+      const QEnt ent{vv[j], vb[j], ve[j], nl};
+      if (r < avail) {
+        loc[loc_n + r] = ent;
+      } else {
+        const unsigned int gi = base + (unsigned int)(r - avail);
+        if (gi < cap)
+          st_qent(&q_next[gi], ent);
+        else
+          atomicOr(a.overflow, 1u);
+      }
+      r++;
+    }
+  }
+  loc_n = loc_n + total < kWave ? loc_n + total : kWave;
+  if (!local) loc_n = 0;
+}
+
+// One hop from the lanes' entries e (valid lanes only): the rows of kBigDegree
+// or more neighbours by the whole wave, the others G neighbours per lane per
+// round trip, as in small_lists.
+template <int G>
+__device__ __forceinline__ void hop_small(const BfsArgs &a, const QEnt &e, int ds, int maxd, bool local, QEnt *loc,
+                                          int &loc_n, QEnt *q_next, BfsSlot *nxt, uint32_t cap) {
+  const uint32_t *rowptr = a.g.rowptr;
+  const uint32_t *adj = a.g.adj;
+  const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
+  const int32_t nl = (int32_t)e.lvl + 1;
+  for (int g0 = 0; g0 < maxd; g0 += G) {
+    uint32_t vv[G], vb[G], ve[G];
+    bool c[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      c[j] = g0 + j < ds;
+      vv[j] = adj[c[j] ? e.b + g0 + j : 0u];
+    }
+#pragma unroll
+    for (int j = 0; j < G; j++) vv[j] = c[j] ? vv[j] : 0u;
+    // plain loads: a stale line can only hold a larger level
+    int32_t dd[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) dd[j] = a.dist[c[j] ? vv[j] : 0u];
+#pragma unroll
+    for (int j = 0; j < G; j++) c[j] = c[j] && dd[j] > nl;
+    int32_t won[G];
+    int32_t *dummy = a.dummy + gtid;
+#pragma unroll
+    for (int j = 0; j < G; j++) won[j] = atomicMin(c[j] ? &a.dist[vv[j]] : dummy, nl);
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      const uint32_t r = c[j] ? vv[j] : 0u;
+      vb[j] = rowptr[r];
+      ve[j] = rowptr[r + 1];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < G; j++) c[j] = c[j] && won[j] > nl;
+    hop_emit<G>(a, c, vv, vb, ve, (uint32_t)nl, local, loc, loc_n, q_next, nxt, cap);
+  }
+}
+
+__device__ void hop_expand(const BfsArgs &a, const QEnt &e, bool valid, bool local, QEnt *loc, int &loc_n,
+                           QEnt *q_next, BfsSlot *nxt, uint32_t cap) {
+  const uint32_t *rowptr = a.g.rowptr;
+  const uint32_t *adj = a.g.adj;
+  const int lane = __lane_id();
+  const int d = valid ? (int)(e.e - e.b) : 0;
+  const bool big = d >= kBigDegree;
+  unsigned long long bmask = __ballot(big);
+  while (bmask) {
+    const int l = __ffsll((long long)bmask) - 1;
+    bmask &= bmask - 1;
+    const uint32_t bb = __shfl(e.b, l), be = __shfl(e.e, l);
+    const uint32_t nl = __shfl(e.lvl, l) + 1u;
+    for (uint32_t k0 = bb; k0 < be; k0 += kWave) {
+      const uint32_t k = k0 + lane;
+      bool f = false;
+      uint32_t v = 0, vb = 0, ve = 0;
+      if (k < be) {
+        v = adj[k];
+        if (a.dist[v] > (int32_t)nl && atomicMin(&a.dist[v], (int32_t)nl) > (int32_t)nl) {
+          f = true;
+          vb = rowptr[v];
+          ve = rowptr[v + 1];
+        }
+      }
+      hop_emit<1>(a, &f, &v, &vb, &ve, nl, local, loc, loc_n, q_next, nxt, cap);
+    }
+  }
+  const int ds = big ? 0 : d;
+  const int maxd = wave_max_i32(ds);
+  if (maxd > 0) {
+    if (maxd <= 8)
+      hop_small<8>(a, e, ds, maxd, local, loc, loc_n, q_next, nxt, cap);
+    else if (maxd <= 16 || maxd > 24)
+      hop_small<16>(a, e, ds, maxd, local, loc, loc_n, q_next, nxt, cap);
+    else
+      hop_small<24>(a, e, ds, maxd, local, loc, loc_n, q_next, nxt, cap);
+  }
+}
+
+// The multi-hop traversal in one cooperative launch.  *arrive must be 0 at
+// launch; *err receives 1 if a barrier wait timed out.
+__global__ __launch_bounds__(kBfsBlock) void bfs_hops_kernel(BfsArgs a, unsigned int *arrive, unsigned int *err) {
+  if (blockIdx.x % a.bstride) return;
+  const unsigned int nb = (unsigned int)work_grid(a);
+  const int64_t n = a.g.n;
+  const uint32_t cap = (a.qcap > 0 && a.qcap < (uint64_t)(n + 1)) ? a.qcap : (uint32_t)(n + 1);  // queue entries
+  const int lane = __lane_id();
+  const int w = threadIdx.x / kWave;
+  const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
+  const int64_t nwaves = work_grid(a) * (blockDim.x / kWave);
+  const int64_t wrank = (int64_t)w * work_grid(a) + work_block(a);
+  __shared__ QEnt locq[2][kBfsBlock / kWave][kWave];
+  unsigned int bar = 0;
+  bfs_init(a);
+  if (grid_barrier(arrive, nb * ++bar, err, a.backoff)) return;
+  const int64_t maxR = n + 1;
+  bool finished = false;
+  for (int64_t R = 0; R <= maxR; R++) {
+    BfsSlot *cur = &a.slots[R % 3];
+    BfsSlot *nxt = &a.slots[(R + 1) % 3];
+    BfsSlot *old = &a.slots[(R + 2) % 3];
+    const LevelCounts cnt = read_counts(cur);
+    if (cnt.nq == 0) {
+      finished = true;
+      break;
+    }
+    const uint32_t nq = cnt.nq < cap ? cnt.nq : cap;
+    if (gtid == 0) {
+      st_dev(&old->qs, 0ull);
+      st_dev(&old->huge, 0u);
+    }
+    const QEnt *q_cur = (R & 1) ? a.q1 : a.q0;
+    QEnt *q_next = (R & 1) ? a.q0 : a.q1;
+    const int hb = a.hop_batch;
+    const int64_t nbatch = ((int64_t)nq + hb - 1) / hb;
+    for (int64_t k = wrank; k < nbatch; k += nwaves) {
+      const int64_t idx = k * hb + lane;
+      bool valid = lane < hb && idx < (int64_t)nq;
+      QEnt e{0u, 0u, 0u, 0u};
+      if (valid) e = ld_qent(&q_cur[idx]);
+      int sel = 0;
+      for (int hop = 0; hop < a.hops; hop++) {
+        const bool local = hop + 1 < a.hops;
+        int loc_n = 0;
+        QEnt *loc = &locq[sel ^ 1][w][0];
+        hop_expand(a, e, valid, local, loc, loc_n, q_next, nxt, cap);
+        if (!local || loc_n == 0) break;
+        // the wave's own LDS writes, read back by other lanes of the wave
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        valid = lane < loc_n;
+        e = valid ? loc[lane] : QEnt{0u, 0u, 0u, 0u};
+        __builtin_amdgcn_wave_barrier();
+        sel ^= 1;
+      }
+    }
+    if (grid_barrier(arrive, nb * ++bar, err, a.backoff)) return;
+  }
+  if (!finished && gtid == 0) atomicOr(a.overflow, 1u);  // round limit reached
+  bfs_epilogue(a, true);
+}
+
 __global__ void dist_finalize_kernel(const int32_t *in, int32_t *out, int64_t n, int32_t from,
                                      int32_t to) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -604,6 +824,13 @@ int bfs_setup(phde_ctx *c) {
   // blocks are dealt to the XCDs round-robin)
   c->bfs_stride = 1;
   if (const char *e = getenv("PHDE_BFS_STRIDE")) c->bfs_stride = std::max(1, std::min(64, atoi(e)));
+  c->bfs_hops = 0;
+  if (const char *e = getenv("PHDE_BFS_HOPS")) c->bfs_hops = std::max(0, std::min(64, atoi(e)));
+  // a smaller multi-hop queue (tests of the level-by-level repeat)
+  c->bfs_hop_batch = 16;
+  if (const char *e = getenv("PHDE_BFS_HOP_BATCH")) c->bfs_hop_batch = std::max(1, std::min(kWave, atoi(e)));
+  c->bfs_qcap = 0;
+  if (const char *e = getenv("PHDE_BFS_QCAP")) c->bfs_qcap = (uint32_t)std::max(1, atoi(e));
   c->bfs_vpw = 1;
   if (const char *e = getenv("PHDE_BFS_VPW")) c->bfs_vpw = std::max(1, std::min(kWave, atoi(e)));
   c->bfs_epw = 1024;
@@ -628,14 +855,20 @@ int bfs_setup(phde_ctx *c) {
 }
 
 // A graph with a deep BFS (thousands of narrow levels, each bound by its
-// chain of dependent round trips and the grid barrier) traverses faster on
-// the even blocks only: one block per CU on 4 of the 8 XCDs (blocks are
-// dealt to the XCDs round-robin).  Measured: RGG 4.2M 16.5 -> 15.6 ms per
-// traversal; the same 128 blocks on all 8 XCDs 16.3 ms, on 2 XCDs 15.8 ms.
-// Low-diameter graphs keep every CU for their wide levels (RMAT scale 22:
-// 0.78 ms on 256 CUs, 0.97 ms on 128).
+// chain of dependent round trips and the grid barrier) takes the multi-hop
+// traversal: 12 hops per grid barrier from batches of 16 queue entries per
+// wave.  Measured on RGG 4.2M: 15.5 ms per traversal level by level, 7.7 ms
+// multi-hop (8.1 at 8 hops, 7.8 at 16, 8.4 at 24; 9.4 from batches of 8,
+// 7.8 from 24, 11.0 from 48 at 3 hops); RGG 16.8M 37.9 -> 19.5 ms.  Level
+// by level (PHDE_BFS_HOPS=0) such a graph runs faster on the even blocks
+// only, one block per CU on 4 of the 8 XCDs (blocks are dealt to the XCDs
+// round-robin): 16.5 -> 15.6 ms; the multi-hop traversal prefers every CU
+// (7.7 vs 8.0 ms).  Low-diameter graphs keep the level-by-level traversal on
+// every CU: their wide levels need the whole chip, and the multi-hop one
+// re-expands too much (RMAT scale 22: 0.78 ms, 46 ms at 4 hops).
 void bfs_deep_graph(phde_ctx *c) {
-  if (!getenv("PHDE_BFS_STRIDE") && c->bfs_grid >= 16) c->bfs_stride = 2;
+  if (!getenv("PHDE_BFS_HOPS")) c->bfs_hops = 12;
+  if (!getenv("PHDE_BFS_STRIDE") && c->bfs_hops <= 1 && c->bfs_grid >= 16) c->bfs_stride = 2;
 }
 
 int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, int32_t *min_dist,
@@ -674,12 +907,18 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
   hipEvent_t ev = nullptr;
   c->timer.begin(c->stream, PHDE_KERNEL_BFS_LEVEL, &ev);
   const int grid = c->bfs_grid;
+  bool hops = c->bfs_persistent && c->bfs_hops > 1;
+  a.hops = c->bfs_hops;
+  a.qcap = c->bfs_qcap;
+  a.hop_batch = c->bfs_hop_batch;
+  a.overflow = c->d_bar + 2;
   if (c->bfs_persistent) {
-    PHDE_TRY_RC(fill_u32(c, c->d_bar, 2, 0u));
+    PHDE_TRY_RC(fill_u32(c, c->d_bar, 3, 0u));
     unsigned int *arrive = c->d_bar, *err = c->d_bar + 1;
     void *args[] = {&a, &arrive, &err};
-    const hipError_t e = hipLaunchCooperativeKernel((const void *)bfs_persistent_kernel, dim3(grid),
-                                                    dim3(kBfsBlock), args, 0, c->stream);
+    const hipError_t e =
+        hipLaunchCooperativeKernel(hops ? (const void *)bfs_hops_kernel : (const void *)bfs_persistent_kernel,
+                                   dim3(grid), dim3(kBfsBlock), args, 0, c->stream);
     if (e != hipSuccess) {
       // cooperative launches can be unavailable (measured: in a second
       // process sharing the GPU); this context then launches one kernel
@@ -687,6 +926,7 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
       (void)hipGetLastError();
       c->bfs_persistent = false;
       a.bstride = 1;
+      hops = false;
     }
   }
   if (!c->bfs_persistent) {
@@ -703,12 +943,23 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
   if (sync_result || c->bfs_persistent) {
     PHDE_HIP(hipMemcpyAsync(c->h_kc, c->d_kc, sizeof(KcResult), hipMemcpyDeviceToHost, c->stream));
     if (c->bfs_persistent)
-      PHDE_HIP(hipMemcpyAsync(c->h_bar, c->d_bar + 1, sizeof(unsigned int), hipMemcpyDeviceToHost,
+      PHDE_HIP(hipMemcpyAsync(c->h_bar, c->d_bar + 1, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost,
                               c->stream));
     PHDE_HIP(hipStreamSynchronize(c->stream));
-    if (c->bfs_persistent && *c->h_bar) {
+    if (c->bfs_persistent && c->h_bar[0]) {
       set_error("BFS grid barrier timed out (blocks not co-resident?)");
       return PHDE_ERR_DEVICE;
+    }
+    if (hops && c->h_bar[1]) {
+      // the multi-hop traversal outgrew its queue: repeat it level by level.
+      // Its epilogue has already folded its levels into min_dist, but every
+      // level it wrote is the length of a real path, so at least the exact
+      // level, and min_dist ends the same after the exact pass.
+      const int saved = c->bfs_hops;
+      c->bfs_hops = 0;
+      rc = bfs_run(c, src, alpha, beta, dist_out, min_dist, sync_result);
+      c->bfs_hops = saved;
+      return rc;
     }
   }
   return PHDE_OK;

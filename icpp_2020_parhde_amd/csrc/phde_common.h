@@ -109,6 +109,9 @@ struct phde_ctx {
   bool bfs_persistent = true;
   int bar_backoff = 1;
   int bfs_stride = 1;  // persistent BFS: every bfs_stride-th block works
+  int bfs_hops = 0;    // persistent BFS: hops per grid barrier (multi-hop traversal; 0: level by level)
+  uint32_t bfs_qcap = 0;  // multi-hop traversal: queue entries usable (0: all n + 1)
+  int bfs_hop_batch = 16;  // multi-hop traversal: queue entries a wave starts a round with
   int bfs_vpw = 1;     // narrow top-down levels: fewest queue entries per wave
   int bfs_epw = 1024;   // narrow top-down levels: edges per wave aimed at
   unsigned int *d_bar = nullptr, *h_bar = nullptr;  // grid-barrier counter + error flag
@@ -179,8 +182,8 @@ inline bool renumbered(const phde_ctx *c) { return c->d_orig != nullptr; }
 
 // bfs.hip
 int bfs_setup(phde_ctx *c);
-// Called once a graph is known to have a deep BFS: the persistent traversal
-// then runs on half the CUs (see bfs.hip).
+// Called once a graph is known to have a deep BFS: its traversals then take
+// the multi-hop kernel (several hops per grid barrier, see bfs.hip).
 void bfs_deep_graph(phde_ctx *c);
 // Runs one traversal from `src`, writing levels into dist_out (device,
 // kUnvisited for unreached).  When min_dist != nullptr the k-centers update

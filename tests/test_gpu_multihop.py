@@ -1,0 +1,64 @@
+"""The multi-hop traversal (a wave expands its part of the frontier for
+several hops between grid barriers; levels corrected by atomicMin when a
+shorter path arrives later) against the oracle's BFS: the levels must be
+bit-exact whatever the timing.  Forced on small graphs with PHDE_BFS_HOPS
+(contexts read it when they are created); PHDE_BFS_QCAP shrinks its queue
+so that the level-by-level repeat after an overflow is exercised too."""
+import numpy as np
+import pytest
+
+from icpp_2020_parhde_amd import layout as L
+from icpp_2020_parhde_amd.graphs import Graph
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+GRAPHS = {
+    "path": lambda: Graph.path(3000),
+    "mesh128": lambda: Graph.mesh(128, 128),
+    "grid3d": lambda: Graph.grid3d(20, 18, 16, wmax=0),
+    "rgg20k": lambda: Graph.rgg(20000, 10.0, seed=2),
+    "holed": lambda: Graph.holed_grid(120, 90, 10, 77),
+    "rmat14": lambda: Graph.rmat(14, 16, seed=3),
+    "star": lambda: Graph.star(20000),
+}
+
+
+@pytest.fixture(scope="module")
+def graphs():
+    return {k: f() for k, f in GRAPHS.items()}
+
+
+@pytest.mark.parametrize("hops", [2, 5, 16])
+@pytest.mark.parametrize("name", list(GRAPHS))
+def test_multihop_bfs_bit_exact(gpu, graphs, name, hops, monkeypatch):
+    g = graphs[name]
+    monkeypatch.setenv("PHDE_BFS_HOPS", str(hops))
+    with L.Context(g) as ctx:
+        for src in (0, g.n // 2, g.n - 1):
+            want = O.seq_bfs(g, src)
+            for _ in range(2):
+                assert np.array_equal(ctx.bfs(src), want), (name, hops, src)
+
+
+@pytest.mark.parametrize("name", ["path", "rgg20k", "rmat14"])
+def test_multihop_queue_overflow_repeats_exactly(gpu, graphs, name, monkeypatch):
+    g = graphs[name]
+    monkeypatch.setenv("PHDE_BFS_HOPS", "4")
+    monkeypatch.setenv("PHDE_BFS_QCAP", "3")
+    with L.Context(g) as ctx:
+        for src in (0, g.n - 1):
+            assert np.array_equal(ctx.bfs(src), O.seq_bfs(g, src)), (name, src)
+
+
+def test_multihop_layout_and_depth_match_level_by_level(gpu, graphs, monkeypatch):
+    """k-centers pivots, the layout and the traversal depths (the stats'
+    level count) do not depend on how the levels were computed."""
+    g = graphs["holed"]
+    with L.Context(g) as ctx:
+        X0, st0 = ctx.layout(s=12, num_coord=2)
+    monkeypatch.setenv("PHDE_BFS_HOPS", "6")
+    with L.Context(g) as ctx:
+        X1, st1 = ctx.layout(s=12, num_coord=2)
+    assert np.array_equal(X0, X1)
+    assert st0.levels_total == st1.levels_total
