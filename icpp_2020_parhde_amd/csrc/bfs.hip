@@ -28,6 +28,7 @@
 #include <cstdio>
 
 #include "device_util.h"
+#include "hops.h"
 #include "levels.h"
 #include "phde_common.h"
 
@@ -36,24 +37,6 @@ namespace phde {
 constexpr int kBfsBlock = 256;
 constexpr int kHugeDegree = 8192;
 constexpr int kBigDegree = 32;
-
-// Frontier queue entry: the vertex and its adjacency range, so the next
-// level reads one 16-byte entry instead of entry + row pointers; lvl is the
-// vertex's level when it was queued (used by the multi-hop traversal).
-struct QEnt {
-  uint32_t v, b, e, lvl;
-};
-
-__device__ __forceinline__ QEnt ld_qent(const QEnt *q) {
-  const unsigned long long *p = reinterpret_cast<const unsigned long long *>(q);
-  const unsigned long long x = ld_dev(p), y = ld_dev(p + 1);
-  return QEnt{(uint32_t)x, (uint32_t)(x >> 32), (uint32_t)y, (uint32_t)(y >> 32)};
-}
-__device__ __forceinline__ void st_qent(QEnt *q, QEnt v) {
-  unsigned long long *p = reinterpret_cast<unsigned long long *>(q);
-  st_dev(p, (unsigned long long)v.v | ((unsigned long long)v.b << 32));
-  st_dev(p + 1, (unsigned long long)v.e | ((unsigned long long)v.lvl << 32));
-}
 
 // Append the lanes' `found` vertices (with row ranges) to the next frontier
 // (queue, huge list, degree sum).  Must be called by all 64 lanes.
@@ -600,130 +583,6 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_persistent_kernel(BfsArgs a, un
 // queue holds, *overflow is set and the host repeats the traversal level by
 // level.
 
-// Wave-collective: place each lane's winners (c[j]: vertex vv[j], rows
-// [vb[j], ve[j]), level nl) in the wave's local queue loc while it has room
-// (only when `local`), the rest in the global queue.  loc_n is the local
-// count (wave-uniform), updated.
-template <int G>
-__device__ __forceinline__ void hop_emit(const BfsArgs &a, const bool *c, const uint32_t *vv, const uint32_t *vb,
-                                         const uint32_t *ve, uint32_t nl, bool local, QEnt *loc, int &loc_n,
-                                         QEnt *q_next, BfsSlot *nxt, uint32_t cap) {
-  int cnt = 0;
-#pragma unroll
-  for (int j = 0; j < G; j++) cnt += c[j] ? 1 : 0;
-  int total = 0;
-  const int off = wave_excl_scan(cnt, &total);
-  if (total == 0) return;
-  const int avail = local ? kWave - loc_n : 0;
-  const int nglob = total > avail ? total - avail : 0;
-  unsigned int base = 0;
-  if (__lane_id() == 0 && nglob)
-    base = (unsigned int)(atomicAdd(&nxt->qs, (unsigned long long)nglob << 32) >> 32);
-  base = __shfl(base, 0);
-  int r = off;
-#pragma unroll
-  for (int j = 0; j < G; j++) {
-    if (c[j]) {
-      const QEnt ent{vv[j], vb[j], ve[j], nl};
-      if (r < avail) {
-        loc[loc_n + r] = ent;
-      } else {
-        const unsigned int gi = base + (unsigned int)(r - avail);
-        if (gi < cap)
-          st_qent(&q_next[gi], ent);
-        else
-          atomicOr(a.overflow, 1u);
-      }
-      r++;
-    }
-  }
-  loc_n = loc_n + total < kWave ? loc_n + total : kWave;
-  if (!local) loc_n = 0;
-}
-
-// One hop from the lanes' entries e (valid lanes only): the rows of kBigDegree
-// or more neighbours by the whole wave, the others G neighbours per lane per
-// round trip, as in small_lists.
-template <int G>
-__device__ __forceinline__ void hop_small(const BfsArgs &a, const QEnt &e, int ds, int maxd, bool local, QEnt *loc,
-                                          int &loc_n, QEnt *q_next, BfsSlot *nxt, uint32_t cap) {
-  const uint32_t *rowptr = a.g.rowptr;
-  const uint32_t *adj = a.g.adj;
-  const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
-  const int32_t nl = (int32_t)e.lvl + 1;
-  for (int g0 = 0; g0 < maxd; g0 += G) {
-    uint32_t vv[G], vb[G], ve[G];
-    bool c[G];
-#pragma unroll
-    for (int j = 0; j < G; j++) {
-      c[j] = g0 + j < ds;
-      vv[j] = adj[c[j] ? e.b + g0 + j : 0u];
-    }
-#pragma unroll
-    for (int j = 0; j < G; j++) vv[j] = c[j] ? vv[j] : 0u;
-    // plain loads: a stale line can only hold a larger level
-    int32_t dd[G];
-#pragma unroll
-    for (int j = 0; j < G; j++) dd[j] = a.dist[c[j] ? vv[j] : 0u];
-#pragma unroll
-    for (int j = 0; j < G; j++) c[j] = c[j] && dd[j] > nl;
-    int32_t won[G];
-    int32_t *dummy = a.dummy + gtid;
-#pragma unroll
-    for (int j = 0; j < G; j++) won[j] = atomicMin(c[j] ? &a.dist[vv[j]] : dummy, nl);
-#pragma unroll
-    for (int j = 0; j < G; j++) {
-      const uint32_t r = c[j] ? vv[j] : 0u;
-      vb[j] = rowptr[r];
-      ve[j] = rowptr[r + 1];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (int j = 0; j < G; j++) c[j] = c[j] && won[j] > nl;
-    hop_emit<G>(a, c, vv, vb, ve, (uint32_t)nl, local, loc, loc_n, q_next, nxt, cap);
-  }
-}
-
-__device__ void hop_expand(const BfsArgs &a, const QEnt &e, bool valid, bool local, QEnt *loc, int &loc_n,
-                           QEnt *q_next, BfsSlot *nxt, uint32_t cap) {
-  const uint32_t *rowptr = a.g.rowptr;
-  const uint32_t *adj = a.g.adj;
-  const int lane = __lane_id();
-  const int d = valid ? (int)(e.e - e.b) : 0;
-  const bool big = d >= kBigDegree;
-  unsigned long long bmask = __ballot(big);
-  while (bmask) {
-    const int l = __ffsll((long long)bmask) - 1;
-    bmask &= bmask - 1;
-    const uint32_t bb = __shfl(e.b, l), be = __shfl(e.e, l);
-    const uint32_t nl = __shfl(e.lvl, l) + 1u;
-    for (uint32_t k0 = bb; k0 < be; k0 += kWave) {
-      const uint32_t k = k0 + lane;
-      bool f = false;
-      uint32_t v = 0, vb = 0, ve = 0;
-      if (k < be) {
-        v = adj[k];
-        if (a.dist[v] > (int32_t)nl && atomicMin(&a.dist[v], (int32_t)nl) > (int32_t)nl) {
-          f = true;
-          vb = rowptr[v];
-          ve = rowptr[v + 1];
-        }
-      }
-      hop_emit<1>(a, &f, &v, &vb, &ve, nl, local, loc, loc_n, q_next, nxt, cap);
-    }
-  }
-  const int ds = big ? 0 : d;
-  const int maxd = wave_max_i32(ds);
-  if (maxd > 0) {
-    if (maxd <= 8)
-      hop_small<8>(a, e, ds, maxd, local, loc, loc_n, q_next, nxt, cap);
-    else if (maxd <= 16 || maxd > 24)
-      hop_small<16>(a, e, ds, maxd, local, loc, loc_n, q_next, nxt, cap);
-    else
-      hop_small<24>(a, e, ds, maxd, local, loc, loc_n, q_next, nxt, cap);
-  }
-}
-
 // The multi-hop traversal in one cooperative launch.  *arrive must be 0 at
 // launch; *err receives 1 if a barrier wait timed out.
 __global__ __launch_bounds__(kBfsBlock) void bfs_hops_kernel(BfsArgs a, unsigned int *arrive, unsigned int *err) {
@@ -731,12 +590,19 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_hops_kernel(BfsArgs a, unsigned
   const unsigned int nb = (unsigned int)work_grid(a);
   const int64_t n = a.g.n;
   const uint32_t cap = (a.qcap > 0 && a.qcap < (uint64_t)(n + 1)) ? a.qcap : (uint32_t)(n + 1);  // queue entries
-  const int lane = __lane_id();
-  const int w = threadIdx.x / kWave;
   const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
-  const int64_t nwaves = work_grid(a) * (blockDim.x / kWave);
-  const int64_t wrank = (int64_t)w * work_grid(a) + work_block(a);
   __shared__ QEnt locq[2][kBfsBlock / kWave][kWave];
+  HopCtx h;
+  h.g = a.g;
+  h.lv = a.dist;
+  h.dummy = a.dummy + gtid;
+  h.overflow = a.overflow;
+  h.bound = kUnvisited - 1;
+  h.cap = cap;
+  h.hops = a.hops;
+  h.batch = a.hop_batch;
+  h.wrank = (int64_t)(threadIdx.x / kWave) * work_grid(a) + work_block(a);
+  h.nwaves = work_grid(a) * (blockDim.x / kWave);
   unsigned int bar = 0;
   bfs_init(a);
   if (grid_barrier(arrive, nb * ++bar, err, a.backoff)) return;
@@ -751,36 +617,12 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_hops_kernel(BfsArgs a, unsigned
       finished = true;
       break;
     }
-    const uint32_t nq = cnt.nq < cap ? cnt.nq : cap;
     if (gtid == 0) {
       st_dev(&old->qs, 0ull);
       st_dev(&old->huge, 0u);
     }
-    const QEnt *q_cur = (R & 1) ? a.q1 : a.q0;
-    QEnt *q_next = (R & 1) ? a.q0 : a.q1;
-    const int hb = a.hop_batch;
-    const int64_t nbatch = ((int64_t)nq + hb - 1) / hb;
-    for (int64_t k = wrank; k < nbatch; k += nwaves) {
-      const int64_t idx = k * hb + lane;
-      bool valid = lane < hb && idx < (int64_t)nq;
-      QEnt e{0u, 0u, 0u, 0u};
-      if (valid) e = ld_qent(&q_cur[idx]);
-      int sel = 0;
-      for (int hop = 0; hop < a.hops; hop++) {
-        const bool local = hop + 1 < a.hops;
-        int loc_n = 0;
-        QEnt *loc = &locq[sel ^ 1][w][0];
-        hop_expand(a, e, valid, local, loc, loc_n, q_next, nxt, cap);
-        if (!local || loc_n == 0) break;
-        // the wave's own LDS writes, read back by other lanes of the wave
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        valid = lane < loc_n;
-        e = valid ? loc[lane] : QEnt{0u, 0u, 0u, 0u};
-        __builtin_amdgcn_wave_barrier();
-        sel ^= 1;
-      }
-    }
+    hop_round(h, (R & 1) ? a.q1 : a.q0, cnt.nq < cap ? cnt.nq : cap, (R & 1) ? a.q0 : a.q1, &nxt->qs,
+              &locq[0][0][0]);
     if (grid_barrier(arrive, nb * ++bar, err, a.backoff)) return;
   }
   if (!finished && gtid == 0) atomicOr(a.overflow, 1u);  // round limit reached

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "device_util.h"
+#include "hops.h"
 #include "learned.h"
 #include "traverse.h"
 
@@ -530,6 +531,13 @@ struct SelArgs {
   int32_t *dummy;  // one scratch word per thread: target of masked-off claims
   const int32_t *d12;  // the two feature traversals' levels (n each)
   int first, K, r, backoff;
+  // multi-hop bounded BFS (hops > 1): 16-byte entries in q0 / q1, counts
+  // in the high halves of cnt64[3]; *overflow set when the queue would
+  // overflow (the caller repeats the selection level by level)
+  int hops, hop_batch;
+  uint32_t qcap;  // queue entries usable (0: n + 1)
+  unsigned long long *cnt64;
+  unsigned int *overflow;
 };
 
 __device__ __forceinline__ uint32_t orderable(float f) {
@@ -640,6 +648,48 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *ar
       i++;
     }
     // ---- bounded BFS from the round's picks
+    if (a.hops > 1) {
+      // several hops per grid barrier (hops.h): the same sd at the end
+      QEnt *hq[2] = {reinterpret_cast<QEnt *>(a.q0), reinterpret_cast<QEnt *>(a.q1)};
+      if (gtid == 0) {
+        for (int j = 0; j < nr; j++) {
+          const uint32_t p = (uint32_t)rp[j];
+          st_dev(&a.sd[p], 0);
+          st_qent(&hq[G & 1][j], QEnt{p, rowptr[p], rowptr[p + 1], 0u});
+        }
+        st_dev(&a.cnt64[G % 3], (unsigned long long)nr << 32);
+        st_dev(&a.cnt64[(G + 1) % 3], 0ull);
+      }
+      grid_barrier(arrive, nb * ++bar, err, a.backoff);
+      if (ld_dev(err)) return;
+      HopCtx h;
+      h.g = a.g;
+      h.lv = a.sd;
+      h.dummy = a.dummy + gtid;
+      h.overflow = a.overflow;
+      h.bound = a.r;
+      h.cap = (a.qcap > 0 && a.qcap < (uint64_t)(n + 1)) ? a.qcap : (uint32_t)(n + 1);
+      h.hops = a.hops;
+      h.batch = a.hop_batch;
+      h.wrank = (int64_t)(threadIdx.x / kWave) * gridDim.x + blockIdx.x;
+      h.nwaves = nwaves;
+      __shared__ QEnt locq[2][256 / kWave][kWave];
+      __shared__ unsigned int sh_nq;
+      for (;;) {
+        if (threadIdx.x == 0) sh_nq = (unsigned int)(ld_dev(&a.cnt64[G % 3]) >> 32);
+        __syncthreads();
+        const unsigned int nq = sh_nq;
+        __syncthreads();
+        if (nq == 0) break;
+        if (gtid == 0) st_dev(&a.cnt64[(G + 2) % 3], 0ull);
+        hop_round(h, hq[G & 1], nq < h.cap ? nq : h.cap, hq[(G + 1) & 1], &a.cnt64[(G + 1) % 3], &locq[0][0][0]);
+        grid_barrier(arrive, nb * ++bar, err, a.backoff);
+        G++;
+        if (ld_dev(err)) return;
+      }
+      G++;  // the next round's picks go to a fresh queue
+      continue;
+    }
     uint32_t *q0 = (G & 1) ? a.q1 : a.q0;
     if (gtid == 0) {
       for (int j = 0; j < nr; j++) {
@@ -890,8 +940,19 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
   a.K = K;
   a.r = std::max(1, std::min(r, 2046));
   a.backoff = c->bar_backoff;
-  (void)n;
-  PHDE_TRY_RC(fill_u32(c, c->d_bar, 2, 0u));
+  a.hops = c->bfs_hops;
+  a.hop_batch = c->bfs_hop_batch;
+  a.qcap = c->bfs_qcap;
+  a.cnt64 = reinterpret_cast<unsigned long long *>(c->d_slots) + 4;
+  a.overflow = c->d_bar + 2;
+  // the multi-hop search can (rarely) outgrow its queue: keep the state it
+  // starts from, to repeat the selection level by level then
+  DBuf<int32_t> sd0;
+  if (a.hops > 1) {
+    PHDE_TRY_RC(sd0.alloc(n));
+    PHDE_HIP(hipMemcpyAsync(sd0.p, c->lp_sd, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
+  }
+  PHDE_TRY_RC(fill_u32(c, c->d_bar, 3, 0u));
   unsigned int *arrive = c->d_bar, *err = c->d_bar + 1;
   void *args[] = {&a, &arrive, &err};
   hipEvent_t ev;
@@ -905,13 +966,21 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
   }
   c->timer.end(c->stream, PHDE_KERNEL_OTHER, ev);
   std::vector<int64_t> h(K);
-  unsigned int herr = 0;
+  unsigned int herr[2] = {0u, 0u};
   PHDE_HIP(hipMemcpyAsync(h.data(), pk.p, sizeof(int64_t) * K, hipMemcpyDeviceToHost, c->stream));
-  PHDE_HIP(hipMemcpyAsync(&herr, err, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipMemcpyAsync(herr, err, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
-  if (herr) {
+  if (herr[0]) {
     set_error("pivot selection grid barrier timed out");
     return PHDE_ERR_DEVICE;
+  }
+  if (a.hops > 1 && herr[1]) {
+    PHDE_HIP(hipMemcpyAsync(c->lp_sd, sd0.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
+    const int saved = c->bfs_hops;
+    c->bfs_hops = 0;
+    const int rc = greedy_select(c, score_dev, first, K, r, picks);
+    c->bfs_hops = saved;
+    return rc;
   }
   for (int i = first; i < K; i++) picks[i] = h[i];
   return PHDE_OK;

@@ -62,3 +62,36 @@ def test_multihop_layout_and_depth_match_level_by_level(gpu, graphs, monkeypatch
         X1, st1 = ctx.layout(s=12, num_coord=2)
     assert np.array_equal(X0, X1)
     assert st0.levels_total == st1.levels_total
+
+
+SEL_GRAPHS = {
+    "mesh": lambda: Graph.mesh(60, 70),
+    "rgg": lambda: Graph.rgg(8000, 10.0, seed=11),
+    "holed": lambda: Graph.holed_grid(80, 90, 6, 12),
+    "rmat": lambda: Graph.rmat(12, 16, seed=13),
+}
+
+
+@pytest.mark.parametrize("hops", [3, 12])
+@pytest.mark.parametrize("name", list(SEL_GRAPHS))
+def test_multihop_greedy_selection_bit_exact(gpu, name, hops, monkeypatch):
+    """The greedy pivot selection's bounded BFS run several hops per grid
+    barrier picks exactly the oracle's pivots (and so does the level-by-level
+    repeat after a forced queue overflow)."""
+    from oracle import learned as OL
+    g = SEL_GRAPHS[name]()
+    rng = np.random.default_rng(9)
+    score = rng.random(g.n).astype(np.float32)
+    d1 = O.seq_bfs(g, 0)
+    second = int(np.argmax(d1))
+    d2 = O.seq_bfs(g, second)
+    monkeypatch.setenv("PHDE_BFS_HOPS", str(hops))
+    for k, r in [(64, 3), (20, 1), (40, 9)]:
+        want = OL.greedy_select(g, score, d1, d2, 0, second, k, r)
+        with L.Context(g) as ctx:
+            got = ctx.select_pivots(score, k, r, 0)
+        assert got.tolist() == want.tolist(), (name, hops, k, r)
+    monkeypatch.setenv("PHDE_BFS_QCAP", "3")
+    with L.Context(g) as ctx:
+        got = ctx.select_pivots(score, 40, 9, 0)
+    assert got.tolist() == OL.greedy_select(g, score, d1, d2, 0, second, 40, 9).tolist()
