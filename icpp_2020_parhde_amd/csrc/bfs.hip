@@ -595,6 +595,8 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_hops_kernel(BfsArgs a, unsigned
   HopCtx h;
   h.g = a.g;
   h.lv = a.dist;
+  h.col = 0;
+  h.lvl_mask = 0xffffffffu;
   h.dummy = a.dummy + gtid;
   h.overflow = a.overflow;
   h.bound = kUnvisited - 1;

@@ -13,7 +13,9 @@
 // the length of a real path).  Only the amount of work depends on timing.
 // Claims stop at level `bound`.  Entries carry their level.  An entry that
 // would not fit in the global queue sets *overflow; the caller then repeats
-// the search level by level.
+// the search level by level.  Several searches can share the queues: an
+// entry's level word then holds the search's index above bit kHopSrcShift,
+// and search j's levels are the column lv + j * col.
 #pragma once
 
 #include "device_util.h"
@@ -40,10 +42,13 @@ __device__ __forceinline__ void st_qent(QEnt *q, QEnt v) {
 }
 
 constexpr int kHopBigDegree = 32;  // rows at least this long are expanded by the whole wave
+constexpr int kHopSrcShift = 26;   // several searches: index above this bit of the level word
 
 struct HopCtx {
   DevGraph g;
-  int32_t *lv;              // levels, lowered by atomicMin
+  int32_t *lv;              // levels, lowered by atomicMin (search j: lv + j * col)
+  int64_t col;              // column stride of several searches (0: one search)
+  uint32_t lvl_mask;        // the level bits of an entry's level word
   int32_t *dummy;           // this thread's scratch word (masked-off claims)
   unsigned int *overflow;   // set when an entry does not fit in the global queue
   int32_t bound;            // largest level claimed
@@ -100,7 +105,9 @@ __device__ __forceinline__ void hop_small(const HopCtx &h, const QEnt &e, int ds
                                           int &loc_n, QEnt *q_next, unsigned long long *qs_next) {
   const uint32_t *rowptr = h.g.rowptr;
   const uint32_t *adj = h.g.adj;
-  const int32_t nl = (int32_t)e.lvl + 1;
+  const uint32_t tag = e.lvl & ~h.lvl_mask;  // the search's index bits
+  const int32_t nl = (int32_t)(e.lvl & h.lvl_mask) + 1;
+  int32_t *lvc = h.lv + (int64_t)(tag >> kHopSrcShift) * h.col;
   for (int g0 = 0; g0 < maxd; g0 += G) {
     uint32_t vv[G], vb[G], ve[G], lv[G];
     bool c[G];
@@ -115,14 +122,14 @@ __device__ __forceinline__ void hop_small(const HopCtx &h, const QEnt &e, int ds
     // larger one, and the claim below decides
     int32_t dd[G];
 #pragma unroll
-    for (int j = 0; j < G; j++) dd[j] = h.lv[c[j] ? vv[j] : 0u];
+    for (int j = 0; j < G; j++) dd[j] = lvc[c[j] ? vv[j] : 0u];
 #pragma unroll
     for (int j = 0; j < G; j++) c[j] = c[j] && dd[j] > nl;
     // claims, all issued before any result is used (masked-off lanes aim
     // at the scratch word instead of branching)
     int32_t won[G];
 #pragma unroll
-    for (int j = 0; j < G; j++) won[j] = atomicMin(c[j] ? &h.lv[vv[j]] : h.dummy, nl);
+    for (int j = 0; j < G; j++) won[j] = atomicMin(c[j] ? &lvc[vv[j]] : h.dummy, nl);
 #pragma unroll
     for (int j = 0; j < G; j++) {
       const uint32_t r = c[j] ? vv[j] : 0u;
@@ -134,7 +141,7 @@ __device__ __forceinline__ void hop_small(const HopCtx &h, const QEnt &e, int ds
 #pragma unroll
     for (int j = 0; j < G; j++) {
       c[j] = c[j] && won[j] > nl;
-      lv[j] = (uint32_t)nl;
+      lv[j] = (uint32_t)nl | tag;
     }
     hop_emit<G>(h, c, vv, vb, ve, lv, local, loc, loc_n, q_next, qs_next);
   }
@@ -155,14 +162,17 @@ __device__ __forceinline__ void hop_expand(const HopCtx &h, const QEnt &e, bool 
     const int l = __ffsll((long long)bmask) - 1;
     bmask &= bmask - 1;
     const uint32_t bb = __shfl(e.b, l), be = __shfl(e.e, l);
-    const int32_t nl = (int32_t)__shfl(e.lvl, l) + 1;
+    const uint32_t word = __shfl(e.lvl, l);
+    const uint32_t tag = word & ~h.lvl_mask;
+    const int32_t nl = (int32_t)(word & h.lvl_mask) + 1;
+    int32_t *lvc = h.lv + (int64_t)(tag >> kHopSrcShift) * h.col;
     for (uint32_t k0 = bb; k0 < be; k0 += kWave) {
       const uint32_t k = k0 + lane;
       bool f = false;
-      uint32_t v = 0, vb = 0, ve = 0, lv = (uint32_t)nl;
+      uint32_t v = 0, vb = 0, ve = 0, lv = (uint32_t)nl | tag;
       if (k < be) {
         v = adj[k];
-        if (h.lv[v] > nl && atomicMin(&h.lv[v], nl) > nl) {
+        if (lvc[v] > nl && atomicMin(&lvc[v], nl) > nl) {
           f = true;
           vb = rowptr[v];
           ve = rowptr[v + 1];
@@ -200,7 +210,7 @@ __device__ __forceinline__ void hop_round(const HopCtx &h, const QEnt *q_cur, ui
     if (valid) e = ld_qent(&q_cur[idx]);
     int sel = 0;
     for (int hop = 0; hop < h.hops; hop++) {
-      valid = valid && (int32_t)e.lvl < h.bound;
+      valid = valid && (int32_t)(e.lvl & h.lvl_mask) < h.bound;
       const bool local = hop + 1 < h.hops;
       int loc_n = 0;
       QEnt *loc = locq + ((sel ^ 1) * wpb + w) * kWave;

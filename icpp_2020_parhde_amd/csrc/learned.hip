@@ -665,6 +665,8 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *ar
       HopCtx h;
       h.g = a.g;
       h.lv = a.sd;
+      h.col = 0;
+      h.lvl_mask = 0xffffffffu;
       h.dummy = a.dummy + gtid;
       h.overflow = a.overflow;
       h.bound = a.r;

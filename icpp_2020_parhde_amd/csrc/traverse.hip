@@ -25,6 +25,7 @@
 #include <cstdio>
 
 #include "device_util.h"
+#include "hops.h"
 #include "phde_common.h"
 #include "levels.h"
 #include "traverse.h"
@@ -547,6 +548,79 @@ __global__ __launch_bounds__(kTBlock) void msbfs_level_kernel(MsArgs a, int L) {
   }
 }
 
+// ---- multi-source BFS of a graph with a deep BFS: the k searches run as
+// concurrent multi-hop searches (hops.h) in one cooperative launch, search
+// j's levels in column j of D; the same levels as the bit-parallel sweep.
+struct MsHopArgs {
+  DevGraph g;
+  int k;
+  const int64_t *src;
+  int32_t *D;
+  QEnt *q0, *q1;
+  unsigned long long *cnt;  // [3] queue counts (high halves), rotating by round
+  int32_t *dummy;
+  unsigned int *overflow;
+  uint32_t cap;             // queue entries usable
+  int hops, batch, backoff;
+};
+
+__global__ void mshop_fill_kernel(int32_t *D, int64_t total, int32_t v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+    D[i] = v;
+}
+
+__global__ void mshop_source_kernel(MsHopArgs a) {
+  const int j = threadIdx.x;
+  if (j < a.k) {
+    const uint32_t s = (uint32_t)a.src[j];
+    a.D[(int64_t)j * a.g.n + s] = 0;
+    a.q0[j] = QEnt{s, a.g.rowptr[s], a.g.rowptr[s + 1], (uint32_t)j << kHopSrcShift};
+  }
+  if (j == 0) {
+    a.cnt[0] = (unsigned long long)a.k << 32;
+    a.cnt[1] = 0;
+    a.cnt[2] = 0;
+  }
+}
+
+__global__ void mshop_finish_kernel(int32_t *D, int64_t total) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+    if (D[i] == kUnvisited) D[i] = -1;
+}
+
+// *arrive must be 0 at launch; *err receives 1 if a barrier wait timed out.
+__global__ __launch_bounds__(kTBlock) void msbfs_hops_kernel(MsHopArgs a, unsigned int *arrive, unsigned int *err) {
+  const unsigned int nb = gridDim.x;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  HopCtx h;
+  h.g = a.g;
+  h.lv = a.D;
+  h.col = a.g.n;
+  h.lvl_mask = (1u << kHopSrcShift) - 1u;
+  h.dummy = a.dummy + gtid;
+  h.overflow = a.overflow;
+  h.bound = (int32_t)(1u << kHopSrcShift) - 2;
+  h.cap = a.cap;
+  h.hops = a.hops;
+  h.batch = a.batch;
+  h.wrank = (int64_t)(threadIdx.x / kWave) * gridDim.x + blockIdx.x;
+  h.nwaves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  __shared__ QEnt locq[2][kTBlock / kWave][kWave];
+  __shared__ unsigned int sh_nq;
+  unsigned int bar = 0;
+  for (int64_t R = 0;; R++) {
+    if (threadIdx.x == 0) sh_nq = (unsigned int)(ld_dev(&a.cnt[R % 3]) >> 32);
+    __syncthreads();
+    const unsigned int nq = sh_nq;
+    __syncthreads();
+    if (nq == 0) break;
+    if (gtid == 0) st_dev(&a.cnt[(R + 2) % 3], 0ull);
+    hop_round(h, (R & 1) ? a.q1 : a.q0, nq < a.cap ? nq : a.cap, (R & 1) ? a.q0 : a.q1, &a.cnt[(R + 1) % 3],
+              &locq[0][0][0]);
+    if (grid_barrier(arrive, nb * ++bar, err, a.backoff)) return;
+  }
+}
+
 // ------------------------------------------------------------------ host side
 
 // Grid of the multi-source sweep: two 256-thread blocks per CU measured
@@ -650,6 +724,61 @@ int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
     if (!c->ts_cnt) PHDE_HIP(hipMalloc(&c->ts_cnt, 256));
   }
   PHDE_HIP(hipMemcpyAsync(c->ms_src, sources, sizeof(int64_t) * k, hipMemcpyHostToDevice, c->stream));
+  // hops per round and first batch of the concurrent searches (PHDE_MS_HOPS=0:
+  // the bit-parallel sweep for every graph)
+  static const int ms_hops = [] {
+    const char *e = getenv("PHDE_MS_HOPS");
+    return e ? std::max(0, std::min(64, atoi(e))) : 4;
+  }();
+  static const int ms_batch = [] {
+    const char *e = getenv("PHDE_MS_HOP_BATCH");
+    return e ? std::max(1, std::min(kWave, atoi(e))) : 32;
+  }();
+  if (ms_hops > 1 && c->bfs_hops > 1 && c->bfs_persistent && n < ((int64_t)1 << kHopSrcShift) && k <= 64) {
+    // a graph with a deep BFS: concurrent multi-hop searches, one column each
+    MsHopArgs h;
+    h.g = c->g;
+    h.k = k;
+    h.src = c->ms_src;
+    h.D = D;
+    h.q0 = reinterpret_cast<QEnt *>(c->d_q[0]);
+    h.q1 = reinterpret_cast<QEnt *>(c->d_q[1]);
+    h.cnt = reinterpret_cast<unsigned long long *>(c->ts_cnt + 48);
+    h.dummy = reinterpret_cast<int32_t *>(c->ms_dummy);
+    h.overflow = c->d_bar + 2;
+    h.cap = (c->bfs_qcap > 0 && c->bfs_qcap < (uint64_t)(n + 1)) ? c->bfs_qcap : (uint32_t)(n + 1);
+    h.hops = ms_hops;
+    h.batch = ms_batch;
+    h.backoff = c->bar_backoff;
+    hipEvent_t ev;
+    c->timer.begin(c->stream, PHDE_KERNEL_MSBFS, &ev);
+    const int fgrid = std::max(1, c->num_cus) * 4;
+    mshop_fill_kernel<<<fgrid, kTBlock, 0, c->stream>>>(D, n * k, kUnvisited);
+    mshop_source_kernel<<<1, 64, 0, c->stream>>>(h);
+    PHDE_HIP(hipGetLastError());
+    PHDE_TRY_RC(fill_u32(c, c->d_bar, 3, 0u));
+    unsigned int *arrive = c->d_bar, *err = c->d_bar + 1;
+    void *args[] = {&h, &arrive, &err};
+    const hipError_t e = hipLaunchCooperativeKernel((const void *)msbfs_hops_kernel, dim3(c->bfs_grid),
+                                                    dim3(kTBlock), args, 0, c->stream);
+    if (e == hipSuccess) {
+      mshop_finish_kernel<<<fgrid, kTBlock, 0, c->stream>>>(D, n * k);
+      PHDE_HIP(hipGetLastError());
+      c->timer.end(c->stream, PHDE_KERNEL_MSBFS, ev);
+      PHDE_HIP(hipMemcpyAsync(c->h_bar, c->d_bar + 1, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost,
+                              c->stream));
+      PHDE_HIP(hipStreamSynchronize(c->stream));
+      if (c->h_bar[0]) {
+        set_error("multi-source BFS grid barrier timed out");
+        return PHDE_ERR_DEVICE;
+      }
+      if (!c->h_bar[1]) return PHDE_OK;
+      // the queue overflowed: the bit-parallel sweep below recomputes D
+    } else {
+      (void)hipGetLastError();
+      c->timer.end(c->stream, PHDE_KERNEL_MSBFS, ev);
+    }
+  }
   MsArgs a;
   a.g = c->g;
   a.k = k;

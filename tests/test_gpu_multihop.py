@@ -95,3 +95,38 @@ def test_multihop_greedy_selection_bit_exact(gpu, name, hops, monkeypatch):
     with L.Context(g) as ctx:
         got = ctx.select_pivots(score, 40, 9, 0)
     assert got.tolist() == OL.greedy_select(g, score, d1, d2, 0, second, 40, 9).tolist()
+
+
+@pytest.mark.parametrize("hops", [4, 12])
+@pytest.mark.parametrize("name", ["mesh", "rgg", "rmat", "star"])
+@pytest.mark.parametrize("k", [1, 7, 64])
+def test_multihop_msbfs_equals_k_bfs(gpu, name, k, hops, monkeypatch):
+    """k concurrent multi-hop searches, one column each, equal k BFS."""
+    g = {"mesh": lambda: Graph.mesh(70, 90), "rgg": lambda: Graph.rgg(30000, 10.0, seed=8),
+         "rmat": lambda: Graph.rmat(13, 16, seed=4), "star": lambda: Graph.star(10000)}[name]()
+    rng = np.random.default_rng(k)
+    src = rng.choice(g.n, size=k, replace=False)
+    monkeypatch.setenv("PHDE_BFS_HOPS", str(hops))
+    with L.Context(g) as ctx:
+        D = ctx.msbfs(src)
+    for j, s in enumerate(src):
+        assert np.array_equal(D[:, j], O.seq_bfs(g, int(s))), (name, k, j)
+
+
+def test_multihop_msbfs_duplicates_disconnected_and_overflow(gpu, monkeypatch):
+    g = Graph.from_edges(6, [0, 1, 3], [1, 2, 4])
+    monkeypatch.setenv("PHDE_BFS_HOPS", "3")
+    with L.Context(g) as ctx:
+        D = ctx.msbfs([0, 0, 4, 5])
+    assert D[:, 0].tolist() == [0, 1, 2, -1, -1, -1]
+    assert D[:, 1].tolist() == D[:, 0].tolist()
+    assert D[:, 2].tolist() == [-1, -1, -1, 1, 0, -1]
+    assert D[:, 3].tolist() == [-1, -1, -1, -1, -1, 0]
+    # a queue too small for the searches: the bit-parallel sweep repeats them
+    g = Graph.rgg(30000, 10.0, seed=8)
+    src = np.random.default_rng(3).choice(g.n, size=40, replace=False)
+    monkeypatch.setenv("PHDE_BFS_QCAP", "50")
+    with L.Context(g) as ctx:
+        D = ctx.msbfs(src)
+    for j, s in enumerate(src):
+        assert np.array_equal(D[:, j], O.seq_bfs(g, int(s))), j
