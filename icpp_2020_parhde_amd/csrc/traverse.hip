@@ -76,6 +76,8 @@ struct SsspArgs {
   KcResult *kc;
   int *done_dev, *host_flags;
   int32_t *dummy;  // one scratch word per thread: target of masked-off atomics
+  int hops;        // > 1: a wave relaxes what its batch improves within the bucket for up to this many hops
+  int hop_batch;   // near-queue entries a wave starts with (hops > 1)
 };
 
 constexpr int kSsspGroup = 8;  // edges a lane relaxes per round trip
@@ -159,6 +161,170 @@ __device__ void sssp_epilogue(const SsspArgs &a) {
   }
 }
 
+// Near-queue relaxation of one round with several hops per launch (graphs
+// with a deep BFS): a wave takes a batch of the near queue and relaxes the
+// vertices its relaxations improve below the bucket bound itself, up to
+// a.hops hops, through a 64-entry LDS queue that carries each vertex's
+// distance and row range; what does not fit and the last hop's
+// improvements join the next round's near queue (near stamp), improvements
+// beyond the bound the far pile (far stamp), as in the one-hop round.
+// Relaxation stays atomicMin on the distance, so the distances are exact;
+// a vertex may be relaxed more often than in the one-hop rounds.
+__device__ void sssp_relax_hops(const SsspArgs &a, int R, const SsspState &s, unsigned int nq, const uint32_t *ncur,
+                                uint32_t *nnext, unsigned int *nnext_cnt, uint32_t *fcur, unsigned int *fcnt,
+                                QEnt *locq) {
+  constexpr int G = kSsspGroup;
+  const int lane = __lane_id();
+  const int w = threadIdx.x / kWave;
+  const int wpb = blockDim.x / kWave;
+  const int64_t wrank = (int64_t)w * gridDim.x + blockIdx.x;
+  const int64_t nwaves = (int64_t)gridDim.x * wpb;
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t *dum = a.dummy + gtid;
+  const uint32_t *rowptr = a.g.rowptr, *adj = a.g.adj;
+  const int32_t *wt = a.g.w;
+  const int hb = a.hop_batch;
+  const int64_t nbatch = ((int64_t)nq + hb - 1) / hb;
+  for (int64_t k = wrank; k < nbatch; k += nwaves) {
+    const int64_t idx = k * hb + lane;
+    bool valid = lane < hb && idx < (int64_t)nq;
+    uint32_t u = valid ? ncur[idx] : 0u;
+    int32_t du = valid ? a.dist[u] : 0;
+    uint32_t b = valid ? rowptr[u] : 0u, e = valid ? rowptr[u + 1] : 0u;
+    int sel = 0;
+    for (int hop = 0; hop < a.hops; hop++) {
+      const bool local = hop + 1 < a.hops;
+      QEnt *loc = locq + ((sel ^ 1) * wpb + w) * kWave;
+      int loc_n = 0;
+      const int d = valid ? (int)(e - b) : 0;
+      const bool big = d >= 32;
+      unsigned long long bmask = __ballot(big);
+      while (bmask) {  // long rows by the whole wave, improvements to the global piles
+        const int l = __ffsll((long long)bmask) - 1;
+        bmask &= bmask - 1;
+        const uint32_t bb = __shfl(b, l), be = __shfl(e, l);
+        const int32_t bd = __shfl(du, l);
+        for (uint32_t k0 = bb; k0 < be; k0 += kWave) {
+          const uint32_t kk = k0 + lane;
+          bool pn = false, pf = false;
+          uint32_t v = 0;
+          if (kk < be) {
+            v = adj[kk];
+            relax(a, v, bd + wt[kk], s.bound, R, s.epoch, &pn, &pf);
+          }
+          wave_push(pn, v, nnext, nnext_cnt);
+          wave_push(pf, v, fcur, fcnt);
+        }
+      }
+      const int ds = big ? 0 : d;
+      int maxd = ds;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) maxd = max(maxd, __shfl_xor(maxd, o));
+      for (int t0 = 0; t0 < maxd; t0 += G) {
+        uint32_t vv[G], vb[G], ve[G];
+        int32_t nd[G], cur[G], wv[G];
+        bool ok[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          ok[j] = t0 + j < ds;
+          const uint32_t kk = ok[j] ? b + t0 + j : 0u;
+          vv[j] = adj[kk];
+          wv[j] = wt[kk];
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          vv[j] = ok[j] ? vv[j] : 0u;
+          nd[j] = ok[j] ? du + wv[j] : INT32_MAX;
+        }
+#pragma unroll
+        for (int j = 0; j < G; j++) cur[j] = a.dist[vv[j]];
+#pragma unroll
+        for (int j = 0; j < G; j++) ok[j] = ok[j] && nd[j] < cur[j];
+#pragma unroll
+        for (int j = 0; j < G; j++) cur[j] = atomicMin(ok[j] ? &a.dist[vv[j]] : dum, nd[j]);
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          const uint32_t r = ok[j] ? vv[j] : 0u;
+          vb[j] = rowptr[r];
+          ve[j] = rowptr[r + 1];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        bool nr[G], fr[G];
+        int fm = INT32_MAX, cn = 0;
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          const bool imp = ok[j] && nd[j] < cur[j];
+          nr[j] = imp && nd[j] < s.bound;
+          fr[j] = imp && !(nd[j] < s.bound);
+          if (fr[j]) fm = min(fm, nd[j]);
+          cn += nr[j] ? 1 : 0;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) fm = min(fm, __shfl_xor(fm, o));
+        if (lane == 0 && fm != INT32_MAX) atomicMin(&a.fmin[s.epoch & 1], fm);
+        // near improvements: the wave's local queue while it has room
+        int totn = 0;
+        const int offn = wave_excl_scan(cn, &totn);
+        const int avail = local ? kWave - loc_n : 0;
+        bool gn[G];
+        int r = offn;
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          gn[j] = false;
+          if (nr[j]) {
+            if (r < avail)
+              loc[loc_n + r] = QEnt{vv[j], vb[j], ve[j], (uint32_t)nd[j]};
+            else
+              gn[j] = true;
+            r++;
+          }
+        }
+        loc_n = local ? (loc_n + totn < kWave ? loc_n + totn : kWave) : 0;
+        // the others: near / far stamps in one batch of exchanges, then the
+        // global appends
+#pragma unroll
+        for (int j = 0; j < G; j++)
+          cur[j] = atomicExch(gn[j] ? &a.nstamp[vv[j]] : fr[j] ? &a.fstamp[vv[j]] : dum, gn[j] ? R + 1 : s.epoch);
+        int cg = 0, cf = 0;
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          gn[j] = gn[j] && cur[j] != R + 1;
+          fr[j] = fr[j] && cur[j] != s.epoch;
+          cg += gn[j] ? 1 : 0;
+          cf += fr[j] ? 1 : 0;
+        }
+        int totg = 0, totf = 0;
+        const int offg = wave_excl_scan(cg, &totg);
+        const int offf = wave_excl_scan(cf, &totf);
+        unsigned int bg = 0, bf = 0;
+        if (lane == 0) {
+          if (totg) bg = atomicAdd(nnext_cnt, (unsigned int)totg);
+          if (totf) bf = atomicAdd(fcnt, (unsigned int)totf);
+        }
+        bg = __shfl(bg, 0) + (unsigned int)offg;
+        bf = __shfl(bf, 0) + (unsigned int)offf;
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+          if (gn[j]) nnext[bg++] = vv[j];
+          if (fr[j]) fcur[bf++] = vv[j];
+        }
+      }
+      if (!local || loc_n == 0) break;
+      // the wave's own LDS writes, read back by other lanes of the wave
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      valid = lane < loc_n;
+      const QEnt en = valid ? loc[lane] : QEnt{0u, 0u, 0u, 0u};
+      __builtin_amdgcn_wave_barrier();
+      u = en.v;
+      b = en.b;
+      e = en.e;
+      du = (int32_t)en.lvl;
+      sel ^= 1;
+    }
+  }
+}
+
 __global__ __launch_bounds__(kTBlock) void sssp_round_kernel(SsspArgs a, int R) {
   // the round's control words are independent: load them in one batch
   // before the early exit instead of one dependent load after another
@@ -209,6 +375,12 @@ __global__ __launch_bounds__(kTBlock) void sssp_round_kernel(SsspArgs a, int R) 
       a.fmin[(s.epoch + 1) & 1] = INT32_MAX;
     }
     unsigned int *fcnt = &a.fcnt[s.epoch & 1];
+    __shared__ QEnt locq[2][kTBlock / kWave][kWave];
+    if (a.hops > 1) {
+      sssp_relax_hops(a, R, s, nq, ncur, nnext, nnext_cnt, fcur, fcnt, &locq[0][0][0]);
+      if (gtid == 0) a.st[(R + 1) & 1] = nx;
+      return;
+    }
     // Small rows: kSsspGroup edges per lane per round trip, every load and
     // atomic of the group issued before any result is used (masked-off
     // atomics go to the lane's scratch word); near / far appends staged in
@@ -690,6 +862,15 @@ int sssp_run(phde_ctx *c, int64_t src, int32_t delta, int32_t *dist_out, int32_t
   a.done_dev = c->d_done;
   a.host_flags = c->d_flags;
   a.dummy = (int32_t *)c->ms_dummy;
+  // several hops per round on graphs with a deep BFS
+  const char *eh = getenv("PHDE_SSSP_HOPS"), *eb = getenv("PHDE_SSSP_HOP_BATCH");
+  // measured on the weighted 160^3 grid: 9.6 ms per SSSP in one-hop rounds,
+  // 9.1 at 4 hops from 64-entry batches (9.3 at 3, 9.8 at 8; 10.4 from
+  // batches of 32; its wide wavefront keeps every wave busy)
+  const int sssp_hops = eh ? std::max(0, std::min(64, atoi(eh))) : 4;
+  const int sssp_batch = eb ? std::max(1, std::min(kWave, atoi(eb))) : 64;
+  a.hops = c->bfs_hops > 1 ? sssp_hops : 0;
+  a.hop_batch = sssp_batch;
   c->h_flags[kFlagDone] = -1;
   c->h_flags[kFlagProgress] = -1;
   hipEvent_t ev;

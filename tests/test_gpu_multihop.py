@@ -130,3 +130,37 @@ def test_multihop_msbfs_duplicates_disconnected_and_overflow(gpu, monkeypatch):
         D = ctx.msbfs(src)
     for j, s in enumerate(src):
         assert np.array_equal(D[:, j], O.seq_bfs(g, int(s))), j
+
+
+@pytest.fixture(scope="module")
+def wgraphs():
+    from test_gpu_traversal import weighted_rmat
+    return {"grid": Graph.grid3d(24, 20, 18, seed=4, wmax=255), "rmat": weighted_rmat(12, 21),
+            "path": Graph.from_edges(2000, list(range(1999)), list(range(1, 2000)), [1 + (i * 7) % 13 for i in range(1999)])}
+
+
+@pytest.mark.parametrize("hops", [2, 8, 20])
+@pytest.mark.parametrize("name", ["grid", "rmat", "path"])
+@pytest.mark.parametrize("delta", [1, 37, 4096, 0])
+def test_multihop_sssp_bit_exact(gpu, wgraphs, name, delta, hops, monkeypatch):
+    """Delta-stepping rounds that relax several hops per launch within the
+    bucket give Dijkstra's distances exactly."""
+    g = wgraphs[name]
+    monkeypatch.setenv("PHDE_BFS_HOPS", "12")
+    monkeypatch.setenv("PHDE_SSSP_HOPS", str(hops))
+    with L.Context(g) as ctx:
+        for src in (0, g.n // 2, g.n - 1):
+            want = O.dijkstra(g, src)
+            got = ctx.sssp(src, delta).astype(np.int64)
+            assert np.array_equal(got, want), (name, delta, hops, src)
+
+
+def test_multihop_wparhde_layout_matches_one_hop(gpu, wgraphs, monkeypatch):
+    g = wgraphs["grid"]
+    with L.Context(g) as ctx:
+        X0, st0 = ctx.layout(s=10, num_coord=3, algorithm=L.ALGO_WPARHDE)
+    monkeypatch.setenv("PHDE_BFS_HOPS", "12")
+    with L.Context(g) as ctx:
+        X1, st1 = ctx.layout(s=10, num_coord=3, algorithm=L.ALGO_WPARHDE)
+    assert st0.pivots == st1.pivots
+    assert np.array_equal(X0, X1)
