@@ -341,6 +341,53 @@ static int orth_candidate(Orth &orth, const phde_options *o, bool weighted, bool
   return PHDE_OK;
 }
 
+// Candidates cols[0..count) in order through orth_candidate's steps until
+// s columns are accepted; *consumed gets how many were taken up.  FP64
+// ParHDE takes them kCgsBlock at a time (Orth::cgs_block: the same columns,
+// bit for bit, with one pass over the earlier columns per block);
+// PHDE_CGS_BLOCK=0 takes them one at a time.
+static bool cgs_blocked(bool weighted, bool fp32) {
+  static const bool env = [] {
+    const char *e = getenv("PHDE_CGS_BLOCK");
+    return !(e && atoi(e) == 0);
+  }();
+  return !weighted && !fp32 && env;
+}
+
+static int orth_candidates(Orth &orth, const phde_options *o, bool weighted, bool fp32,
+                           const int32_t *const *cols, const unsigned long long *sumsq, int count, int &accepted,
+                           LayoutOut &out, int *consumed = nullptr) {
+  int i = 0;
+  if (cgs_blocked(weighted, fp32)) {
+    while (i < count && accepted < o->s) {
+      const int nc = std::min(kCgsBlock, count - i);
+      double scales[kCgsBlock];
+      for (int b = 0; b < nc; b++) scales[b] = 1 / std::sqrt((double)sumsq[i + b]);
+      std::vector<bool> taken;
+      const int before = accepted;
+      PHDE_TRY(orth.cgs_block(cols + i, scales, nc, o->s, accepted, taken));
+      // candidates looked at: up to the one that completed s (or all)
+      int looked = 0, acc = before;
+      for (int b = 0; b < nc; b++) {
+        if (acc >= o->s) break;
+        looked++;
+        if (taken[b]) {
+          acc++;
+        } else {
+          if (o->verbose) std::printf("discarding vec %d, normdist < 0.001\n", acc + 1);
+          out.ndisc++;
+        }
+      }
+      i += looked;
+    }
+  } else {
+    for (; i < count && accepted < o->s; i++)
+      PHDE_TRY(orth_candidate(orth, o, weighted, fp32, cols[i], sumsq[i], accepted, out));
+  }
+  if (consumed) *consumed = i;
+  return PHDE_OK;
+}
+
 static int not_enough_pivots(const phde_options *o, int accepted, int limit) {
   if (o->verbose)
     std::fprintf(stderr, "Number of Pivots obtained: %d inadequate, increase the Pivots Limits: %d\n",
@@ -396,7 +443,10 @@ static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, Layout
   PHDE_TRY(min_dist.alloc(n));
   PHDE_TRY(fill_i32(c, min_dist.p, n, INT_MAX));
   const bool eager = o->eager && !weighted;
-  PHDE_TRY(cols.alloc(eager ? (size_t)n * limit : (size_t)n));
+  // lazy FP64 ParHDE traverses its first s candidates before taking them
+  // through blocked CGS (the k-centers chain does not depend on them)
+  const bool lazy_blocked = !eager && cgs_blocked(weighted, fp32);
+  PHDE_TRY(cols.alloc(eager ? (size_t)n * limit : lazy_blocked ? (size_t)n * std::min(s, limit) : (size_t)n));
   std::vector<unsigned long long> sumsq(limit + 1, 0);
   KCenters kc;
   kc.next = o->first_pivot;
@@ -426,12 +476,24 @@ static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, Layout
     for (int rc = 1; rc <= limit; rc++) PHDE_TRY(traverse(rc, cols.p + (size_t)(rc - 1) * n));
     out.t_dist += ms_since(t0);
     auto t1 = clk::now();
-    for (int rc = 1; rc <= limit && accepted < s; rc++)
-      PHDE_TRY(orth_candidate(orth, o, weighted, fp32, cols.p + (size_t)(rc - 1) * n, sumsq[rc],
-                              accepted, out));
+    std::vector<const int32_t *> cp(limit);
+    for (int rc = 1; rc <= limit; rc++) cp[rc - 1] = cols.p + (size_t)(rc - 1) * n;
+    PHDE_TRY(orth_candidates(orth, o, weighted, fp32, cp.data(), sumsq.data() + 1, limit, accepted, out));
     out.t_orth += ms_since(t1);
   } else {
-    for (int rc = 1; rc <= limit && accepted < s; rc++) {
+    int rc = 1;
+    if (lazy_blocked) {
+      const int first = std::min(s, limit);
+      auto t0 = clk::now();
+      for (; rc <= first; rc++) PHDE_TRY(traverse(rc, cols.p + (size_t)(rc - 1) * n));
+      out.t_dist += ms_since(t0);
+      auto t1 = clk::now();
+      std::vector<const int32_t *> cp(first);
+      for (int i = 0; i < first; i++) cp[i] = cols.p + (size_t)i * n;
+      PHDE_TRY(orth_candidates(orth, o, weighted, fp32, cp.data(), sumsq.data() + 1, first, accepted, out));
+      out.t_orth += ms_since(t1);
+    }
+    for (; rc <= limit && accepted < s; rc++) {
       auto t0 = clk::now();
       PHDE_TRY(traverse(rc, cols.p));
       auto t1 = clk::now();
@@ -522,8 +584,9 @@ static int layout_hde_random(phde_ctx *c, const phde_options *o, double *coords,
     PHDE_TRY(columns_of(c, o, src.data() + b, kb, D.p, sumsq.data(), out));
     auto t1 = clk::now();
     out.t_dist += std::chrono::duration<double, std::milli>(t1 - t0).count();
-    for (int i = 0; i < kb && accepted < s; i++)
-      PHDE_TRY(orth_candidate(orth, o, weighted, fp32, D.p + (size_t)i * n, sumsq[i], accepted, out));
+    std::vector<const int32_t *> cp(kb);
+    for (int i = 0; i < kb; i++) cp[i] = D.p + (size_t)i * n;
+    PHDE_TRY(orth_candidates(orth, o, weighted, fp32, cp.data(), sumsq.data(), kb, accepted, out));
     out.t_orth += ms_since(t1);
   }
   out.nacc = accepted;
@@ -589,9 +652,12 @@ static int layout_learned(phde_ctx *c, const phde_options *o, double *coords, La
       have = target;
     }
     auto to = clk::now();
-    for (; next < have && accepted < s; next++) {
-      const int32_t *col = next < 2 ? c->lp_d12 + (size_t)next * n : D.p + (size_t)(next - 2) * n;
-      PHDE_TRY(orth_candidate(orth, o, false, fp32, col, sumsq[next], accepted, out));
+    if (next < have && accepted < s) {
+      std::vector<const int32_t *> cp(have - next);
+      for (int i = next; i < have; i++) cp[i - next] = i < 2 ? c->lp_d12 + (size_t)i * n : D.p + (size_t)(i - 2) * n;
+      int used = 0;
+      PHDE_TRY(orth_candidates(orth, o, false, fp32, cp.data(), sumsq.data() + next, have - next, accepted, out, &used));
+      next += used;
     }
     out.t_orth += ms_since(to);
     if (accepted >= s || have >= kmax) break;

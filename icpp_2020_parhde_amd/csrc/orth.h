@@ -1,5 +1,7 @@
 // orth.h — internal interface of the D-orthogonalisation stage (orth.hip).
 #pragma once
+#include <vector>
+
 #include "phde_common.h"
 
 namespace phde {
@@ -7,6 +9,8 @@ namespace phde {
 // Slot of the ones vector in the coefficient / denominator arrays
 // (accepted columns use slots 0..s-1, s <= 64).
 constexpr int kOnesSlot = 64;
+// Candidates per block of the blocked classical Gram-Schmidt.
+constexpr int kCgsBlock = 8;
 
 // Device state of one D-orthogonalisation: S (n x s, column-major, FP64 or
 // FP32 storage), the candidate a (FP64), cached denominators
@@ -20,6 +24,7 @@ struct Orth {
   void *S = nullptr;
   double *a = nullptr, *part = nullptr, *den = nullptr, *coef = nullptr, *scal = nullptr;
   double *h_scal = nullptr, *ones = nullptr;
+  double *blkA = nullptr, *blkAcc = nullptr, *blkPart = nullptr, *blkCoef = nullptr;  // blocked CGS
   const double *deg_ = nullptr;
   double u0 = 0;
 
@@ -32,6 +37,11 @@ struct Orth {
   int mgs_pass(int ncols);  // modified GS, same operands
   int read_norm(double *norm);  // synchronises
   int accept(int j);            // S[:, j] = a / ||a||, den_j
+  // Classical GS of nc <= kCgsBlock candidates (cols[b] * scales[b]) in
+  // order, each as cgs_pass + norm + discard rule + accept would treat it,
+  // until s_target columns are accepted; taken[b]: candidate b accepted.
+  int cgs_block(const int32_t *const *cols, const double *scales, int nc, int s_target, int &accepted,
+                std::vector<bool> &taken);
 };
 
 }  // namespace phde

@@ -15,6 +15,7 @@
 // (per-block partials, then one block), so results are deterministic.
 // The candidate a is an FP64 vector of n entries.
 #include <algorithm>
+#include <vector>
 
 #include "phde_common.h"
 #include "orth.h"
@@ -23,6 +24,7 @@ namespace phde {
 
 constexpr int kOrthBlock = 256;
 constexpr int kColChunk = 16;
+constexpr int kCgsCols = 4;  // columns per block of the blocked dot products
 
 __device__ __forceinline__ double block_sum(double v, double *sh) {
 #pragma unroll
@@ -78,10 +80,10 @@ __global__ __launch_bounds__(kOrthBlock) void cgs_dots_kernel(const double *a, c
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const double ad = a[i] * deg[i];
-    acc0 += ad * u0;
+    acc0 = fma(ad, u0, acc0);
 #pragma unroll
     for (int c = 0; c < kColChunk; c++)
-      if (c < kc) acc[c] += ad * (double)S[(int64_t)(k0 + c) * n + i];
+      if (c < kc) acc[c] = fma(ad, (double)S[(int64_t)(k0 + c) * n + i], acc[c]);
   }
   for (int c = 0; c < kColChunk; c++) {
     if (c >= kc) break;
@@ -110,6 +112,16 @@ __global__ void cgs_coef_kernel(const double *part, int nblk, int ncols, const d
   }
 }
 
+// As cgs_coef_kernel, for partials laid out by slot (ones at kOnesSlot).
+__global__ void cgs_coef_kernel_slots(const double *part, int nblk, int ncols, const double *den, double *coef) {
+  __shared__ double sh[kOrthBlock / kWave];
+  const int slot = blockIdx.x == ncols ? kOnesSlot : blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(int64_t)slot * nblk + b];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) coef[slot] = t * (1 / den[slot]);
+}
+
 // a_i -= sum_k coef_k S_ik + coef_ones * u0, and per-block sums of a_i^2
 // (ParHDE.cpp:245-265).
 template <typename T>
@@ -125,11 +137,11 @@ __global__ __launch_bounds__(kOrthBlock) void cgs_update_kernel(double *a, const
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     double acc = 0;
-    for (int k = 0; k < ncols; k++) acc += sc[k] * (double)S[(int64_t)k * n + i];
-    acc += sc[kOnesSlot] * u0;
+    for (int k = 0; k < ncols; k++) acc = fma(sc[k], (double)S[(int64_t)k * n + i], acc);
+    acc = fma(sc[kOnesSlot], u0, acc);
     const double v = a[i] - acc;
     a[i] = v;
-    ss += v * v;
+    ss = fma(v, v, ss);
   }
   const double t = block_sum(ss, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -224,6 +236,157 @@ __global__ void mgs_axpy_kernel(double *a, const TY *y, const double *c, int64_t
     a[i] -= cc * (double)y[i];
 }
 
+
+// ---------------------------------------------------------------- blocked CGS
+// A block of up to kCgsBlock candidates shares the passes over the columns
+// accepted before the block: one pass for their dot products, one for the
+// columns' part of their updates.  Every per-element sum runs in the same
+// order, over the same rows per thread and through the same reduction tree
+// as in cgs_dots_kernel / cgs_update_kernel, so each candidate ends exactly
+// as if it had been taken alone (the remaining work per candidate covers
+// the columns accepted inside the block).
+
+// Dot products of the candidates A_b (b < nb) with columns [0, m) and the
+// ones vector.  part layout: [b][slot][rowblock] with slot = column or
+// kOnesSlot.  Columns are taken kCgsCols per blockIdx.y.
+template <typename T>
+__global__ __launch_bounds__(kOrthBlock) void cgs_dots_multi_kernel(const double *A, const double *deg,
+                                                                   const T *S, int64_t n, int m, int nb,
+                                                                   double u0, double *part) {
+  __shared__ double sh[kCgsBlock * (kCgsCols + 1)][kOrthBlock / kWave];
+  const int k0 = blockIdx.y * kCgsCols;
+  const int kc = max(0, min(kCgsCols, m - k0));
+  const bool ones = blockIdx.y == 0;
+  double acc[kCgsBlock][kCgsCols + 1];
+#pragma unroll
+  for (int b = 0; b < kCgsBlock; b++)
+#pragma unroll
+    for (int c = 0; c <= kCgsCols; c++) acc[b][c] = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double dg = deg[i];
+    double sv[kCgsCols];
+#pragma unroll
+    for (int c = 0; c < kCgsCols; c++) sv[c] = c < kc ? (double)S[(int64_t)(k0 + c) * n + i] : 0.0;
+#pragma unroll
+    for (int b = 0; b < kCgsBlock; b++) {
+      if (b < nb) {
+        const double ad = A[(int64_t)b * n + i] * dg;
+        acc[b][kCgsCols] = fma(ad, u0, acc[b][kCgsCols]);
+#pragma unroll
+        for (int c = 0; c < kCgsCols; c++) acc[b][c] = fma(ad, sv[c], acc[b][c]);
+      }
+    }
+  }
+  // every value's wave sum as in block_sum, then the waves' sums in order
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int b = 0; b < kCgsBlock; b++)
+#pragma unroll
+    for (int c = 0; c <= kCgsCols; c++) {
+      double v = acc[b][c];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0) sh[b * (kCgsCols + 1) + c][w] = v;
+    }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kCgsBlock * (kCgsCols + 1); t += blockDim.x) {
+    const int b = t / (kCgsCols + 1), c = t % (kCgsCols + 1);
+    if (b >= nb) continue;
+    if (c < kCgsCols && c >= kc) continue;
+    if (c == kCgsCols && !ones) continue;
+    double tot = 0;
+    for (int q = 0; q < (int)(blockDim.x / kWave); q++) tot += sh[t][q];
+    const int slot = c == kCgsCols ? kOnesSlot : k0 + c;
+    part[((int64_t)b * (kOnesSlot + 1) + slot) * gridDim.x + blockIdx.x] = tot;
+  }
+}
+
+// Dot products of one candidate a with columns [k_lo, k_hi) (the block's
+// own accepted columns); part: [slot][rowblock].
+template <typename T>
+__global__ __launch_bounds__(kOrthBlock) void cgs_dots_range_kernel(const double *a, const double *deg,
+                                                                   const T *S, int64_t n, int k_lo, int k_hi,
+                                                                   double *part) {
+  __shared__ double sh[kOrthBlock / kWave];
+  double acc[kCgsBlock];
+#pragma unroll
+  for (int c = 0; c < kCgsBlock; c++) acc[c] = 0;
+  const int kc = k_hi - k_lo;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double ad = a[i] * deg[i];
+#pragma unroll
+    for (int c = 0; c < kCgsBlock; c++)
+      if (c < kc) acc[c] = fma(ad, (double)S[(int64_t)(k_lo + c) * n + i], acc[c]);
+  }
+  for (int c = 0; c < kc; c++) {
+    const double t = block_sum(acc[c], sh);
+    if (threadIdx.x == 0) part[(int64_t)(k_lo + c) * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// coef[b][k] = num / den for k < m and the ones slot, every candidate.
+__global__ void cgs_coef_multi_kernel(const double *part, int nblk, int m, const double *den, double *coef) {
+  __shared__ double sh[kOrthBlock / kWave];
+  const int b = blockIdx.y;
+  const int k = blockIdx.x == m ? kOnesSlot : blockIdx.x;
+  const double *p = part + ((int64_t)b * (kOnesSlot + 1) + k) * nblk;
+  double s = 0;
+  for (int q = threadIdx.x; q < nblk; q += blockDim.x) s += p[q];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) coef[(int64_t)b * (kOnesSlot + 1) + k] = t * (1 / den[k]);
+}
+
+// acc[b][i] = sum_{k < m} coef[b][k] S_ik, in column order (the first part
+// of cgs_update_kernel's sum).
+template <typename T>
+__global__ __launch_bounds__(kOrthBlock) void cgs_update_old_kernel(const T *S, int64_t n, int m, int nb,
+                                                                   const double *coef, double *acc_out) {
+  __shared__ double sc[kCgsBlock][kOnesSlot + 1];
+  for (int t = threadIdx.x; t < kCgsBlock * (kOnesSlot + 1); t += blockDim.x)
+    sc[t / (kOnesSlot + 1)][t % (kOnesSlot + 1)] = t / (kOnesSlot + 1) < nb ? coef[t] : 0.0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double acc[kCgsBlock];
+#pragma unroll
+    for (int b = 0; b < kCgsBlock; b++) acc[b] = 0;
+    for (int k = 0; k < m; k++) {
+      const double sv = (double)S[(int64_t)k * n + i];
+#pragma unroll
+      for (int b = 0; b < kCgsBlock; b++) acc[b] = fma(sc[b][k], sv, acc[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < kCgsBlock; b++)
+      if (b < nb) acc_out[(int64_t)b * n + i] = acc[b];
+  }
+}
+
+// The rest of one candidate's update: acc continues over columns [m, ncols)
+// and the ones term, a = A_b - acc, per-block sums of a^2.
+template <typename T>
+__global__ __launch_bounds__(kOrthBlock) void cgs_update_tail_kernel(double *a, const double *Ab, const double *acc_old,
+                                                                    const T *S, int64_t n, int m, int ncols,
+                                                                    double u0, const double *coef, double *part) {
+  __shared__ double sc[kOnesSlot + 1];
+  __shared__ double sh[kOrthBlock / kWave];
+  for (int k = threadIdx.x; k <= kOnesSlot; k += blockDim.x) sc[k] = coef[k];
+  __syncthreads();
+  double ss = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double acc = acc_old[i];
+    for (int k = m; k < ncols; k++) acc = fma(sc[k], (double)S[(int64_t)k * n + i], acc);
+    acc = fma(sc[kOnesSlot], u0, acc);
+    const double v = Ab[i] - acc;
+    a[i] = v;
+    ss = fma(v, v, ss);
+  }
+  const double t = block_sum(ss, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // ---------------------------------------------------------------- host side
 
 static int grid_for(int64_t n, int cap) {
@@ -272,8 +435,13 @@ void Orth::release() {
   if (scal) (void)hipFree(scal);
   if (ones) (void)hipFree(ones);
   if (h_scal) (void)hipHostFree(h_scal);
+  if (blkA) (void)hipFree(blkA);
+  if (blkAcc) (void)hipFree(blkAcc);
+  if (blkPart) (void)hipFree(blkPart);
+  if (blkCoef) (void)hipFree(blkCoef);
   S = nullptr;
   a = part = den = coef = scal = h_scal = ones = nullptr;
+  blkA = blkAcc = blkPart = blkCoef = nullptr;
 }
 
 // den of the ones vector: sum_i u0*u0*deg_i (ParHDE.cpp:226).
@@ -360,5 +528,61 @@ static int accept_t(Orth &o, int j) {
 }
 
 int Orth::accept(int j) { return fp32 ? accept_t<float>(*this, j) : accept_t<double>(*this, j); }
+
+template <typename T>
+static int cgs_block_t(Orth &o, const int32_t *const *cols, const double *scales, int nc, int s_target,
+                       int &accepted, std::vector<bool> &taken) {
+  hipStream_t st = o.ctx->stream;
+  const T *S = (const T *)o.S;
+  const int64_t n = o.n;
+  const int m = accepted;
+  for (int b = 0; b < nc; b++)
+    cand_from_dist_kernel<<<grid_for(n, 4096), kOrthBlock, 0, st>>>(cols[b], scales[b], o.blkA + (int64_t)b * n, n);
+  const int nchunk = std::max(1, (m + kCgsCols - 1) / kCgsCols);
+  cgs_dots_multi_kernel<T><<<dim3(o.nblk, nchunk), kOrthBlock, 0, st>>>(o.blkA, o.deg_, S, n, m, nc, o.u0,
+                                                                        o.blkPart);
+  cgs_coef_multi_kernel<<<dim3(m + 1, nc), kOrthBlock, 0, st>>>(o.blkPart, o.nblk, m, o.den, o.blkCoef);
+  cgs_update_old_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(S, n, m, nc, o.blkCoef, o.blkAcc);
+  PHDE_HIP(hipGetLastError());
+  taken.assign(nc, false);
+  for (int b = 0; b < nc && accepted < s_target; b++) {
+    const int ncols = accepted;
+    double *pb = o.blkPart + (int64_t)b * (kOnesSlot + 1) * o.nblk;
+    const double *Ab = o.blkA + (int64_t)b * n;
+    if (ncols > m)
+      cgs_dots_range_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(Ab, o.deg_, S, n, m, ncols, pb);
+    // coefficients of every column so far (the ones in slot kOnesSlot)
+    cgs_coef_kernel_slots<<<ncols + 1, kOrthBlock, 0, st>>>(pb, o.nblk, ncols, o.den, o.coef);
+    cgs_update_tail_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(o.a, Ab, o.blkAcc + (int64_t)b * n, S, n, m, ncols,
+                                                             o.u0, o.coef, o.part);
+    norm_finalize_kernel<<<1, kOrthBlock, 0, st>>>(o.part, o.nblk, o.scal);
+    PHDE_HIP(hipGetLastError());
+    double nd = 0;
+    PHDE_TRY_RC(o.read_norm(&nd));
+    if (nd >= 0.001) {
+      PHDE_TRY_RC(o.accept(accepted));
+      accepted++;
+      taken[b] = true;
+    }
+  }
+  return PHDE_OK;
+}
+
+int Orth::cgs_block(const int32_t *const *cols, const double *scales, int nc, int s_target, int &accepted,
+                    std::vector<bool> &taken) {
+  if (nc < 1 || nc > kCgsBlock) {
+    set_error("internal: CGS block size");
+    return PHDE_ERR_ARG;
+  }
+  if (!blkA) {
+    PHDE_HIP(hipMalloc(&blkA, sizeof(double) * (size_t)n * kCgsBlock));
+    PHDE_HIP(hipMalloc(&blkAcc, sizeof(double) * (size_t)n * kCgsBlock));
+    PHDE_HIP(hipMalloc(&blkPart, sizeof(double) * (size_t)nblk * (kOnesSlot + 1) * kCgsBlock));
+    PHDE_HIP(hipMalloc(&blkCoef, sizeof(double) * (size_t)(kOnesSlot + 1) * kCgsBlock));
+  }
+  return fp32 ? cgs_block_t<float>(*this, cols, scales, nc, s_target, accepted, taken)
+              : cgs_block_t<double>(*this, cols, scales, nc, s_target, accepted, taken);
+}
+
 
 }  // namespace phde
