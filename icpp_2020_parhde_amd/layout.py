@@ -163,7 +163,9 @@ class Context:
 
     # --------------------------------------------------------------- layout
     def layout(self, options: Optional[PhdeOptions] = None, **kw):
-        """One layout; returns (coords n x c float64, LayoutStats)."""
+        """One layout; returns (coords n x c float64, LayoutStats).  The
+        coordinates are the library's column-major buffer viewed as n x c
+        (Fortran order, no copy)."""
         o = options if options is not None else default_options(**kw)
         if options is not None:
             for k, v in kw.items():
@@ -172,7 +174,7 @@ class Context:
         st = PhdeStats()
         _check(self.lib.phde_layout(self.h, ctypes.byref(o), as_ptr(X, ctypes.c_double), ctypes.byref(st)),
                "phde_layout")
-        return X.reshape(o.num_coord, self.n).T.copy(), LayoutStats.from_c(st)
+        return X.reshape(o.num_coord, self.n).T, LayoutStats.from_c(st)
 
     def power_eig(self, K: np.ndarray):
         """Reference power method on one s x s matrix (column-major semantics)."""
