@@ -140,6 +140,13 @@ __global__ __launch_bounds__(kBlk) void hub_finish_kernel(DevGraph g, const T *S
   }
 }
 
+// Blocks of the matrix-core Gram: two per CU, fewer for a small n (4 rows
+// per wave and step).
+static int gram_mfma_blocks(const phde_ctx *c, int64_t n) {
+  const int64_t need = ((n + 3) / 4 + (kBlk / kWave) - 1) / (kBlk / kWave);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(need, 2 * (int64_t)std::max(c->num_cus, 1)));
+}
+
 // Per-block partial C[k + i*s] = sum_rows A[r][i] * B[r][k] over row tiles
 // staged in LDS (A, B row-major n x s).
 constexpr int kGramRows = 16;
@@ -190,6 +197,79 @@ __global__ void gram_reduce_kernel(const double *part, int nblk, int ss, double 
   // part holds element (k = e % s, i = e / s) of sum_j A[j][i] B[j][k]; the
   // reference stores it at C[k + i*s] which is exactly index e.
   C[e] = s;
+}
+
+// The same partial Gram sums on the FP64 matrix cores: a wave takes 4 rows
+// at a time and issues one v_mfma_f64_16x16x4f64 per 16 x 16 tile of the
+// (s <= 64, padded to 64) x 64 result, the 16 tiles' sums in registers; each
+// wave writes its own partial (part[wave][i*s + k]), which gram_reduce_kernel
+// sums in a fixed order.  Lane l supplies A[row r0 + l/16][16*mi + l%16]
+// and B[row r0 + l/16][16*ni + l%16]; its output v is row l/16 + 4*v,
+// column l%16 of each tile (measured lane layout of the instruction).
+typedef double gram_v4d __attribute__((ext_vector_type(4)));
+template <typename T>
+__global__ __launch_bounds__(kBlk) void gram_mfma_kernel(const T *A, const T *B, int64_t n, int s, double *part,
+                                                        const double *rw = nullptr) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t wave = ((int64_t)blockIdx.x * kBlk + threadIdx.x) / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * (kBlk / kWave);
+  const int col = lane % 16, kr = lane / 16;
+  gram_v4d acc[4][4];
+#pragma unroll
+  for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+    for (int ni = 0; ni < 4; ni++) acc[mi][ni] = gram_v4d{0.0, 0.0, 0.0, 0.0};
+  const int64_t nsteps = (n + 3) / 4;
+  for (int64_t st = wave; st < nsteps; st += nwaves) {
+    const int64_t r = st * 4 + kr;
+    const bool rok = r < n;
+    const double wr = (rw != nullptr && rok) ? rw[r] : 1.0;
+    double av[4], bv[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int c = q * 16 + col;
+      const bool ok = rok && c < s;
+      av[q] = ok ? (double)A[r * s + c] * wr : 0.0;
+      bv[q] = ok ? (double)B[r * s + c] : 0.0;
+    }
+#pragma unroll
+    for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+      for (int ni = 0; ni < 4; ni++) acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+  }
+  double *out = part + wave * (int64_t)s * s;
+#pragma unroll
+  for (int mi = 0; mi < 4; mi++)
+#pragma unroll
+    for (int ni = 0; ni < 4; ni++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int i = mi * 16 + kr + 4 * v, k = ni * 16 + col;
+        if (i < s && k < s) out[i * s + k] = acc[mi][ni][v];
+      }
+}
+
+// Partial Gram sums of A^T B (row-major n x s; A's rows scaled by rw when
+// given) into part, then C = their sum in a fixed order.  The matrix-core
+// kernel unless PHDE_GRAM_MFMA=0 or s > 64.
+template <typename T>
+static int gram_launch(phde_ctx *c, const T *A, const T *B, int64_t n, int s, double *part, int nblk_gram,
+                       double *C, const double *rw = nullptr) {
+  static const bool mfma = [] {
+    const char *e = getenv("PHDE_GRAM_MFMA");
+    return !(e && atoi(e) == 0);
+  }();
+  int nparts = nblk_gram;
+  if (mfma && s <= 64) {
+    const int blocks = gram_mfma_blocks(c, n);
+    gram_mfma_kernel<T><<<blocks, kBlk, 0, c->stream>>>(A, B, n, s, part, rw);
+    nparts = blocks * (kBlk / kWave);
+  } else {
+    gram_partial_kernel<T><<<nblk_gram, kBlk, 0, c->stream>>>(A, B, n, s, part, rw);
+  }
+  gram_reduce_kernel<<<(s * s + 255) / 256, 256, 0, c->stream>>>(part, nparts, s * s, C);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
 }
 
 // X[i + k*n] = sum_j S[i + j*n] V[j + k*s] (S column-major), and per-block
@@ -369,9 +449,7 @@ static int laplacian_t(phde_ctx *c, const void *Scol, void *Srow, void *LSrow, i
   }
   c->timer.end(c->stream, PHDE_KERNEL_SPMM, ev);
   c->timer.begin(c->stream, PHDE_KERNEL_GRAM, &ev);
-  gram_partial_kernel<T><<<nblk_gram, kBlk, 0, c->stream>>>((const T *)Srow, (const T *)LSrow, n, s,
-                                                           part);
-  gram_reduce_kernel<<<(s * s + 255) / 256, 256, 0, c->stream>>>(part, nblk_gram, s * s, SLS);
+  PHDE_TRY_RC(gram_launch<T>(c, (const T *)Srow, (const T *)LSrow, n, s, part, nblk_gram, SLS));
   c->timer.end(c->stream, PHDE_KERNEL_GRAM, ev);
   PHDE_HIP(hipGetLastError());
   return PHDE_OK;
@@ -385,7 +463,8 @@ int Lap::setup(phde_ctx *c, int64_t n, int s_, bool fp32_) {
   nblk_gram = blocks_for(c, (n + kGramRows - 1) / kGramRows, 2);
   PHDE_HIP(hipMalloc(&Srow, esz * (size_t)n * s));
   PHDE_HIP(hipMalloc(&LSrow, esz * (size_t)n * s));
-  PHDE_HIP(hipMalloc(&part, sizeof(double) * (size_t)std::max(nblk_gram, 1) * s * s));
+  const int nparts = std::max(nblk_gram, gram_mfma_blocks(c, n) * (kBlk / kWave));
+  PHDE_HIP(hipMalloc(&part, sizeof(double) * (size_t)std::max(nparts, 1) * s * s));
   PHDE_HIP(hipMalloc(&SLS, sizeof(double) * s * s));
   PHDE_TRY_RC(spmm_hubs(c));
   if (c->sp_nchunks > 0) PHDE_HIP(hipMalloc(&hub_part, sizeof(double) * (size_t)c->sp_nchunks * s));
@@ -416,15 +495,12 @@ int Lap::gram_self(const void *Acol) {
   if (fp32) {
     col_to_row_kernel<float><<<blocks_for(ctx, (n + 63) / 64), kBlk, 0, ctx->stream>>>(
         (const float *)Acol, (float *)Srow, n, s);
-    gram_partial_kernel<float><<<nblk_gram, kBlk, 0, ctx->stream>>>(
-        (const float *)Srow, (const float *)Srow, n, s, part);
+    PHDE_TRY_RC(gram_launch<float>(ctx, (const float *)Srow, (const float *)Srow, n, s, part, nblk_gram, SLS));
   } else {
     col_to_row_kernel<double><<<blocks_for(ctx, (n + 63) / 64), kBlk, 0, ctx->stream>>>(
         (const double *)Acol, (double *)Srow, n, s);
-    gram_partial_kernel<double><<<nblk_gram, kBlk, 0, ctx->stream>>>(
-        (const double *)Srow, (const double *)Srow, n, s, part);
+    PHDE_TRY_RC(gram_launch<double>(ctx, (const double *)Srow, (const double *)Srow, n, s, part, nblk_gram, SLS));
   }
-  gram_reduce_kernel<<<(s * s + 255) / 256, 256, 0, ctx->stream>>>(part, nblk_gram, s * s, SLS);
   ctx->timer.end(ctx->stream, PHDE_KERNEL_GRAM, ev);
   PHDE_HIP(hipGetLastError());
   return PHDE_OK;
@@ -461,16 +537,14 @@ int weighted_gram(phde_ctx *c, const void *S_dev, bool fp32, int s, double *out_
   if (fp32) {
     col_to_row_kernel<float><<<blocks_for(c, (n + 63) / 64), kBlk, 0, c->stream>>>(
         (const float *)S_dev, (float *)tmp.Srow, n, s);
-    gram_partial_kernel<float><<<tmp.nblk_gram, kBlk, 0, c->stream>>>(
-        (const float *)tmp.Srow, (const float *)tmp.Srow, n, s, tmp.part, c->g.degw);
+    PHDE_TRY_RC(gram_launch<float>(c, (const float *)tmp.Srow, (const float *)tmp.Srow, n, s, tmp.part,
+                                   tmp.nblk_gram, out_dev, c->g.degw));
   } else {
     col_to_row_kernel<double><<<blocks_for(c, (n + 63) / 64), kBlk, 0, c->stream>>>(
         (const double *)S_dev, (double *)tmp.Srow, n, s);
-    gram_partial_kernel<double><<<tmp.nblk_gram, kBlk, 0, c->stream>>>(
-        (const double *)tmp.Srow, (const double *)tmp.Srow, n, s, tmp.part, c->g.degw);
+    PHDE_TRY_RC(gram_launch<double>(c, (const double *)tmp.Srow, (const double *)tmp.Srow, n, s, tmp.part,
+                                    tmp.nblk_gram, out_dev, c->g.degw));
   }
-  gram_reduce_kernel<<<(s * s + 255) / 256, 256, 0, c->stream>>>(tmp.part, tmp.nblk_gram, s * s,
-                                                                  out_dev);
   c->timer.end(c->stream, PHDE_KERNEL_GRAM, ev);
   PHDE_HIP(hipGetLastError());
   PHDE_HIP(hipStreamSynchronize(c->stream));
