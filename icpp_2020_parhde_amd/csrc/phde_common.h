@@ -115,6 +115,8 @@ struct phde_ctx {
   int bfs_vpw = 1;     // narrow top-down levels: fewest queue entries per wave
   int bfs_epw = 1024;   // narrow top-down levels: edges per wave aimed at
   unsigned int *d_bar = nullptr, *h_bar = nullptr;  // grid-barrier counter + error flag
+  void *h_stage = nullptr;  // pinned staging of large results copied to the caller
+  size_t h_stage_bytes = 0;
   int32_t *d_dummy = nullptr;                       // per-thread scratch of the BFS claims
   // mapped host flags polled by the level driver (levels.h) and their
   // device-side aliases; d_done is a device-memory copy for the kernels
