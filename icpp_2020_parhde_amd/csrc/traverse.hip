@@ -940,7 +940,14 @@ int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
     PHDE_TRY_RC(fill_u32(c, c->d_bar, 3, 0u));
     unsigned int *arrive = c->d_bar, *err = c->d_bar + 1;
     void *args[] = {&h, &arrive, &err};
-    const hipError_t e = hipLaunchCooperativeKernel((const void *)msbfs_hops_kernel, dim3(c->bfs_grid),
+    // blocks per CU: 52 searches on RGG 4.2M 78.1 ms at one, 79.0 at two;
+    // on RGG 16.8M 302.5 ms at one, 291.9 at two
+    const char *epc = getenv("PHDE_MS_HOP_BLOCKS_PER_CU");
+    const int per_cu = epc ? std::max(1, std::min(2, atoi(epc))) : (n >= ((int64_t)1 << 23) ? 2 : 1);
+    int resident = 0;
+    PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, msbfs_hops_kernel, kTBlock, 0));
+    const int hgrid = std::max(1, c->num_cus) * std::max(1, std::min(per_cu, resident));
+    const hipError_t e = hipLaunchCooperativeKernel((const void *)msbfs_hops_kernel, dim3(hgrid),
                                                     dim3(kTBlock), args, 0, c->stream);
     if (e == hipSuccess) {
       mshop_finish_kernel<<<fgrid, kTBlock, 0, c->stream>>>(D, n * k);
