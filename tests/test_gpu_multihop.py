@@ -164,3 +164,17 @@ def test_multihop_wparhde_layout_matches_one_hop(gpu, wgraphs, monkeypatch):
         X1, st1 = ctx.layout(s=10, num_coord=3, algorithm=L.ALGO_WPARHDE)
     assert st0.pivots == st1.pivots
     assert np.array_equal(X0, X1)
+
+
+@pytest.mark.parametrize("name", ["rgg", "rmat"])
+def test_multihop_msbfs_two_blocks_per_cu(gpu, name, monkeypatch):
+    """The concurrent searches on two blocks per CU (the setting of large
+    graphs) give the same levels."""
+    g = {"rgg": lambda: Graph.rgg(30000, 10.0, seed=8), "rmat": lambda: Graph.rmat(13, 16, seed=4)}[name]()
+    src = np.random.default_rng(5).choice(g.n, size=52, replace=False)
+    monkeypatch.setenv("PHDE_BFS_HOPS", "12")
+    monkeypatch.setenv("PHDE_MS_HOP_BLOCKS_PER_CU", "2")
+    with L.Context(g) as ctx:
+        D = ctx.msbfs(src)
+    for j, s in enumerate(src):
+        assert np.array_equal(D[:, j], O.seq_bfs(g, int(s))), (name, j)
