@@ -252,16 +252,21 @@ struct LayoutOut {
 // result goes to a pinned buffer of the context in one copy, and host
 // threads copy it on to the caller.
 static int result_to_host(phde_ctx *c, void *host, const void *dev, size_t bytes) {
-  if (bytes < ((size_t)8 << 20)) {
+  auto direct = [&]() -> int {
     PHDE_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
     PHDE_HIP(hipStreamSynchronize(c->stream));
     return PHDE_OK;
-  }
+  };
+  if (bytes < ((size_t)8 << 20)) return direct();
   if (c->h_stage_bytes < bytes) {
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     c->h_stage = nullptr;
     c->h_stage_bytes = 0;
-    PHDE_HIP(hipHostMalloc(&c->h_stage, bytes, hipHostMallocDefault));
+    if (hipHostMalloc(&c->h_stage, bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();  // no pinned memory to spare: the runtime's own staging
+      c->h_stage = nullptr;
+      return direct();
+    }
     c->h_stage_bytes = bytes;
   }
   PHDE_HIP(hipMemcpyAsync(c->h_stage, dev, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -276,12 +281,19 @@ static int result_to_host(phde_ctx *c, void *host, const void *dev, size_t bytes
   }
   const size_t part = ((bytes + nt - 1) / nt + 4095) & ~(size_t)4095;
   std::vector<std::thread> th;
-  for (size_t t = 0; t < nt; t++) {
-    const size_t b = t * part;
-    if (b >= bytes) break;
-    const size_t len = std::min(part, bytes - b);
-    th.emplace_back([=] { std::memcpy(dst + b, src + b, len); });
+  size_t done = 0;  // bytes handed to threads, in order
+  try {
+    for (size_t t = 0; t < nt; t++) {
+      const size_t b = t * part;
+      if (b >= bytes) break;
+      const size_t len = std::min(part, bytes - b);
+      th.emplace_back([=] { std::memcpy(dst + b, src + b, len); });
+      done = b + len;
+    }
+  } catch (...) {
+    // no thread to spare: the calling thread copies the rest
   }
+  if (done < bytes) std::memcpy(dst + done, src + done, bytes - done);
   for (auto &t : th) t.join();
   return PHDE_OK;
 }
