@@ -144,9 +144,50 @@ __global__ __launch_bounds__(kNB) void net_l1_fwd_kernel(DevGraph g, const float
   }
 }
 
+// Neighbour gathers of the hidden layers, kGat neighbours of a row per
+// round trip: ids [k0, k0 + kGat) of a row starting at b with rem = d - k0
+// of them left.  The loads are unconditional (straight-line code keeps
+// them all in flight); past a row's end the id loads read adj[0] and the
+// row gathers read vertex 0, so every access stays in bounds.
+constexpr int kGat = 16;
+
+__device__ __forceinline__ void gather_ids(const uint32_t *adj, uint32_t b, int rem, uint32_t (&ids)[kGat]) {
+#pragma unroll
+  for (int j = 0; j < kGat; j++) ids[j] = adj[j < rem ? b + (uint32_t)j : 0u];
+}
+
+__device__ __forceinline__ void gather_rows(const uint32_t *hin2, int c2, const uint32_t (&ids)[kGat], int rem,
+                                            uint32_t (&x)[kGat]) {
+#pragma unroll
+  for (int j = 0; j < kGat; j++) x[j] = hin2[(int64_t)(j < rem ? ids[j] : 0u) * 32 + c2];
+}
+
+// Adds the chunk's rows to (s0, s1) in the row's order: whole groups of 4
+// as ((x0 + x1) + x2) + x3, the last < 4 one at a time.
+__device__ __forceinline__ void sum_rows(const uint32_t (&x)[kGat], int rem, float &s0, float &s1) {
+#pragma unroll
+  for (int q = 0; q < kGat; q += 4) {
+    if (q + 4 <= rem) {
+      s0 += bf16_lo(x[q]) + bf16_lo(x[q + 1]) + bf16_lo(x[q + 2]) + bf16_lo(x[q + 3]);
+      s1 += bf16_hi(x[q]) + bf16_hi(x[q + 1]) + bf16_hi(x[q + 2]) + bf16_hi(x[q + 3]);
+    } else {
+#pragma unroll
+      for (int j = q; j < q + 4; j++) {
+        if (j < rem) {
+          s0 += bf16_lo(x[j]);
+          s1 += bf16_hi(x[j]);
+        }
+      }
+    }
+  }
+}
+
 // Hidden layer (128 -> 64 on MFMA).  Each wave owns a 32-vertex tile: the
 // two half-waves gather [h_v, mean_u h_u] rows (32 lanes x 2 columns) into
 // LDS as BF16, then 16 MFMAs multiply the tile by W (held transposed in LDS).
+// The tile's 33 row pointers are loaded once (lane r: rowptr[v0 + r]); a
+// row's first kGat neighbour ids are loaded while the previous row's
+// gathers are in flight.
 __global__ __launch_bounds__(kNB) void net_hidden_fwd_kernel(DevGraph g, const uint16_t *hin,
                                                             const uint16_t *Wb, const float *bias,
                                                             uint16_t *hout, uint16_t *za_out) {
@@ -163,34 +204,53 @@ __global__ __launch_bounds__(kNB) void net_hidden_fwd_kernel(DevGraph g, const u
   const int wpb = kNB / kWave;
   for (int64_t t0 = (int64_t)blockIdx.x * wpb; t0 < ntiles; t0 += (int64_t)gridDim.x * wpb) {
     const int64_t tile = t0 + w;
-    if (tile < ntiles) {
+    if (tile < ntiles) {  // wave-uniform
+      const int64_t v0 = tile * 32;
+      const uint32_t rp = (lane <= 32 && v0 + lane <= n) ? g.rowptr[v0 + lane] : 0u;
+      // start and degree of the tile's row r (degree 0 past the graph's end)
+      auto row = [&](int r, uint32_t &b) {
+        b = __shfl(rp, r);
+        const uint32_t e = __shfl(rp, r + 1);
+        return v0 + r < n ? (int)(e - b) : 0;
+      };
+      uint32_t b, ids[kGat];
+      int d = row(half, b);
+      gather_ids(g.adj, b, d, ids);
       for (int it = 0; it < 16; it++) {
         const int r = 2 * it + half;
-        const int64_t v = tile * 32 + r;
-        uint32_t selfp = 0, aggp = 0;
+        const int64_t v = v0 + r;
+        uint32_t x[kGat];
+        gather_rows(hin2, c2, ids, d, x);
+        const uint32_t selfp = v < n ? hin2[v * 32 + c2] : 0u;
+        // the next row's ids, loaded while this row's gathers are in flight
+        uint32_t bn = 0, nids[kGat];
+        int dn = 0;
+        if (it + 1 < 16) {
+          dn = row(r + 2, bn);
+          gather_ids(g.adj, bn, dn, nids);
+        }
+        float s0 = 0, s1 = 0;
+        sum_rows(x, d, s0, s1);
+        const int dmax = wave_max_i32(d);
+        for (int k0 = kGat; k0 < dmax; k0 += kGat) {  // rows longer than kGat
+          const int rem = d - k0;
+          uint32_t cids[kGat], cx[kGat];
+          gather_ids(g.adj, b + (uint32_t)k0, rem, cids);
+          gather_rows(hin2, c2, cids, rem, cx);
+          sum_rows(cx, rem, s0, s1);
+        }
+        uint32_t aggp = 0;
         if (v < n) {
-          selfp = hin2[v * 32 + c2];
-          const uint32_t b = g.rowptr[v], e = g.rowptr[v + 1];
-          float s0 = 0, s1 = 0;
-          uint32_t k = b;
-          for (; k + 4 <= e; k += 4) {
-            const uint32_t u0 = g.adj[k], u1 = g.adj[k + 1], u2 = g.adj[k + 2], u3 = g.adj[k + 3];
-            const uint32_t x0 = hin2[(int64_t)u0 * 32 + c2], x1 = hin2[(int64_t)u1 * 32 + c2];
-            const uint32_t x2 = hin2[(int64_t)u2 * 32 + c2], x3 = hin2[(int64_t)u3 * 32 + c2];
-            s0 += bf16_lo(x0) + bf16_lo(x1) + bf16_lo(x2) + bf16_lo(x3);
-            s1 += bf16_hi(x0) + bf16_hi(x1) + bf16_hi(x2) + bf16_hi(x3);
-          }
-          for (; k < e; k++) {
-            const uint32_t x0 = hin2[(int64_t)g.adj[k] * 32 + c2];
-            s0 += bf16_lo(x0);
-            s1 += bf16_hi(x0);
-          }
           const float inv = inv_degree(g, v);
           aggp = pack_bf16(s0 * inv, s1 * inv);
           if (za_out) reinterpret_cast<uint32_t *>(za_out)[v * 32 + c2] = aggp;
         }
         reinterpret_cast<uint32_t *>(A + r * kAP)[c2] = selfp;
         reinterpret_cast<uint32_t *>(A + r * kAP + 64)[c2] = aggp;
+        b = bn;
+        d = dn;
+#pragma unroll
+        for (int j = 0; j < kGat; j++) ids[j] = nids[j];
       }
     }
     __syncthreads();
@@ -862,10 +922,13 @@ static int net_forward(phde_mpnn *net, phde_ctx *c, bool train) {
   const int nb = net->nblk;
   const float4 *x = (const float4 *)c->lp_feat;
   net_l1_fwd_kernel<<<nb, kNB, 0, c->stream>>>(c->g, x, net->p, train ? net->xa : nullptr, net->h1);
-  net_hidden_fwd_kernel<<<nb, kNB, 0, c->stream>>>(c->g, net->h1, net->wb, net->p + kOffB2, net->h2,
-                                                  train ? net->za2 : nullptr);
-  net_hidden_fwd_kernel<<<nb, kNB, 0, c->stream>>>(c->g, net->h2, net->wb + 128 * 64, net->p + kOffB3,
-                                                  net->h3, train ? net->za3 : nullptr);
+  // the hidden layers' LDS (51 KB per block) leaves room for 3 blocks per
+  // CU, and their gathers are latency-bound: fill all three
+  const int nbh = nb / 2 * 3;
+  net_hidden_fwd_kernel<<<nbh, kNB, 0, c->stream>>>(c->g, net->h1, net->wb, net->p + kOffB2, net->h2,
+                                                   train ? net->za2 : nullptr);
+  net_hidden_fwd_kernel<<<nbh, kNB, 0, c->stream>>>(c->g, net->h2, net->wb + 128 * 64, net->p + kOffB3,
+                                                   net->h3, train ? net->za3 : nullptr);
   PHDE_HIP(hipGetLastError());
   return PHDE_OK;
 }
