@@ -278,6 +278,43 @@ __global__ __launch_bounds__(kNB) void net_hidden_fwd_kernel(DevGraph g, const u
   }
 }
 
+// Scores without a target: one lane per vertex, its 64 hidden values read
+// as 8 x 16 bytes.  The 64 terms h_k * wo_k are summed in the pairing of
+// net_readout_kernel's butterfly as lane 0 sees it: first
+// fma(h_k, wo_k, h_{k+32} * wo_{k+32}) for k < 32 (that kernel's compiled
+// first step fuses the lane's own product into the add), then k with
+// k + 16, ..., k + 1 — so the scores are bit-identical to it.
+__global__ __launch_bounds__(kNB) void net_score_kernel(const uint16_t *h3, const float *P, int64_t n,
+                                                       float *y) {
+#pragma clang fp contract(off)
+  const float bo = P[kOffBo];
+  float wo[64];
+#pragma unroll
+  for (int k = 0; k < 64; k++) wo[k] = P[kOffWo + k];
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+    const uint4 *row = reinterpret_cast<const uint4 *>(h3 + v * 64);
+    float h[64], p[32];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const uint4 u = row[q];
+      const uint32_t wd[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        h[8 * q + 2 * i] = bf16_lo(wd[i]);
+        h[8 * q + 2 * i + 1] = bf16_hi(wd[i]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 32; k++) p[k] = __builtin_fmaf(h[k], wo[k], h[k + 32] * wo[k + 32]);
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) {
+#pragma unroll
+      for (int k = 0; k < o; k++) p[k] = p[k] + p[k + o];
+    }
+    y[v] = p[0] + bo;
+  }
+}
+
 // Read-out y = h3 . wo + bo, one wave per vertex (lane = column).  With a
 // target it also starts the backward pass: the loss, dL/dy, the read-out
 // gradients, delta3 = dL/dy * wo * [h3 > 0] (stored BF16 for the MFMA
@@ -941,8 +978,9 @@ int net_score(phde_ctx *c, phde_mpnn *net) {
   hipEvent_t ev;
   c->timer.begin(c->stream, PHDE_KERNEL_OTHER, &ev);
   PHDE_TRY_RC(net_forward(net, c, false));
-  net_readout_kernel<<<net->nblk, kNB, 0, c->stream>>>(net->h3, net->p, nullptr, 0.0f, c->g.n,
-                                                      c->lp_score, nullptr, nullptr, nullptr);
+  const int64_t n = c->g.n;
+  const int nbs = (int)std::max<int64_t>(1, std::min<int64_t>((n + kNB - 1) / kNB, (int64_t)net->nblk * 8));
+  net_score_kernel<<<nbs, kNB, 0, c->stream>>>(net->h3, net->p, n, c->lp_score);
   PHDE_HIP(hipGetLastError());
   c->timer.end(c->stream, PHDE_KERNEL_OTHER, ev);
   return PHDE_OK;
