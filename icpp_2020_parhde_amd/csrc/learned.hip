@@ -87,64 +87,7 @@ __global__ void feature_kernel(DevGraph g, const int32_t *d1, const int32_t *d2,
 
 // ------------------------------------------------------------------ forward
 
-// Layer 1 (4 -> 64, FP32): one wave per 64 vertices; lane i first gathers
-// vertex i's neighbour mean, then lane j computes output column j of every
-// vertex of the group (coalesced 128-byte rows).
-__global__ __launch_bounds__(kNB) void net_l1_fwd_kernel(DevGraph g, const float4 *x, const float *P,
-                                                        float4 *xa_out, uint16_t *h1) {
-  __shared__ float zs[kNB / kWave][64][9];
-  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-  float W[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) W[k] = P[kOffW1 + k * 64 + lane];
-  const float bias = P[kOffB1 + lane];
-  const int64_t n = g.n, ngroups = (n + 63) / 64;
-  const int wpb = kNB / kWave;
-  for (int64_t g0 = (int64_t)blockIdx.x * wpb; g0 < ngroups; g0 += (int64_t)gridDim.x * wpb) {
-    const int64_t grp = g0 + w;
-    const int64_t v = grp * 64 + lane;
-    float4 xv = make_float4(0, 0, 0, 0), ag = make_float4(0, 0, 0, 0);
-    if (grp < ngroups && v < n) {
-      xv = x[v];
-      const uint32_t b = g.rowptr[v], e = g.rowptr[v + 1];
-      for (uint32_t k = b; k < e; k++) {
-        const float4 xu = x[g.adj[k]];
-        ag.x += xu.x;
-        ag.y += xu.y;
-        ag.z += xu.z;
-        ag.w += xu.w;
-      }
-      const float inv = inv_degree(g, v);
-      ag.x *= inv;
-      ag.y *= inv;
-      ag.z *= inv;
-      ag.w *= inv;
-      if (xa_out) xa_out[v] = ag;
-    }
-    zs[w][lane][0] = xv.x;
-    zs[w][lane][1] = xv.y;
-    zs[w][lane][2] = xv.z;
-    zs[w][lane][3] = xv.w;
-    zs[w][lane][4] = ag.x;
-    zs[w][lane][5] = ag.y;
-    zs[w][lane][6] = ag.z;
-    zs[w][lane][7] = ag.w;
-    __syncthreads();
-    if (grp < ngroups) {
-      const int64_t left = n - grp * 64;
-      const int cnt = left < 64 ? (int)left : 64;
-      for (int j = 0; j < cnt; j++) {
-        float a = bias;
-#pragma unroll
-        for (int k = 0; k < 8; k++) a += zs[w][j][k] * W[k];
-        h1[(grp * 64 + j) * 64 + lane] = bf16_bits(fmaxf(a, 0.0f));
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Neighbour gathers of the hidden layers, kGat neighbours of a row per
+// Neighbour gathers of the network layers, kGat neighbours of a row per
 // round trip: ids [k0, k0 + kGat) of a row starting at b with rem = d - k0
 // of them left.  The loads are unconditional (straight-line code keeps
 // them all in flight); past a row's end the id loads read adj[0] and the
@@ -179,6 +122,82 @@ __device__ __forceinline__ void sum_rows(const uint32_t (&x)[kGat], int rem, flo
         }
       }
     }
+  }
+}
+
+// Layer 1 (4 -> 64, FP32): one wave per 64 vertices; lane i first gathers
+// vertex i's neighbour mean, then lane j computes output column j of every
+// vertex of the group (coalesced 128-byte rows).
+__global__ __launch_bounds__(kNB) void net_l1_fwd_kernel(DevGraph g, const float4 *x, const float *P,
+                                                        float4 *xa_out, uint16_t *h1) {
+  __shared__ float zs[kNB / kWave][64][9];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  float W[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) W[k] = P[kOffW1 + k * 64 + lane];
+  const float bias = P[kOffB1 + lane];
+  const int64_t n = g.n, ngroups = (n + 63) / 64;
+  const int wpb = kNB / kWave;
+  for (int64_t g0 = (int64_t)blockIdx.x * wpb; g0 < ngroups; g0 += (int64_t)gridDim.x * wpb) {
+    const int64_t grp = g0 + w;
+    const int64_t v = grp * 64 + lane;
+    float4 xv = make_float4(0, 0, 0, 0), ag = make_float4(0, 0, 0, 0);
+    const bool own = grp < ngroups && v < n;  // grp is wave-uniform
+    uint32_t b = 0;
+    int d = 0;
+    if (own) {
+      b = g.rowptr[v];
+      d = (int)(g.rowptr[v + 1] - b);
+    }
+    // neighbours added one at a time in row order, loaded kGat per round
+    // trip (unconditional loads, clamped past the row's end)
+    const int dmax = wave_max_i32(d);
+    for (int k0 = 0; k0 < dmax; k0 += kGat) {
+      const int rem = d - k0;
+      uint32_t ids[kGat];
+      gather_ids(g.adj, b + (uint32_t)k0, rem, ids);
+      float4 xu[kGat];
+#pragma unroll
+      for (int j = 0; j < kGat; j++) xu[j] = x[j < rem ? ids[j] : 0u];
+#pragma unroll
+      for (int j = 0; j < kGat; j++) {
+        if (j < rem) {
+          ag.x += xu[j].x;
+          ag.y += xu[j].y;
+          ag.z += xu[j].z;
+          ag.w += xu[j].w;
+        }
+      }
+    }
+    if (own) {
+      xv = x[v];
+      const float inv = inv_degree(g, v);
+      ag.x *= inv;
+      ag.y *= inv;
+      ag.z *= inv;
+      ag.w *= inv;
+      if (xa_out) xa_out[v] = ag;
+    }
+    zs[w][lane][0] = xv.x;
+    zs[w][lane][1] = xv.y;
+    zs[w][lane][2] = xv.z;
+    zs[w][lane][3] = xv.w;
+    zs[w][lane][4] = ag.x;
+    zs[w][lane][5] = ag.y;
+    zs[w][lane][6] = ag.z;
+    zs[w][lane][7] = ag.w;
+    __syncthreads();
+    if (grp < ngroups) {
+      const int64_t left = n - grp * 64;
+      const int cnt = left < 64 ? (int)left : 64;
+      for (int j = 0; j < cnt; j++) {
+        float a = bias;
+#pragma unroll
+        for (int k = 0; k < 8; k++) a += zs[w][j][k] * W[k];
+        h1[(grp * 64 + j) * 64 + lane] = bf16_bits(fmaxf(a, 0.0f));
+      }
+    }
+    __syncthreads();
   }
 }
 
