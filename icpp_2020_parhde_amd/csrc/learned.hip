@@ -528,18 +528,27 @@ __global__ __launch_bounds__(kNB) void net_agg_bwd_kernel(DevGraph g, const floa
   const int64_t hw = (int64_t)blockIdx.x * (kNB / 32) + half_id, nhw = (int64_t)gridDim.x * (kNB / 32);
   for (int64_t u = hw; u < g.n; u += nhw) {
     float2 acc = gs2[u * 32 + c2];
-    const uint32_t b = g.rowptr[u], e = g.rowptr[u + 1];
-    uint32_t k = b;
-    for (; k + 2 <= e; k += 2) {
-      const float2 t0 = ga2[(int64_t)g.adj[k] * 32 + c2];
-      const float2 t1 = ga2[(int64_t)g.adj[k + 1] * 32 + c2];
-      acc.x += t0.x + t1.x;
-      acc.y += t0.y + t1.y;
-    }
-    if (k < e) {
-      const float2 t0 = ga2[(int64_t)g.adj[k] * 32 + c2];
-      acc.x += t0.x;
-      acc.y += t0.y;
+    const uint32_t b = g.rowptr[u];
+    const int d = (int)(g.rowptr[u + 1] - b);
+    // kGat neighbours per round trip; added in row order as pairs
+    // (t0 + t1), an odd last one alone
+    for (int k0 = 0; k0 < d; k0 += kGat) {
+      const int rem = d - k0;
+      uint32_t ids[kGat];
+      gather_ids(g.adj, b + (uint32_t)k0, rem, ids);
+      float2 t[kGat];
+#pragma unroll
+      for (int j = 0; j < kGat; j++) t[j] = ga2[(int64_t)(j < rem ? ids[j] : 0u) * 32 + c2];
+#pragma unroll
+      for (int q = 0; q < kGat; q += 2) {
+        if (q + 2 <= rem) {
+          acc.x += t[q].x + t[q + 1].x;
+          acc.y += t[q].y + t[q + 1].y;
+        } else if (q < rem) {
+          acc.x += t[q].x;
+          acc.y += t[q].y;
+        }
+      }
     }
     const uint32_t hp = hp2[u * 32 + c2];
     const float d0 = bf16_lo(hp) > 0.0f ? acc.x : 0.0f;
