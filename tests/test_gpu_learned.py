@@ -238,3 +238,42 @@ def test_trained_network_gains_over_no_network(gpu):
         print(f"n={g.n}: trained {100 * gt:+.3f}%, no network {100 * g0:+.3f}%")
     assert all(gt <= 0.02 for gt, _ in gaps)
     assert np.mean([gt for gt, _ in gaps]) < np.mean([g0 for _, g0 in gaps])
+
+
+def _chunk_boundary_graph():
+    """A path over 1000 vertices (the last 32-vertex tile partial) with
+    vertices whose degrees sit at the network gathers' 16-neighbour chunk
+    boundaries (16, 17, 32, 33, 48, 49) and one hub of degree 302."""
+    n = 1000
+    rng = np.random.default_rng(21)
+    u = list(range(n - 1))
+    v = list(range(1, n))
+    hubs = ((100, 16), (200, 17), (300, 32), (400, 33), (500, 48), (600, 49), (700, 302))
+    for hub, deg in hubs:
+        cand = [w for w in range(n) if abs(w - hub) > 1 and w not in dict(hubs)]
+        for w in rng.choice(cand, size=deg - 2, replace=False):
+            u.append(hub)
+            v.append(int(w))
+    return Graph.from_edges(n, np.array(u), np.array(v))
+
+
+def test_forward_and_gradient_chunk_boundaries(gpu):
+    g = _chunk_boundary_graph()
+    deg = g.degrees
+    assert {16, 17, 32, 33, 48, 49, 302} <= set(int(d) for d in deg)
+    rng = np.random.default_rng(4)
+    with L.Context(g) as ctx, L.PivotNet(seed=1) as net:
+        x, _ = ctx.pivot_features(0)
+        t = rng.random(g.n).astype(np.float32)
+        ctx.set_training_data(x, t)
+        p = _random_params(9)
+        net.params = p
+        y = net.scores(ctx)
+        loss, grad = net.gradient(ctx)
+    loss_o, grad_o, y_o = OL.loss_and_grad(g, x, t, p, emulate=True)
+    err_y = np.max(np.abs(y - y_o)) / np.max(np.abs(y_o))
+    print(f"chunk boundaries: forward rel err {err_y:.2e}, loss {loss:.6g} vs {loss_o:.6g}")
+    assert err_y < 5e-3
+    assert loss == pytest.approx(loss_o, rel=5e-3)
+    rel = np.linalg.norm(grad - grad_o) / np.linalg.norm(grad_o)
+    assert rel < 1e-2
