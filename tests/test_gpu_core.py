@@ -15,6 +15,7 @@ from oracle import pyoracle as O
 pytestmark = pytest.mark.gpu
 GOLDEN = Path(__file__).resolve().parent / "golden"
 FP64_TOL = 1e-8  # relative, after eigenvector sign alignment
+FP32_TOL = 1e-5  # FP32 S and LS, FP64 accumulations and eigensolve, against the FP64 oracle
 
 
 def aligned_rel_err(X, Y):
@@ -149,7 +150,8 @@ def test_layout_mesh128_golden(gpu):
         assert np.array_equal(Xe, X)  # eager and lazy: identical outputs
 
 
-@pytest.mark.parametrize("name,s", [("mesh128", 50), ("rgg20k", 50), ("rmat14", 20), ("holed", 30)])
+@pytest.mark.parametrize("name,s", [("mesh128", 50), ("rgg20k", 50), ("rmat14", 20), ("rmat14", 50),
+                                    ("holed", 30)])
 def test_layout_fp64_parity(gpu, graphs, name, s):
     g = graphs[name]
     ref = O.parhde(g, s, 2, lazy=True)
@@ -163,7 +165,7 @@ def test_layout_fp64_parity(gpu, graphs, name, s):
     assert err < FP64_TOL
 
 
-@pytest.mark.parametrize("name,s", [("mesh128", 50), ("rgg20k", 50), ("rmat14", 20)])
+@pytest.mark.parametrize("name,s", [("mesh128", 50), ("rgg20k", 50), ("rmat14", 20), ("rmat14", 50)])
 def test_layout_fp32_vs_fp64_oracle(gpu, graphs, name, s):
     g = graphs[name]
     ref = O.parhde(g, s, 2, lazy=True)
@@ -172,7 +174,7 @@ def test_layout_fp32_vs_fp64_oracle(gpu, graphs, name, s):
     assert st.pivots == ref["pivots"]
     err = aligned_rel_err(X, ref["coords"])
     print(f"{name} s={s} fp32 coord rel err {err:.2e}")
-    assert err < 1e-4
+    assert err < FP32_TOL
 
 
 @pytest.mark.parametrize("algo", [L.ALGO_PIVOTMDS, L.ALGO_PHDE])

@@ -14,7 +14,7 @@ from icpp_2020_parhde_amd.graphs import Graph
 from oracle import learned as OL
 from oracle import pyoracle as O
 
-from test_gpu_core import aligned_rel_err, FP64_TOL
+from test_gpu_core import aligned_rel_err, FP32_TOL, FP64_TOL
 
 pytestmark = pytest.mark.gpu
 INF = 2**30 - 1
@@ -108,7 +108,7 @@ def test_sssp(gpu):
 def test_parhde_layout(gpu, name, s):
     g = GRAPHS[name]()
     ref = O.parhde(g, s, 2, lazy=True)
-    for prec, tol in ((L.PREC_FP64, FP64_TOL), (L.PREC_FP32, 1e-4)):
+    for prec, tol in ((L.PREC_FP64, FP64_TOL), (L.PREC_FP32, FP32_TOL)):
         with context(g, True) as ctx:
             X, st = ctx.layout(s=s, num_coord=2, precision=prec)
         assert st.pivots == ref["pivots"]
