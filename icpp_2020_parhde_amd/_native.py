@@ -48,7 +48,7 @@ class PhdeOptions(ctypes.Structure):
         ("judge_tol", ctypes.c_double),
         ("nms_rho", ctypes.c_double),
         ("judge", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("diagnostics", ctypes.c_int32),
     ]
 
 
@@ -73,6 +73,11 @@ class PhdeStats(ctypes.Structure):
         ("hall_energy_kcenters", ctypes.c_double),
         ("fallback", ctypes.c_int32),
         ("n_candidates", ctypes.c_int32),
+        ("n_examined", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("normdist", ctypes.c_double * 192),
+        ("eigenvalues", ctypes.c_double * 64),
+        ("dorth_err", ctypes.c_double),
     ]
 
 

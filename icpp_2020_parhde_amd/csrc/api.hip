@@ -5,7 +5,7 @@
 // ParHDE::run (variants/ParHDE.cpp:45-352), WParHDE::run
 // (variants/WParHDE.cpp:40-291), PivotMDS::run (variants/PivotMDS.cpp:68-147)
 // and PHDE::run (variants/PHDE.cpp:50-136), followed by getCoordinates
-// (variants/DrawingBase.cpp:264-273).  All n-sized work runs on the GPU; the
+// (variants/DrawingBase.cpp:196-205).  All n-sized work runs on the GPU; the
 // host keeps the pivot bookkeeping (the `sources` set of the dedupe rule),
 // the accept/discard decision per candidate and the start values of the
 // power method.
@@ -20,8 +20,10 @@
 #include <thread>
 #include <vector>
 
+#include "device_util.h"
 #include "eig.h"
 #include "learned.h"
+#include "mds.h"
 #include "orth.h"
 #include "phde_common.h"
 #include "spmm.h"
@@ -97,8 +99,8 @@ __global__ void degree_kernel(const uint32_t *rowptr, const int32_t *w, int64_t 
 }
 
 __global__ void gather_cols_kernel(const double *K, int dim, int c, bool largest, double *out) {
-  // eigenvector selection of DrawingBase.cpp:138-140 (Largest) and
-  // :168-170 (First)
+  // eigenvector selection of DrawingBase.cpp:70-72 (Largest) and
+  // :100-102 (First)
   for (int e = threadIdx.x; e < dim * c; e += blockDim.x) {
     const int i = e % dim, j = e / dim;
     out[e] = largest ? K[i + (dim - j - 1) * dim] : K[i + j * dim];
@@ -201,12 +203,12 @@ static double ms_since(clk::time_point a) {
 
 // k-centers bookkeeping shared by every algorithm: the first maximiser of
 // the running min distance becomes the next source, with the reference's
-// dedupe rule (ParHDE.cpp:142-155; DrawingBase.cpp:225-237).
+// dedupe rule (ParHDE.cpp:142-155; DrawingBase.cpp:157-168).
 struct KCenters {
   std::set<int64_t> sources;
   int64_t next = 0;
   // record_old: ParHDE / WParHDE insert the argmax (ParHDE.cpp:143,155);
-  // distancePhaseComputation inserts the current source (DrawingBase.cpp:206,237).
+  // distancePhaseComputation inserts the current source (DrawingBase.cpp:138,169).
   int step(phde_ctx *c, const int32_t *min_dist_dev, const KcResult &kc, bool check_dup,
            bool record_argmax, int64_t current) {
     const int64_t maxval = (int64_t)(kc.key >> 32);
@@ -231,15 +233,24 @@ struct KCenters {
   }
 };
 
+// S^T D S (generalised eigenproblem's right-hand side) via the Gram kernel
+// on D^{1/2} S.
+int weighted_gram(phde_ctx *c, const void *S_dev, bool fp32, int s, double *out_dev);
+
 struct LayoutOut {
   phde_stats *st = nullptr;
   double t_dist = 0, t_orth = 0, t_mmult = 0, t_eig = 0, t_coords = 0, t_select = 0;
   double energy = 0;  // Hall energy of the layout, when requested
-  int ntrav = 0, nacc = 0, ndisc = 0, iters = 0;
+  int ntrav = 0, nacc = 0, ndisc = 0, iters = 0, nexam = 0;
   long long levels = 0;
   void pivot(int64_t p) {
     if (st && ntrav < 192) st->pivots[ntrav] = p;
     ntrav++;
+  }
+  // ||a|| of the next examined candidate after its projection
+  void examined(double normdist) {
+    if (st && nexam < 192) st->normdist[nexam] = normdist;
+    nexam++;
   }
 };
 
@@ -298,9 +309,11 @@ static int result_to_host(phde_ctx *c, void *host, const void *dev, size_t bytes
   return PHDE_OK;
 }
 
+// S_dev's rows are in the context's internal numbering unless
+// caller_rows (then the coordinates need no reordering; no Hall energy).
 static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, bool fp32, int s,
                          double *K_dev, const double *B_dev, bool largest, double *coords,
-                         LayoutOut &out, bool want_energy = false) {
+                         LayoutOut &out, bool want_energy = false, bool caller_rows = false) {
   const int nc = o->num_coord;
   auto t0 = clk::now();
   DBuf<double> init, evals, work, V, evecs;
@@ -338,6 +351,10 @@ static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, 
     return PHDE_ERR_EIG;
   }
   out.iters = o->eig_mode == PHDE_EIG_POWER ? h_status[1] : 0;
+  if (out.st) {
+    PHDE_HIP(hipMemcpyAsync(out.st->eigenvalues, evals.p, sizeof(double) * s, hipMemcpyDeviceToHost, c->stream));
+    PHDE_HIP(hipStreamSynchronize(c->stream));
+  }
   auto t1 = clk::now();
   DBuf<double> X, part, inv;
   PHDE_TRY(X.alloc((size_t)c->g.n * nc));
@@ -345,7 +362,7 @@ static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, 
   PHDE_TRY(inv.alloc(nc));
   PHDE_TRY(coords_compute(c, S_dev, fp32, s, nc, V.p, X.p, part.p, inv.p));
   DBuf<double> Xo;  // rows in the caller's numbering
-  if (renumbered(c)) {
+  if (renumbered(c) && !caller_rows) {
     PHDE_TRY(Xo.alloc((size_t)c->g.n * nc));
     PHDE_TRY(rows_to_original(c, X.p, Xo.p, nc, sizeof(double)));
   }
@@ -361,18 +378,42 @@ static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, 
   return PHDE_OK;
 }
 
-// S^T D S (generalised eigenproblem's right-hand side) via the Gram kernel
-// on D^{1/2} S.
-int weighted_gram(phde_ctx *c, const void *S_dev, bool fp32, int s, double *out_dev);
+
+// max_ij |B_ij / sqrt(B_ii B_jj) - delta_ij| of B = [1 S]^T D [1 S]: how far
+// the accepted columns and the ones vector are from D-orthogonal (the
+// reference's columns are D-orthogonal and of unit L2 norm, so B is
+// diagonal, not the identity, and is normalised by its diagonal first).
+static int dorth_error(phde_ctx *c, Orth &orth, bool fp32, int s, double *err) {
+  DBuf<double> G;
+  PHDE_TRY(G.alloc((size_t)s * s));
+  PHDE_TRY(weighted_gram(c, orth.S, fp32, s, G.p));
+  std::vector<double> g((size_t)s * s), ones(s + 1);
+  PHDE_HIP(hipMemcpyAsync(g.data(), G.p, sizeof(double) * s * s, hipMemcpyDeviceToHost, c->stream));
+  PHDE_TRY(orth.ones_dots(ones.data()));  // synchronises
+  const int d = s + 1;
+  std::vector<double> B((size_t)d * d);
+  B[0] = ones[s];
+  for (int k = 0; k < s; k++) B[k + 1] = B[(size_t)(k + 1) * d] = ones[k];
+  for (int i = 0; i < s; i++)
+    for (int k = 0; k < s; k++) B[(size_t)(i + 1) * d + k + 1] = g[(size_t)i * s + k];
+  double e = 0;
+  for (int i = 0; i < d; i++)
+    for (int k = 0; k < d; k++) {
+      const double v = B[(size_t)i * d + k] / std::sqrt(B[(size_t)i * d + i] * B[(size_t)k * d + k]);
+      e = std::max(e, std::fabs(v - (i == k ? 1.0 : 0.0)));
+    }
+  *err = e;
+  return PHDE_OK;
+}
 
 // One candidate column through CGS (ParHDE) or MGS (WParHDE), with the
 // reference's column normalisation 1/sqrt(sum d^2) (ParHDE.cpp:176-185) and
 // discard rule ||a|| < 0.001 (ParHDE.cpp:269-283); `accepted` counts the
 // columns of S so far.
 static int orth_candidate(Orth &orth, const phde_options *o, bool weighted, bool fp32,
-                          const int32_t *col, unsigned long long sumsq, int &accepted,
+                          const int32_t *col, double sumsq, int &accepted,
                           LayoutOut &out) {
-  const double scale = 1 / std::sqrt((double)sumsq);
+  const double scale = 1 / std::sqrt(sumsq);
   PHDE_TRY(orth.load_candidate(col, scale));
   const int passes = fp32 ? 2 : 1;
   for (int p = 0; p < passes; p++) {
@@ -383,6 +424,7 @@ static int orth_candidate(Orth &orth, const phde_options *o, bool weighted, bool
   }
   double nd = 0;
   PHDE_TRY(orth.read_norm(&nd));
+  out.examined(nd);
   if (nd < 0.001) {
     if (o->verbose) std::printf("discarding vec %d, normdist %g\n", accepted + 1, nd);
     out.ndisc++;
@@ -394,39 +436,43 @@ static int orth_candidate(Orth &orth, const phde_options *o, bool weighted, bool
 }
 
 // Candidates cols[0..count) in order through orth_candidate's steps until
-// s columns are accepted; *consumed gets how many were taken up.  FP64
-// ParHDE takes them kCgsBlock at a time (Orth::cgs_block: the same columns,
-// bit for bit, with one pass over the earlier columns per block);
+// s columns are accepted; *consumed gets how many were taken up.  ParHDE
+// takes them kCgsBlock at a time (Orth::cgs_block: FP64 the same columns,
+// bit for bit, with one pass over the earlier columns per block; FP32 CGS2
+// with its two passes over the earlier columns shared by the block);
 // PHDE_CGS_BLOCK=0 takes them one at a time.
 static bool cgs_blocked(bool weighted, bool fp32) {
   static const bool env = [] {
     const char *e = getenv("PHDE_CGS_BLOCK");
     return !(e && atoi(e) == 0);
   }();
-  return !weighted && !fp32 && env;
+  (void)fp32;
+  return !weighted && env;
 }
 
 static int orth_candidates(Orth &orth, const phde_options *o, bool weighted, bool fp32,
-                           const int32_t *const *cols, const unsigned long long *sumsq, int count, int &accepted,
+                           const int32_t *const *cols, const double *sumsq, int count, int &accepted,
                            LayoutOut &out, int *consumed = nullptr) {
   int i = 0;
   if (cgs_blocked(weighted, fp32)) {
     while (i < count && accepted < o->s) {
       const int nc = std::min(kCgsBlock, count - i);
       double scales[kCgsBlock];
-      for (int b = 0; b < nc; b++) scales[b] = 1 / std::sqrt((double)sumsq[i + b]);
+      for (int b = 0; b < nc; b++) scales[b] = 1 / std::sqrt(sumsq[i + b]);
       std::vector<bool> taken;
+      std::vector<double> norms;
       const int before = accepted;
-      PHDE_TRY(orth.cgs_block(cols + i, scales, nc, o->s, accepted, taken));
+      PHDE_TRY(orth.cgs_block(cols + i, scales, nc, o->s, accepted, taken, norms));
       // candidates looked at: up to the one that completed s (or all)
       int looked = 0, acc = before;
       for (int b = 0; b < nc; b++) {
         if (acc >= o->s) break;
         looked++;
+        out.examined(norms[b]);
         if (taken[b]) {
           acc++;
         } else {
-          if (o->verbose) std::printf("discarding vec %d, normdist < 0.001\n", acc + 1);
+          if (o->verbose) std::printf("discarding vec %d, normdist %g\n", acc + 1, norms[b]);
           out.ndisc++;
         }
       }
@@ -454,6 +500,7 @@ static int not_enough_pivots(const phde_options *o, int accepted, int limit) {
 static int hde_tail(phde_ctx *c, const phde_options *o, Orth &orth, bool fp32, double *coords,
                     LayoutOut &out, bool want_energy) {
   const int s = o->s;
+  if (o->diagnostics && out.st) PHDE_TRY(dorth_error(c, orth, fp32, s, &out.st->dorth_err));
   auto t2 = clk::now();
   Lap lap;
   PHDE_TRY(lap.setup(c, c->g.n, s, fp32));
@@ -499,7 +546,7 @@ static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, Layout
   // through blocked CGS (the k-centers chain does not depend on them)
   const bool lazy_blocked = !eager && cgs_blocked(weighted, fp32);
   PHDE_TRY(cols.alloc(eager ? (size_t)n * limit : lazy_blocked ? (size_t)n * std::min(s, limit) : (size_t)n));
-  std::vector<unsigned long long> sumsq(limit + 1, 0);
+  std::vector<double> sumsq(limit + 1, 0);
   KCenters kc;
   kc.next = o->first_pivot;
   int accepted = 0;
@@ -518,7 +565,7 @@ static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, Layout
       return PHDE_ERR_UNREACHED;
     }
     out.levels += r.depth;
-    sumsq[run_count] = r.sumsq;
+    sumsq[run_count] = u128_to_double(r.sumsq);
     const bool first = weighted ? accepted == 0 : run_count == 1;
     return kc.step(c, min_dist.p, r, !first, true, src);
   };
@@ -560,7 +607,7 @@ static int layout_hde(phde_ctx *c, const phde_options *o, double *coords, Layout
 }
 
 // Sources of the random strategies: srand(seed), then rand() % n per source
-// (ParHDE.cpp:160-162, DrawingBase.cpp:242-244; the reference seeds with
+// (ParHDE.cpp:160-162, DrawingBase.cpp:175-177; the reference seeds with
 // time(NULL), ParHDE.cpp:113).  RandFine and RandCoarse differ in the
 // reference only in how each column is traversed (DOBFS vs sequential BFS,
 // ParHDE.cpp:163-171); both give the same levels, here from the multi-source
@@ -577,7 +624,7 @@ static std::vector<int64_t> random_sources(const phde_options *o, int64_t n, int
 // a time for weighted graphs).  D must hold n * k entries; sumsq gets the
 // per-column sums of squares.
 static int columns_of(phde_ctx *c, const phde_options *o, const int64_t *src, int k, int32_t *D,
-                      unsigned long long *sumsq, LayoutOut &out) {
+                      double *sumsq, LayoutOut &out) {
   const int64_t n = c->g.n;
   std::vector<int64_t> isrc(k);
   for (int i = 0; i < k; i++) {
@@ -587,26 +634,24 @@ static int columns_of(phde_ctx *c, const phde_options *o, const int64_t *src, in
   if (c->g.w) {
     for (int i = 0; i < k; i++) {
       PHDE_TRY(sssp_run(c, isrc[i], o->delta, D + (size_t)i * n, nullptr, false));
-      std::vector<unsigned long long> sq(2);
-      PHDE_TRY(column_sumsq(c, D + (size_t)i * n, 1, sq.data()));
-      if (sq[1]) {
+      unsigned long long unreached = 0;
+      PHDE_TRY(column_sumsq(c, D + (size_t)i * n, 1, sumsq + i, &unreached));
+      if (unreached) {
         set_error("graph is not connected");
         return PHDE_ERR_UNREACHED;
       }
-      sumsq[i] = sq[0];
     }
     return PHDE_OK;
   }
   for (int b = 0; b < k; b += 64) {
     const int kb = std::min(64, k - b);
     PHDE_TRY(msbfs_run(c, isrc.data() + b, kb, D + (size_t)b * n));
-    std::vector<unsigned long long> sq(kb + 1);
-    PHDE_TRY(column_sumsq(c, D + (size_t)b * n, kb, sq.data()));
-    if (sq[kb]) {
-      set_error("graph is not connected: " + std::to_string(sq[kb]) + " unreached entries");
+    unsigned long long unreached = 0;
+    PHDE_TRY(column_sumsq(c, D + (size_t)b * n, kb, sumsq + b, &unreached));
+    if (unreached) {
+      set_error("graph is not connected: " + std::to_string(unreached) + " unreached entries");
       return PHDE_ERR_UNREACHED;
     }
-    for (int i = 0; i < kb; i++) sumsq[b + i] = sq[i];
   }
   return PHDE_OK;
 }
@@ -628,7 +673,7 @@ static int layout_hde_random(phde_ctx *c, const phde_options *o, double *coords,
   const int batch = weighted ? 1 : 64;
   DBuf<int32_t> D;
   PHDE_TRY(D.alloc((size_t)n * std::min(batch, limit)));
-  std::vector<unsigned long long> sumsq(64);
+  std::vector<double> sumsq(64);
   int accepted = 0;
   for (int b = 0; b < limit && accepted < s; b += batch) {
     const int kb = std::min(batch, limit - b);
@@ -679,8 +724,8 @@ static int layout_learned(phde_ctx *c, const phde_options *o, double *coords, La
   out.pivot(picks[0]);
   out.pivot(picks[1]);
   out.levels += e1 + e2;
-  std::vector<unsigned long long> sumsq(kmax + 1);
-  PHDE_TRY(column_sumsq(c, c->lp_d12, 2, sumsq.data()));
+  std::vector<double> sumsq(kmax + 1);
+  PHDE_TRY(column_sumsq(c, c->lp_d12, 2, sumsq.data(), nullptr));
   DBuf<int32_t> D;
   PHDE_TRY(D.alloc((size_t)n * std::max(1, kmax - 2)));
   Orth orth;
@@ -721,57 +766,8 @@ static int layout_learned(phde_ctx *c, const phde_options *o, double *coords, La
   return hde_tail(c, o, orth, fp32, coords, out, true);
 }
 
-__global__ void center_kernel(const int32_t *D, int64_t n, int s, const double *colterm,
-                              const double *rowterm, double total, bool dbl, double *C) {
-  // PivotMDS doubleCentering (PivotMDS.cpp:53-63) when dbl, PHDE
-  // columnCentering (PHDE.cpp:30-47) otherwise.
-  const double rN = (double)1 / (double)n, rK = (double)1 / (double)s;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * s;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t j = e % n;
-    const int i = (int)(e / n);
-    const double d = (double)D[e];
-    if (dbl)
-      C[e] = -0.5 * (d * d - (rN * colterm[i]) - (rK * rowterm[j]) + (rK * rN * total));
-    else
-      C[e] = d - colterm[i];
-  }
-}
-
-__global__ void center_sums_kernel(const int32_t *D, int64_t n, int s, bool squares,
-                                   double *rowterm, double *colpart) {
-  // row sums of D^2 (one thread per vertex, columns in order) and per-block
-  // column partials
-  __shared__ double sh[64][4];
-  double cp[64];
-  for (int i = 0; i < s; i++) cp[i] = 0;
-  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    double rs = 0;
-    for (int i = 0; i < s; i++) {
-      const double d = (double)D[(int64_t)i * n + j];
-      const double t = squares ? d * d : d;
-      rs += t;
-      cp[i] += t;
-    }
-    if (rowterm) rowterm[j] = rs;
-  }
-  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
-  for (int i = 0; i < s; i++) {
-    double v = cp[i];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane == 0) sh[i][w] = v;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < s; i += blockDim.x) {
-    double v = 0;
-    for (int q = 0; q < (int)(blockDim.x / 64); q++) v += sh[i][q];
-    colpart[(int64_t)i * gridDim.x + blockIdx.x] = v;
-  }
-}
-
-// PivotMDS / PHDE driver: s k-centers traversals (DrawingBase.cpp:193-240),
-// centring, C^T C, eig(Largest), C * V.
+// PivotMDS / PHDE driver: s k-centers traversals (DrawingBase.cpp:125-172),
+// centring, C^T C, eig(Largest), C * V, all in the caller's numbering.
 static int layout_mds(phde_ctx *c, const phde_options *o, double *coords, LayoutOut &out) {
   const int64_t n = c->g.n;
   const int s = o->s;
@@ -784,9 +780,9 @@ static int layout_mds(phde_ctx *c, const phde_options *o, double *coords, Layout
   kc.next = o->first_pivot;
   auto t0 = clk::now();
   if (o->strategy == PHDE_PIVOTS_RANDFINE || o->strategy == PHDE_PIVOTS_RANDCOARSE) {
-    // DrawingBase.cpp:240-260: s random sources
+    // DrawingBase.cpp:173-190: s random sources
     const std::vector<int64_t> src = random_sources(o, n, s);
-    std::vector<unsigned long long> sq(s);
+    std::vector<double> sq(s);
     PHDE_TRY(columns_of(c, o, src.data(), s, D.p, sq.data(), out));
   }
   for (int rc = 0; rc < s && o->strategy == PHDE_PIVOTS_KCENTERS; rc++) {
@@ -807,40 +803,55 @@ static int layout_mds(phde_ctx *c, const phde_options *o, double *coords, Layout
   }
   out.t_dist += ms_since(t0);
   auto t1 = clk::now();
-  const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, c->num_cus * 4));
-  DBuf<double> rowterm, colpart, C;
+  DBuf<int32_t> Dc;  // the columns in the caller's numbering
+  if (renumbered(c)) {
+    PHDE_TRY(Dc.alloc((size_t)n * s));
+    PHDE_TRY(rows_to_original(c, D.p, Dc.p, s, sizeof(int32_t)));
+  }
+  const int32_t *Dv = Dc.p ? Dc.p : D.p;
+  DBuf<double> rowterm, C, colterm_d;
+  DBuf<unsigned long long> acc;
   PHDE_TRY(rowterm.alloc(n));
-  PHDE_TRY(colpart.alloc((size_t)s * nb));
+  PHDE_TRY(acc.alloc(2 * (size_t)s));
   PHDE_TRY(C.alloc((size_t)n * s));
-  center_sums_kernel<<<nb, 256, 0, c->stream>>>(D.p, n, s, dbl, rowterm.p, colpart.p);
-  PHDE_HIP(hipGetLastError());
-  std::vector<double> cp((size_t)s * nb), colterm(s);
-  PHDE_HIP(hipMemcpyAsync(cp.data(), colpart.p, sizeof(double) * s * nb, hipMemcpyDeviceToHost,
-                          c->stream));
+  PHDE_TRY(colterm_d.alloc(s));
+  PHDE_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long) * 2 * s, c->stream));
+  PHDE_TRY(mds_center_sums(c, Dv, n, s, dbl, rowterm.p, acc.p));
+  std::vector<unsigned long long> ha(2 * (size_t)s);
+  PHDE_HIP(hipMemcpyAsync(ha.data(), acc.p, sizeof(unsigned long long) * 2 * s, hipMemcpyDeviceToHost, c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
+  std::vector<double> colsum(s), colterm(s);
+  bool inexact = false;
+  for (int i = 0; i < s; i++) {
+    colsum[i] = u128_to_double(&ha[2 * (size_t)i]);
+    inexact |= ha[2 * (size_t)i + 1] != 0 || ha[2 * (size_t)i] >= (1ull << 53);
+  }
+  if (inexact) {
+    // a total at or above 2^53: the reference's sequential sums round
+    PHDE_TRY(mds_column_sums_seq(c, Dv, n, s, dbl, colterm_d.p));
+    PHDE_HIP(hipMemcpyAsync(colsum.data(), colterm_d.p, sizeof(double) * s, hipMemcpyDeviceToHost, c->stream));
+    PHDE_HIP(hipStreamSynchronize(c->stream));
+  }
   double total = 0;
   for (int i = 0; i < s; i++) {
-    double v = 0;
-    for (int b = 0; b < nb; b++) v += cp[(size_t)i * nb + b];
-    colterm[i] = dbl ? v : v / (double)n;
-    total += v;
+    colterm[i] = dbl ? colsum[i] : colsum[i] / (double)n;  // PHDE.cpp:40: columnSum /= n
+    total += colsum[i];                                     // PivotMDS.cpp:49
   }
-  DBuf<double> colterm_d;
-  PHDE_TRY(colterm_d.alloc(s));
-  PHDE_HIP(hipMemcpyAsync(colterm_d.p, colterm.data(), sizeof(double) * s, hipMemcpyHostToDevice,
-                          c->stream));
-  center_kernel<<<(int)std::min<int64_t>((n * s + 255) / 256, 16384), 256, 0, c->stream>>>(
-      D.p, n, s, colterm_d.p, rowterm.p, total, dbl, C.p);
-  PHDE_HIP(hipGetLastError());
+  PHDE_HIP(hipMemcpyAsync(colterm_d.p, colterm.data(), sizeof(double) * s, hipMemcpyHostToDevice, c->stream));
+  const double rN = (double)1 / (double)n, rK = (double)1 / (double)s;
+  PHDE_TRY(mds_center(c, Dv, n, s, colterm_d.p, rowterm.p, rN, rK, total, dbl, C.p));
   out.t_orth += ms_since(t1);
   auto t2 = clk::now();
-  Lap lap;
-  PHDE_TRY(lap.setup(c, n, s, false));
-  PHDE_TRY(lap.gram_self(C.p));
+  DBuf<double> K;
+  PHDE_TRY(K.alloc((size_t)s * s));
+  hipEvent_t ev;
+  c->timer.begin(c->stream, PHDE_KERNEL_GRAM, &ev);
+  PHDE_TRY(mds_ctc(c, C.p, n, s, K.p));
+  c->timer.end(c->stream, PHDE_KERNEL_GRAM, ev);
   PHDE_HIP(hipStreamSynchronize(c->stream));
   out.t_mmult += ms_since(t2);
   out.nacc = s;
-  return finish_layout(c, o, C.p, false, s, lap.SLS, nullptr, true, coords, out);
+  return finish_layout(c, o, C.p, false, s, K.p, nullptr, true, coords, out, false, true);
 }
 
 }  // namespace phde
@@ -1121,6 +1132,7 @@ int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats
     stats->hall_energy = out.energy;
     stats->hall_energy_kcenters = e_kc;
     stats->fallback = fallback;
+    stats->n_examined = out.nexam;
     for (int k = 0; k < 16; k++) {
       stats->kernel_ms[k] = c->timer.ms[k];
       stats->kernel_launches[k] = c->timer.launches[k];

@@ -130,7 +130,8 @@ __device__ void bfs_init(const BfsArgs &a) {
     d0.edges_to_check = a.g.m;
     a.dir[0] = d0;
     st_dev(&a.kc->key, 0ull);
-    st_dev(&a.kc->sumsq, 0ull);
+    st_dev(&a.kc->sumsq[0], 0ull);
+    st_dev(&a.kc->sumsq[1], 0ull);
     st_dev(&a.kc->unreached, 0ull);
     st_dev(reinterpret_cast<unsigned long long *>(&a.kc->depth), 0ull);
     *a.done_dev = -1;
@@ -147,14 +148,15 @@ __device__ void bfs_epilogue(const BfsArgs &a, bool want_depth = false) {
   const int64_t gtid = work_block(a) * blockDim.x + threadIdx.x;
   const int64_t gsize = work_grid(a) * blockDim.x;
   const int lane = __lane_id();
-  unsigned long long key = 0, sumsq = 0, unreached = 0;
+  unsigned long long key = 0, unreached = 0;
+  U128 sumsq{0, 0};
   int32_t maxd = 0;
   for (int64_t i = gtid; i < n; i += gsize) {
     const int32_t d = ld_dev(&a.dist[i]);
     if (d == kUnvisited) {
       unreached++;
     } else {
-      sumsq += (unsigned long long)d * (unsigned long long)d;
+      u128_add(sumsq, (unsigned long long)d * (unsigned long long)d);
       maxd = d > maxd ? d : maxd;
     }
     if (a.min_dist) {
@@ -173,13 +175,13 @@ __device__ void bfs_epilogue(const BfsArgs &a, bool want_depth = false) {
   for (int o = 32; o > 0; o >>= 1) {
     const unsigned long long ok = __shfl_xor(key, o);
     key = ok > key ? ok : key;
-    sumsq += __shfl_xor(sumsq, o);
     unreached += __shfl_xor(unreached, o);
   }
+  sumsq = wave_sum_u128(sumsq);
   if (want_depth) maxd = wave_max_i32(maxd);
   if (lane == 0) {
     if (a.min_dist && key) atomicMax(&a.kc->key, key);
-    if (sumsq) atomicAdd(&a.kc->sumsq, sumsq);
+    atomic_add_u128(a.kc->sumsq, sumsq);
     if (unreached) atomicAdd(&a.kc->unreached, unreached);
     if (want_depth && maxd > 0) atomicMax(reinterpret_cast<unsigned long long *>(&a.kc->depth), (unsigned long long)maxd);
   }

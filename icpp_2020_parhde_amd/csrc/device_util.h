@@ -19,6 +19,37 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
   return v;
 }
 
+// Exact sum of squared distances as a 128-bit integer (lo, hi words).  A
+// weighted distance reaches 2^30 - 2, its square 2^60, so a few dozen
+// squares already pass 2^64; path-like unweighted graphs pass it above
+// n = 3.8M.  Each word is added with its carry, so the total is exact at
+// any size; the host rounds it to a double once (u128_to_double).
+struct U128 {
+  unsigned long long lo, hi;
+};
+__device__ __forceinline__ void u128_add(U128 &a, unsigned long long v) {
+  a.lo += v;
+  a.hi += a.lo < v ? 1ull : 0ull;
+}
+__device__ __forceinline__ U128 wave_sum_u128(U128 a) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long lo = __shfl_xor(a.lo, o), hi = __shfl_xor(a.hi, o);
+    a.hi += hi;
+    u128_add(a, lo);
+  }
+  return a;
+}
+// acc[0] += v.lo, acc[1] += v.hi + carry: the carry out of the low word is
+// known from the value the atomic returns.
+__device__ __forceinline__ void atomic_add_u128(unsigned long long *acc, U128 v) {
+  if (v.lo) {
+    const unsigned long long old = atomicAdd(&acc[0], v.lo);
+    if (old + v.lo < old) v.hi++;
+  }
+  if (v.hi) atomicAdd(&acc[1], v.hi);
+}
+
 __device__ __forceinline__ int wave_max_i32(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));

@@ -131,7 +131,7 @@ DrawingBase::DrawingBase(unsigned int *rowPtrs, unsigned int *adj, int32_t *weig
   phde_default_options(&opts);
   opts.s = numSources;
   opts.num_coord = numCoord;
-  // doRand as spectralDrawing.cpp:379-391 maps -p (0 RandCoarse, 1 RandFine,
+  // doRand as spectralDrawing.cpp:88-95 maps -p (0 RandCoarse, 1 RandFine,
   // 2 Kcenters); 3 selects the learned proposal (setPivotNet)
   opts.strategy = doRand == 3   ? PHDE_PIVOTS_LEARNED
                   : doRand == 2 ? PHDE_PIVOTS_KCENTERS
@@ -149,7 +149,7 @@ DrawingBase::~DrawingBase() {
 }
 
 void DrawingBase::compute_eigens(double *matrix, int dimension, double *eigVecs, EigenChoice c) {
-  // DrawingBase.cpp:107-176 (non-MKL): power method, then the first or last
+  // DrawingBase.cpp:39-108 (non-MKL): power method, then the first or last
   // numCoord columns.
   std::vector<double> w(dimension);
   if (phde_power_eig(ctx, matrix, w.data(), dimension, nullptr) != PHDE_OK) {
@@ -165,7 +165,7 @@ void DrawingBase::compute_eigens(double *matrix, int dimension, double *eigVecs,
 void DrawingBase::DistanceAlgorithm(unsigned int *row, unsigned int *col, long source, long numNodes,
                                     long numEdges, double *dists, int32_t *wts, int32_t delta,
                                     int alpha, int beta) {
-  (void)beta;  // DrawingBase.cpp:182 always passes beta = 18
+  (void)beta;  // DrawingBase.cpp:114 always passes beta = 18
   if (wts == nullptr)
     DOBFS(row, col, source, numNodes, numEdges, dists, alpha, 18);
   else
@@ -182,7 +182,7 @@ void DrawingBase::seqBfs(unsigned int start, double *distCol) {
 
 void DrawingBase::distancePhaseComputation(double **distMatrix, double *eltDistance, int trial,
                                            int alpha, int beta, int32_t delta) {
-  // k-centers traversals (DrawingBase.cpp:193-240) through the library's
+  // k-centers traversals (DrawingBase.cpp:125-172) through the library's
   // multi-column distance call; the device computes columns one source at a
   // time, the sources chosen exactly as the reference does.
   auto t0 = std::chrono::high_resolution_clock::now();
@@ -213,7 +213,7 @@ void DrawingBase::distancePhaseComputation(double **distMatrix, double *eltDista
 }
 
 double *DrawingBase::getCoordinates() {
-  // DrawingBase.cpp:264-273: normalise every column in place, on each call.
+  // DrawingBase.cpp:196-205: normalise every column in place, on each call.
   for (int i = 0; i < numCoord; i++) {
     const double nrm = 1 / parComp.normL2(n, coordinates + (size_t)i * n);
     for (long j = 0; j < n; j++) coordinates[j + (size_t)i * n] *= nrm;
@@ -222,7 +222,7 @@ double *DrawingBase::getCoordinates() {
 }
 
 void DrawingBase::computeCoordinates(double *mat) {
-  // DrawingBase.cpp:275-287.
+  // DrawingBase.cpp:207-219.
   for (long i = 0; i < n; i++)
     for (int k = 0; k < numCoord; k++) {
       double x = 0;

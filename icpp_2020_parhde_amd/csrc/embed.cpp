@@ -1,4 +1,4 @@
-// embed.cpp — the `embed` command, drop-in for spectralDrawing.cpp:1-203.
+// embed.cpp — the `embed` command, drop-in for spectralDrawing.cpp:1-202.
 //
 //   embed <csr> [-w x] [-c n] [-a algo] [-p strat] [-r s] [-h]
 //         [-f fp64|fp32] [-e power|jacobi|generalized] [-E]

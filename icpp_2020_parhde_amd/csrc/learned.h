@@ -46,7 +46,7 @@ constexpr int kSelectUnavailable = -100;
 
 // Exact per-column sums of squares of n*k int32 columns, and the count of
 // unreached entries (-1, or >= 2^30-1).  out: host, k + 1 entries.
-int column_sumsq(phde_ctx *c, const int32_t *D, int k, unsigned long long *out);
+int column_sumsq(phde_ctx *c, const int32_t *D, int k, double *sumsq, unsigned long long *unreached);
 
 void learned_release(phde_ctx *c);
 

@@ -1,6 +1,6 @@
 // learned.hip — learned pivot proposal.  No reference counterpart: the
 // reference picks pivots by k-centers or at random (ParHDE.cpp:118-172,
-// DrawingBase.cpp:193-262).  The k-centers path stays the judge and the
+// DrawingBase.cpp:119-194).  The k-centers path stays the judge and the
 // fallback (api.hip).
 //
 // Features   x_v = (log(1 + deg_v) / 4, d1_v / ecc1, d2_v / ecc2, 1), where
@@ -905,23 +905,26 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *ar
   }
 }
 
+// Per column: exact 128-bit sum of squares into out[2 col], out[2 col + 1];
+// unreached entries counted into out[2 k].
 __global__ void sumsq_cols_kernel(const int32_t *D, int64_t n, unsigned long long *out) {
   const int col = blockIdx.y;
   const int32_t *c = D + (int64_t)col * n;
-  unsigned long long s = 0, neg = 0;
+  U128 s{0, 0};
+  unsigned long long neg = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t d = c[i];
     if (d < 0 || d >= 0x3fffffff)  // -1 (BFS) or 2^30-1 (SSSP): unreached
       neg++;
     else
-      s += (unsigned long long)d * (unsigned long long)d;
+      u128_add(s, (unsigned long long)d * (unsigned long long)d);
   }
-  s = wave_sum_u64(s);
+  s = wave_sum_u128(s);
   neg = wave_sum_u64(neg);
   if (__lane_id() == 0) {
-    atomicAdd(&out[col], s);
-    if (neg) atomicAdd(&out[gridDim.y], neg);
+    atomic_add_u128(out + 2 * col, s);
+    if (neg) atomicAdd(&out[2 * gridDim.y], neg);
   }
 }
 
@@ -1117,16 +1120,19 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
   return PHDE_OK;
 }
 
-int column_sumsq(phde_ctx *c, const int32_t *D, int k, unsigned long long *out) {
+int column_sumsq(phde_ctx *c, const int32_t *D, int k, double *sumsq, unsigned long long *unreached) {
   DBuf<unsigned long long> acc;
-  PHDE_TRY_RC(acc.alloc(k + 1));
-  PHDE_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long) * (k + 1), c->stream));
+  PHDE_TRY_RC(acc.alloc(2 * (size_t)k + 1));
+  PHDE_HIP(hipMemsetAsync(acc.p, 0, sizeof(unsigned long long) * (2 * k + 1), c->stream));
   const int bx = std::max(1, std::min<int>((int)((c->g.n + 255) / 256), std::max(1, c->num_cus * 4 / k)));
   sumsq_cols_kernel<<<dim3(bx, k), 256, 0, c->stream>>>(D, c->g.n, acc.p);
   PHDE_HIP(hipGetLastError());
-  PHDE_HIP(hipMemcpyAsync(out, acc.p, sizeof(unsigned long long) * (k + 1), hipMemcpyDeviceToHost,
+  std::vector<unsigned long long> h(2 * (size_t)k + 1);
+  PHDE_HIP(hipMemcpyAsync(h.data(), acc.p, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost,
                           c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < k; i++) sumsq[i] = u128_to_double(&h[2 * (size_t)i]);
+  if (unreached) *unreached = h[2 * (size_t)k];
   return PHDE_OK;
 }
 

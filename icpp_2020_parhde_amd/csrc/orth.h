@@ -38,10 +38,16 @@ struct Orth {
   int read_norm(double *norm);  // synchronises
   int accept(int j);            // S[:, j] = a / ||a||, den_j
   // Classical GS of nc <= kCgsBlock candidates (cols[b] * scales[b]) in
-  // order, each as cgs_pass + norm + discard rule + accept would treat it,
-  // until s_target columns are accepted; taken[b]: candidate b accepted.
+  // order, each as cgs_pass + norm + discard rule + accept would treat it
+  // (FP64 storage: bit for bit; FP32 storage: CGS2, the projection applied
+  // twice, with the passes over the earlier columns shared by the block),
+  // until s_target columns are accepted; taken[b]: candidate b accepted,
+  // norms[b]: its norm after the projection.
   int cgs_block(const int32_t *const *cols, const double *scales, int nc, int s_target, int &accepted,
-                std::vector<bool> &taken);
+                std::vector<bool> &taken, std::vector<double> &norms);
+  // host out[k] = sum_i deg_i u0 S_ik for k < s, out[s] = sum_i deg_i u0^2
+  // (the D-inner products of the ones vector); synchronises
+  int ones_dots(double *out);
 };
 
 }  // namespace phde

@@ -387,6 +387,74 @@ __global__ __launch_bounds__(kOrthBlock) void cgs_update_tail_kernel(double *a, 
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// ---------------------------------------------------------------- blocked CGS2
+// FP32 storage path.  Each candidate is projected twice (CGS2) against the
+// ones vector and the accepted columns; for a block of candidates the two
+// passes over the m columns accepted before the block (and the ones
+// vector) are shared: A_b -= sum_k coef_bk S_k + coef_b,ones u0, twice,
+// with coefficients from the block's dot products of that pass.  Each
+// candidate is then projected twice against the columns accepted inside
+// the block before it, then normed and judged.  In exact arithmetic this is
+// the same projection as candidate-by-candidate CGS2 (the accepted columns
+// and the ones vector are D-orthogonal); the order of the FP64 sums differs.
+
+// A_b -= sum_{k < m} coef[b][k] S_k + coef[b][ones] u0, every candidate b.
+template <typename T>
+__global__ __launch_bounds__(kOrthBlock) void cgs_block_sub_kernel(double *A, const T *S, int64_t n, int m, int nb,
+                                                                  double u0, const double *coef) {
+  __shared__ double sc[kCgsBlock][kOnesSlot + 1];
+  for (int t = threadIdx.x; t < kCgsBlock * (kOnesSlot + 1); t += blockDim.x)
+    sc[t / (kOnesSlot + 1)][t % (kOnesSlot + 1)] = t / (kOnesSlot + 1) < nb ? coef[t] : 0.0;
+  __syncthreads();
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double acc[kCgsBlock];
+#pragma unroll
+    for (int b = 0; b < kCgsBlock; b++) acc[b] = sc[b][kOnesSlot] * u0;
+    for (int k = 0; k < m; k++) {
+      const double sv = (double)S[(int64_t)k * n + i];
+#pragma unroll
+      for (int b = 0; b < kCgsBlock; b++) acc[b] = fma(sc[b][k], sv, acc[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < kCgsBlock; b++)
+      if (b < nb) A[(int64_t)b * n + i] -= acc[b];
+  }
+}
+
+// coef[k] = num_k / den_k for the columns k in [k_lo, k_hi) (partials laid
+// out by column as cgs_dots_range_kernel writes them).
+__global__ void cgs_coef_range_kernel(const double *part, int nblk, int k_lo, const double *den, double *coef) {
+  __shared__ double sh[kOrthBlock / kWave];
+  const int k = k_lo + blockIdx.x;
+  double s = 0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(int64_t)k * nblk + b];
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) coef[k] = t * (1 / den[k]);
+}
+
+// a -= sum_{k in [k_lo, k_hi)} coef_k S_k, per-block sums of a^2.
+template <typename T>
+__global__ __launch_bounds__(kOrthBlock) void cgs_sub_range_kernel(double *a, const T *S, int64_t n, int k_lo,
+                                                                  int k_hi, const double *coef, double *part) {
+  __shared__ double sc[kCgsBlock];
+  __shared__ double sh[kOrthBlock / kWave];
+  const int kc = k_hi - k_lo;
+  for (int k = threadIdx.x; k < kc; k += blockDim.x) sc[k] = coef[k_lo + k];
+  __syncthreads();
+  double ss = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double acc = 0;
+    for (int k = 0; k < kc; k++) acc = fma(sc[k], (double)S[(int64_t)(k_lo + k) * n + i], acc);
+    const double v = a[i] - acc;
+    a[i] = v;
+    ss = fma(v, v, ss);
+  }
+  const double t = block_sum(ss, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // ---------------------------------------------------------------- host side
 
 static int grid_for(int64_t n, int cap) {
@@ -531,7 +599,7 @@ int Orth::accept(int j) { return fp32 ? accept_t<float>(*this, j) : accept_t<dou
 
 template <typename T>
 static int cgs_block_t(Orth &o, const int32_t *const *cols, const double *scales, int nc, int s_target,
-                       int &accepted, std::vector<bool> &taken) {
+                       int &accepted, std::vector<bool> &taken, std::vector<double> &norms) {
   hipStream_t st = o.ctx->stream;
   const T *S = (const T *)o.S;
   const int64_t n = o.n;
@@ -545,6 +613,7 @@ static int cgs_block_t(Orth &o, const int32_t *const *cols, const double *scales
   cgs_update_old_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(S, n, m, nc, o.blkCoef, o.blkAcc);
   PHDE_HIP(hipGetLastError());
   taken.assign(nc, false);
+  norms.assign(nc, 0.0);
   for (int b = 0; b < nc && accepted < s_target; b++) {
     const int ncols = accepted;
     double *pb = o.blkPart + (int64_t)b * (kOnesSlot + 1) * o.nblk;
@@ -559,6 +628,7 @@ static int cgs_block_t(Orth &o, const int32_t *const *cols, const double *scales
     PHDE_HIP(hipGetLastError());
     double nd = 0;
     PHDE_TRY_RC(o.read_norm(&nd));
+    norms[b] = nd;
     if (nd >= 0.001) {
       PHDE_TRY_RC(o.accept(accepted));
       accepted++;
@@ -568,8 +638,55 @@ static int cgs_block_t(Orth &o, const int32_t *const *cols, const double *scales
   return PHDE_OK;
 }
 
+template <typename T>
+static int cgs2_block_t(Orth &o, const int32_t *const *cols, const double *scales, int nc, int s_target,
+                        int &accepted, std::vector<bool> &taken, std::vector<double> &norms) {
+  hipStream_t st = o.ctx->stream;
+  T *S = (T *)o.S;
+  const int64_t n = o.n;
+  const int m = accepted;
+  for (int b = 0; b < nc; b++)
+    cand_from_dist_kernel<<<grid_for(n, 4096), kOrthBlock, 0, st>>>(cols[b], scales[b], o.blkA + (int64_t)b * n, n);
+  const int nchunk = std::max(1, (m + kCgsCols - 1) / kCgsCols);
+  for (int pass = 0; pass < 2; pass++) {
+    cgs_dots_multi_kernel<T><<<dim3(o.nblk, nchunk), kOrthBlock, 0, st>>>(o.blkA, o.deg_, S, n, m, nc, o.u0,
+                                                                          o.blkPart);
+    cgs_coef_multi_kernel<<<dim3(m + 1, nc), kOrthBlock, 0, st>>>(o.blkPart, o.nblk, m, o.den, o.blkCoef);
+    cgs_block_sub_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(o.blkA, S, n, m, nc, o.u0, o.blkCoef);
+  }
+  PHDE_HIP(hipGetLastError());
+  taken.assign(nc, false);
+  norms.assign(nc, 0.0);
+  for (int b = 0; b < nc && accepted < s_target; b++) {
+    double *ab = o.blkA + (int64_t)b * n;
+    if (accepted > m) {
+      for (int pass = 0; pass < 2; pass++) {
+        cgs_dots_range_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(ab, o.deg_, S, n, m, accepted, o.part);
+        cgs_coef_range_kernel<<<accepted - m, kOrthBlock, 0, st>>>(o.part, o.nblk, m, o.den, o.coef);
+        cgs_sub_range_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(ab, S, n, m, accepted, o.coef, o.part);
+      }
+    } else {
+      sumsq_kernel<<<o.nblk, kOrthBlock, 0, st>>>(ab, n, o.part);
+    }
+    norm_finalize_kernel<<<1, kOrthBlock, 0, st>>>(o.part, o.nblk, o.scal);
+    PHDE_HIP(hipGetLastError());
+    double nd = 0;
+    PHDE_TRY_RC(o.read_norm(&nd));
+    norms[b] = nd;
+    if (nd >= 0.001) {
+      T *Sj = S + (int64_t)accepted * n;
+      cgs_accept_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(ab, o.scal, o.deg_, Sj, n, o.part);
+      den_finalize_kernel<<<1, kOrthBlock, 0, st>>>(o.part, o.nblk, o.den + accepted);
+      PHDE_HIP(hipGetLastError());
+      accepted++;
+      taken[b] = true;
+    }
+  }
+  return PHDE_OK;
+}
+
 int Orth::cgs_block(const int32_t *const *cols, const double *scales, int nc, int s_target, int &accepted,
-                    std::vector<bool> &taken) {
+                    std::vector<bool> &taken, std::vector<double> &norms) {
   if (nc < 1 || nc > kCgsBlock) {
     set_error("internal: CGS block size");
     return PHDE_ERR_ARG;
@@ -580,9 +697,32 @@ int Orth::cgs_block(const int32_t *const *cols, const double *scales, int nc, in
     PHDE_HIP(hipMalloc(&blkPart, sizeof(double) * (size_t)nblk * (kOnesSlot + 1) * kCgsBlock));
     PHDE_HIP(hipMalloc(&blkCoef, sizeof(double) * (size_t)(kOnesSlot + 1) * kCgsBlock));
   }
-  return fp32 ? cgs_block_t<float>(*this, cols, scales, nc, s_target, accepted, taken)
-              : cgs_block_t<double>(*this, cols, scales, nc, s_target, accepted, taken);
+  // FP32 storage: CGS2 (two projections per candidate); FP64: one, bit for
+  // bit as the candidate-by-candidate kernels
+  return fp32 ? cgs2_block_t<float>(*this, cols, scales, nc, s_target, accepted, taken, norms)
+              : cgs_block_t<double>(*this, cols, scales, nc, s_target, accepted, taken, norms);
 }
+
+template <typename T>
+static int ones_dots_t(Orth &o, double *out) {
+  hipStream_t st = o.ctx->stream;
+  PHDE_TRY_RC(fill_f64(o.ctx, o.a, o.n, o.u0));
+  const int nchunk = std::max(1, (o.s + kColChunk - 1) / kColChunk);
+  cgs_dots_kernel<T><<<dim3(o.nblk, nchunk), kOrthBlock, 0, st>>>(o.a, o.deg_, (const T *)o.S, o.n, o.s, o.u0,
+                                                                  o.part);
+  PHDE_HIP(hipGetLastError());
+  std::vector<double> h((size_t)o.nblk * (o.s + 1));
+  PHDE_HIP(hipMemcpyAsync(h.data(), o.part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  PHDE_HIP(hipStreamSynchronize(st));
+  for (int k = 0; k <= o.s; k++) {
+    double v = 0;
+    for (int b = 0; b < o.nblk; b++) v += h[(size_t)k * o.nblk + b];
+    out[k] = v;
+  }
+  return PHDE_OK;
+}
+
+int Orth::ones_dots(double *out) { return fp32 ? ones_dots_t<float>(*this, out) : ones_dots_t<double>(*this, out); }
 
 
 }  // namespace phde

@@ -35,10 +35,17 @@ struct BfsSlot {
   unsigned int pad[5];
 };
 
+// The exact 128-bit sum of squares (low, high word) rounded once to a
+// double; equal to the reference's sequential double sum of squares
+// (ParHDE.cpp:176-185) whenever that sum stays below 2^53.
+inline double u128_to_double(const unsigned long long w[2]) {
+  return (double)(((unsigned __int128)w[1] << 64) | (unsigned __int128)w[0]);
+}
+
 // Result of the fused k-centers epilogue of one traversal.
 struct KcResult {
   unsigned long long key;        // (min_dist << 32) | (0xffffffff - vertex), maximised
-  unsigned long long sumsq;      // sum of dist^2 (exact)
+  unsigned long long sumsq[2];   // sum of dist^2, exact 128-bit integer (low, high word)
   unsigned long long unreached;  // vertices left at kUnvisited
   long long depth;               // eccentricity of the source (BFS levels)
 };

@@ -5,11 +5,11 @@
 //  - L*S (ParHDE.cpp:288-300, WParHDE.cpp:229-241): LS_ik = deg_i S_ik minus
 //    sum over neighbours j of (w_ij) S_jk, subtracted in adjacency order
 //    (rows longer than kHubRow: chunk by chunk, see below).
-//  - S^T(LS) (DrawingBase.cpp:289-312 matrixTransMultiplication):
+//  - S^T(LS) (DrawingBase.cpp:221-244 matrixTransMultiplication):
 //    C[k + i*s] = sum_j S_ji LS_jk, the transposed storage the eigensolver
 //    consumes.
-//  - computeCoordinates (DrawingBase.cpp:275-287) and getCoordinates'
-//    column normalisation (DrawingBase.cpp:264-273).
+//  - computeCoordinates (DrawingBase.cpp:207-219) and getCoordinates'
+//    column normalisation (DrawingBase.cpp:196-205).
 //
 // Layout: the SpMM gathers whole neighbour rows, so it works on a row-major
 // copy of S (n x s, row i contiguous): one wave per vertex, lane k holds
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(kBlk) void coords_kernel(const T *S, const double *
   }
 }
 
-// inv[k] = 1 / sqrt(sum_b part[k][b]) (getCoordinates, DrawingBase.cpp:268).
+// inv[k] = 1 / sqrt(sum_b part[k][b]) (getCoordinates, DrawingBase.cpp:200).
 __global__ void coords_norm_kernel(const double *part, int nblk, int c, double *inv) {
   const int k = threadIdx.x;
   if (k >= c) return;

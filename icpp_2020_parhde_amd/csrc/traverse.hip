@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kTBlock) void sssp_init_kernel(SsspArgs a) {
     s0.bound = a.delta;
     a.st[0] = s0;
     a.kc->key = 0;
-    a.kc->sumsq = 0;
+    a.kc->sumsq[0] = a.kc->sumsq[1] = 0;
     a.kc->unreached = 0;
     a.kc->depth = 0;
     *a.done_dev = -1;
@@ -128,13 +128,14 @@ __device__ void sssp_epilogue(const SsspArgs &a) {
   const int64_t n = a.g.n;
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
-  unsigned long long key = 0, sumsq = 0, unreached = 0;
+  unsigned long long key = 0, unreached = 0;
+  U128 sumsq{0, 0};
   for (int64_t i = gtid; i < n; i += gsize) {
     const int32_t d = a.dist[i];
     if (d >= kDistInf)
       unreached++;
     else
-      sumsq += (unsigned long long)d * (unsigned long long)d;
+      u128_add(sumsq, (unsigned long long)d * (unsigned long long)d);
     if (a.min_dist) {
       int32_t md = a.min_dist[i];
       if (d < md) {
@@ -151,12 +152,12 @@ __device__ void sssp_epilogue(const SsspArgs &a) {
   for (int o = 32; o > 0; o >>= 1) {
     const unsigned long long ok = __shfl_xor(key, o);
     key = ok > key ? ok : key;
-    sumsq += __shfl_xor(sumsq, o);
     unreached += __shfl_xor(unreached, o);
   }
+  sumsq = wave_sum_u128(sumsq);
   if (__lane_id() == 0) {
     if (a.min_dist && key) atomicMax(&a.kc->key, key);
-    if (sumsq) atomicAdd(&a.kc->sumsq, sumsq);
+    atomic_add_u128(a.kc->sumsq, sumsq);
     if (unreached) atomicAdd(&a.kc->unreached, unreached);
   }
 }

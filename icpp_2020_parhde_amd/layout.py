@@ -73,17 +73,22 @@ class LayoutStats:
     hall_energy_kcenters: float = 0.0
     fallback: int = 0
     n_candidates: int = 0
+    normdist: list = field(default_factory=list)     # ||a|| of each examined candidate after projection
+    eigenvalues: list = field(default_factory=list)  # the s eigenvalue estimates, solver order
+    dorth_err: float = 0.0                           # with diagnostics=1: D-orthogonality error of [1 S]
 
     @classmethod
-    def from_c(cls, s: PhdeStats) -> "LayoutStats":
+    def from_c(cls, s: PhdeStats, dim: int = 0) -> "LayoutStats":
         nt = int(s.n_traversals)
+        ne = min(int(s.n_examined), 192)
         return cls(s.ms_total, s.ms_distance, s.ms_orth, s.ms_mmult, s.ms_eig, s.ms_coords, nt,
                    int(s.n_accepted), int(s.n_discarded), int(s.eig_iters), int(s.levels_total),
                    [int(s.pivots[i]) for i in range(min(nt, 192))],
                    {k: float(s.kernel_ms[i]) for i, k in enumerate(KERNEL_CLASSES)},
                    {k: int(s.kernel_launches[i]) for i, k in enumerate(KERNEL_CLASSES)},
                    float(s.ms_select), float(s.hall_energy), float(s.hall_energy_kcenters),
-                   int(s.fallback), int(s.n_candidates))
+                   int(s.fallback), int(s.n_candidates), [float(s.normdist[i]) for i in range(ne)],
+                   [float(s.eigenvalues[i]) for i in range(min(dim, 64))], float(s.dorth_err))
 
 
 def default_options(**kw) -> PhdeOptions:
@@ -174,7 +179,7 @@ class Context:
         st = PhdeStats()
         _check(self.lib.phde_layout(self.h, ctypes.byref(o), as_ptr(X, ctypes.c_double), ctypes.byref(st)),
                "phde_layout")
-        return X.reshape(o.num_coord, self.n).T, LayoutStats.from_c(st)
+        return X.reshape(o.num_coord, self.n).T, LayoutStats.from_c(st, o.s)
 
     def power_eig(self, K: np.ndarray):
         """Reference power method on one s x s matrix (column-major semantics)."""
@@ -346,7 +351,7 @@ class DrawingBase:
         return 0
 
     def getCoordinates(self) -> np.ndarray:
-        # DrawingBase.cpp:264-273: columns normalised in place on every call
+        # DrawingBase.cpp:196-205: columns normalised in place on every call
         self.coordinates = self.coordinates / np.linalg.norm(self.coordinates, axis=0)
         return self.coordinates
 

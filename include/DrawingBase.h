@@ -41,18 +41,18 @@ class DrawingBase {
   phde_options opts;
   phde_stats lastStats;
 
-  // DrawingBase.cpp:289-312: C[k + i*m] = sum_j A[j + i*k] B[j + k*k]
+  // DrawingBase.cpp:221-244: C[k + i*m] = sum_j A[j + i*k] B[j + k*k]
   void matrixTransMultiplication(double *A, double *B, double *C, long m, long k, long n);
-  // DrawingBase.cpp:187-262: numSources k-centers (or random) traversals
+  // DrawingBase.cpp:119-194: numSources k-centers (or random) traversals
   // into *distMatrix (n x numSources, column-major, levels as doubles).
   void distancePhaseComputation(double **, double *, int, int, int, int32_t);
-  // DrawingBase.cpp:178-185: BFS when weights == nullptr, else SSSP.
+  // DrawingBase.cpp:110-117: BFS when weights == nullptr, else SSSP.
   void DistanceAlgorithm(unsigned int *row, unsigned int *col, long source, long numNodes,
                          long numEdges, double *dists, int32_t *weights = nullptr,
                          int32_t delta = 1, int alpha = 15, int beta = 18);
-  // DrawingBase.cpp:275-287: coordinates = mat * eigenVectors.
+  // DrawingBase.cpp:207-219: coordinates = mat * eigenVectors.
   void computeCoordinates(double *);
-  // DrawingBase.cpp:314-340: levels from `start` into distCol.
+  // DrawingBase.cpp:246-271: levels from `start` into distCol.
   void seqBfs(unsigned int start, double *distCol);
   // Runs `tries` layouts of algorithm `algo` on the GPU and prints the
   // reference's mean-time lines.

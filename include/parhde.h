@@ -44,9 +44,9 @@ typedef struct phde_ctx phde_ctx;
 /* Learned pivot proposal network (learned-pivot strategy). */
 typedef struct phde_mpnn phde_mpnn;
 
-/* Algorithms (spectralDrawing.cpp:392-406 `-a`). */
+/* Algorithms (spectralDrawing.cpp:101-115 `-a`, :177-186). */
 enum { PHDE_ALGO_PIVOTMDS = 0, PHDE_ALGO_PHDE = 1, PHDE_ALGO_WPARHDE = 2, PHDE_ALGO_PARHDE = 3 };
-/* Pivot strategies (spectralDrawing.cpp:379-391 `-p`). */
+/* Pivot strategies (spectralDrawing.cpp:88-95 `-p`). */
 enum { PHDE_PIVOTS_RANDCOARSE = 0, PHDE_PIVOTS_RANDFINE = 1, PHDE_PIVOTS_KCENTERS = 2, PHDE_PIVOTS_LEARNED = 3 };
 /* Arithmetic of the D-orthogonalisation / SpMM stages. */
 enum { PHDE_PREC_FP64 = 0, PHDE_PREC_FP32 = 1 };
@@ -72,7 +72,7 @@ typedef struct phde_options {
   double judge_tol;    /* with judge: keep k-centers when E_learned > (1 + judge_tol) E_kcenters; default 0.02 */
   double nms_rho;      /* suppression radius r = nms_rho * ecc / sqrt(s) hops; <= 0 picks 1.0 */
   int32_t judge;       /* 1: also run the k-centers layout and keep the better one (see judge_tol) */
-  int32_t reserved;
+  int32_t diagnostics; /* 1: also measure the D-orthogonality error of S (phde_stats.dorth_err) */
 } phde_options;
 
 typedef struct phde_stats {
@@ -96,6 +96,15 @@ typedef struct phde_stats {
   double hall_energy_kcenters; /* of the k-centers layout when judged, else 0 */
   int32_t fallback;            /* 1 when the k-centers layout was returned instead */
   int32_t n_candidates;        /* pivots proposed by the selection */
+  /* numerical diagnostics (ParHDE / WParHDE) */
+  int32_t n_examined;          /* candidates taken through the orthogonalisation (<= 192 recorded) */
+  int32_t reserved;
+  double normdist[192];        /* per examined candidate, in order: ||a||_2 after the projection, the
+                                  value of the discard test ||a|| < 0.001 (ParHDE.cpp:267-283) */
+  double eigenvalues[64];      /* the s eigenvalue estimates of the s*s eigensolve, in the solver's
+                                  output order (power method: after its reversal, ParallelComponents.cpp:132-140) */
+  double dorth_err;            /* with options.diagnostics: max_ij |B_ij / sqrt(B_ii B_jj) - delta_ij| of
+                                  B = [1 S]^T D [1 S] (D = degrees); 0 otherwise */
 } phde_stats;
 
 enum {
@@ -121,7 +130,7 @@ int phde_device_count(int *count);
 
 /* Create a context holding the graph in HBM.  Replaces the borrowed graph
  * arrays of the DrawingBase constructor (variants/DrawingBase.h:41,
- * DrawingBase.cpp:84-94); the host arrays may be freed afterwards. */
+ * DrawingBase.cpp:16-37); the host arrays may be freed afterwards. */
 int phde_create(const uint32_t *rowptr, const uint32_t *adj, const int32_t *weights, int64_t n,
                 int64_t m, phde_ctx **out);
 /* Same, from device-resident arrays (copied device-to-device). */
@@ -154,7 +163,7 @@ int phde_msbfs(phde_ctx *ctx, const int64_t *sources, int k, int32_t *dist);
 
 /* One complete layout: DrawingBase::run + getCoordinates (ParHDE.cpp:45-352,
  * WParHDE.cpp:40-291, PivotMDS.cpp:68-147, PHDE.cpp:50-136,
- * DrawingBase.cpp:264-273).  coords: n*num_coord column-major, each column
+ * DrawingBase.cpp:196-205).  coords: n*num_coord column-major, each column
  * of unit L2 norm.  stats optional. */
 int phde_layout(phde_ctx *ctx, const phde_options *opt, double *coords, phde_stats *stats);
 

@@ -103,7 +103,7 @@ double par_sum(int64_t n, F term) {
 }
 
 // k-centers step shared by ParHDE.cpp:124-140, WParHDE.cpp:117-135 and
-// DrawingBase.cpp:210-224: min_dists = min(min_dists, col), then the first
+// DrawingBase.cpp:142-156: min_dists = min(min_dists, col), then the first
 // (lowest index) maximiser.
 void kcenter_argmax(int64_t n, const double *col, double *min_dists, double *val, int64_t *idx) {
   const int T = omp_get_max_threads();
@@ -148,7 +148,7 @@ int64_t kcenter_dedupe(int64_t n, const double *min_dists, double maxval, int64_
 }
 
 void distance_column(const or_graph *g, int64_t src, double *col, int32_t delta) {
-  // DrawingBase::DistanceAlgorithm (DrawingBase.cpp:178-185): BFS when
+  // DrawingBase::DistanceAlgorithm (DrawingBase.cpp:110-117): BFS when
   // unweighted, delta-stepping otherwise; beta is always 18.
   if (g->w == nullptr)
     or_dobfs(g, src, col, 15, 18);
@@ -156,7 +156,7 @@ void distance_column(const or_graph *g, int64_t src, double *col, int32_t delta)
     or_deltastep(g, src, delta, col);
 }
 
-// matrixTransMultiplication (DrawingBase.cpp:289-312): C[k + i*mdim] =
+// matrixTransMultiplication (DrawingBase.cpp:221-244): C[k + i*mdim] =
 // sum_j A[j + i*kdim] * B[j + k*kdim], j ascending.
 void mat_trans_mult(const double *A, const double *B, double *C, int64_t mdim, int64_t kdim,
                     int64_t ndim) {
@@ -169,7 +169,7 @@ void mat_trans_mult(const double *A, const double *B, double *C, int64_t mdim, i
     }
 }
 
-// computeCoordinates (DrawingBase.cpp:275-287).
+// computeCoordinates (DrawingBase.cpp:207-219).
 void compute_coords(int64_t n, int s, int c, const double *mat, const double *eig, double *X) {
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; i++)
@@ -180,7 +180,7 @@ void compute_coords(int64_t n, int s, int c, const double *mat, const double *ei
     }
 }
 
-// getCoordinates (DrawingBase.cpp:264-273).
+// getCoordinates (DrawingBase.cpp:196-205).
 void normalise_coords(int64_t n, int c, double *X) {
   for (int k = 0; k < c; k++) {
     const double nrm = 1 / norm_seq(n, X + k * n);
@@ -286,7 +286,7 @@ extern "C" {
 int or_num_threads(void) { return omp_get_max_threads(); }
 
 void or_seq_bfs(const or_graph *g, int64_t src, int32_t *dist) {
-  // DrawingBase::seqBfs (DrawingBase.cpp:314-340) with int levels.
+  // DrawingBase::seqBfs (DrawingBase.cpp:246-271) with int levels.
   const int64_t n = g->n;
   for (int64_t i = 0; i < n; i++) dist[i] = -1;
   std::vector<uint32_t> q(n);
@@ -551,7 +551,7 @@ int or_power_eig(double *K, double *eValues, int64_t n, int32_t *iters) {
 int or_compute_eigens(double *K, int64_t dim, int numCoord, int which, double *eigvec,
                       double *eigval, int32_t *iters) {
   // compute_eigens + mkl_eigen_value_vector_comp_{first,largest}
-  // (DrawingBase.cpp:107-176), non-MKL branch.
+  // (DrawingBase.cpp:39-108), non-MKL branch.
   std::vector<double> w(dim);
   if (or_power_eig(K, w.data(), dim, iters) != 0) return OR_ERR_EIG;
   for (int64_t j = 0; j < numCoord; j++)
@@ -779,7 +779,7 @@ int or_wparhde(const or_graph *g, int s, int numCoord, int32_t delta, or_result 
 }
 
 namespace {
-// distancePhaseComputation (DrawingBase.cpp:187-262), k-centers branch:
+// distancePhaseComputation (DrawingBase.cpp:119-194), k-centers branch:
 // exactly s traversals, columns kept as raw distances.
 void distance_phase(const or_graph *g, int s, double *D, or_result *r) {
   const int64_t n = g->n;

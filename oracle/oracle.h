@@ -45,7 +45,7 @@ typedef struct or_result {
 
 enum { OR_OK = 0, OR_ERR_ARG = -1, OR_ERR_PIVOTS = -2, OR_ERR_EIG = -3, OR_ERR_WEIGHTS = -4 };
 
-/* Textbook queue BFS (DrawingBase.cpp:314-340 shape).  dist[v] = level, -1 if unreached. */
+/* Textbook queue BFS (DrawingBase.cpp:246-271 shape).  dist[v] = level, -1 if unreached. */
 void or_seq_bfs(const or_graph *g, int64_t src, int32_t *dist);
 /* Direction-optimising BFS (gapbs/bfs.cc:134-185), levels written as
  * doubles to reached vertices only. */
@@ -60,7 +60,7 @@ void or_dijkstra(const or_graph *g, int64_t src, int64_t *dist);
  * when the iteration produced NaN/Inf.  iters (optional) gets the count. */
 int or_power_eig(double *K, double *eValues, int64_t n, int32_t *iters);
 
-/* The reference eigen wrappers (DrawingBase.cpp:119-176): which = 0 First,
+/* The reference eigen wrappers (DrawingBase.cpp:51-108): which = 0 First,
  * 1 Largest; eigvec is dim*numCoord. */
 int or_compute_eigens(double *K, int64_t dim, int numCoord, int which, double *eigvec,
                       double *eigval, int32_t *iters);

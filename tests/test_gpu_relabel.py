@@ -122,12 +122,7 @@ def test_wparhde_pivotmds_phde(gpu):
         X, st = ctx.layout(s=10, num_coord=3, algorithm=L.ALGO_WPARHDE, delta=0)
     assert st.pivots == ref["pivots"]
     assert aligned_rel_err(X, ref["coords"]) < FP64_TOL
-    # PivotMDS / PHDE: the reference's power method stops at |<e_i,t_i>| >=
-    # 0.999 and their s x s matrices have a near-null direction, so their
-    # coordinates move with the rounding of C^T C (the oracle itself, run on
-    # the renumbered graph, differs by up to ~30% on these graphs).  The
-    # pivots are exact; the coordinates are compared through the Jacobi
-    # eigensolver against the same context without renumbering.
+    # PivotMDS / PHDE through the Jacobi eigensolver: renumbered and not agree
     h = GRAPHS["holed"]()
     for algo, fn in ((L.ALGO_PIVOTMDS, O.pivotmds), (L.ALGO_PHDE, O.phde)):
         ref = fn(h, 12, 2)
@@ -137,6 +132,44 @@ def test_wparhde_pivotmds_phde(gpu):
                 out[r] = ctx.layout(s=12, num_coord=2, algorithm=algo, eig_mode=L.EIG_JACOBI)
         assert out[True][1].pivots == out[False][1].pivots == ref["pivots"], algo
         assert aligned_rel_err(out[True][0], out[False][0]) < FP64_TOL, algo
+
+
+@pytest.mark.parametrize("algo", [L.ALGO_PIVOTMDS, L.ALGO_PHDE])
+def test_mds_power_method_renumbered(gpu, algo):
+    """The reference's power method stops at |<e_i,t_i>| >= 0.999 and C^T C
+    has a near-null direction, so PivotMDS / PHDE coordinates follow the
+    rounding of C^T C: the library forms it in the caller's row order with
+    the reference's sequential sums, and matches the oracle on renumbered
+    graphs (a forced one and one the context renumbers by its own rule)."""
+    fn = O.pivotmds if algo == L.ALGO_PIVOTMDS else O.phde
+    cases = [(GRAPHS["holed"](), True, 16), (Graph.rgg(100000, 10.0, seed=31), None, 20)]
+    for g, force, s in cases:
+        ref = fn(g, s, 2)
+        assert ref["rc"] == 0
+        if force:
+            with context(g, True) as ctx:
+                X, st = ctx.layout(s=s, num_coord=2, algorithm=algo)
+        else:
+            with L.Context(g) as ctx:
+                assert g.n >= 65536 and ctx.vertex_order()[1]  # renumbered by the context's rule
+                X, st = ctx.layout(s=s, num_coord=2, algorithm=algo)
+        assert st.pivots == ref["pivots"], algo
+        assert st.eig_iters == ref["eig_iters"], (algo, st.eig_iters, ref["eig_iters"])
+        err = aligned_rel_err(X, ref["coords"])
+        print(f"algo {algo} n={g.n} s={s}: coordinate rel err {err:.2e}")
+        assert err < FP64_TOL, (algo, g.n, err)
+
+
+def test_mds_weighted_large_distances(gpu):
+    """Weighted PivotMDS whose squared distances pass 2^53: the column sums
+    of squares round, and the library then sums them in the reference's
+    sequential order."""
+    g = Graph.grid3d(12, 11, 10, seed=8, wmax=1 << 22)
+    ref = O.pivotmds(g, 10, 2)
+    with context(g, True) as ctx:
+        X, st = ctx.layout(s=10, num_coord=2, algorithm=L.ALGO_PIVOTMDS)
+    assert st.pivots == ref["pivots"]
+    assert aligned_rel_err(X, ref["coords"]) < FP64_TOL
 
 
 @pytest.mark.parametrize("strategy", [L.PIVOTS_RANDFINE, L.PIVOTS_RANDCOARSE])

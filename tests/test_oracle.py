@@ -287,3 +287,49 @@ def test_lcc_matches_scipy():
     sub = csr_matrix(g)[keep][:, keep]
     assert np.array_equal(h.rowptr.astype(np.int64), sub.indptr)
     assert np.array_equal(h.adj.astype(np.int64), sub.indices)
+
+
+def _mds_sequential(g, pivots, dbl):
+    """The reference's PivotMDS / PHDE arithmetic with every product and sum
+    rounded on its own and every sum a sequential chain (np.cumsum runs in
+    index order): doubleCentering (PivotMDS.cpp:30-66) or columnCentering
+    (PHDE.cpp:30-47), then C^T C with j ascending (DrawingBase.cpp:221-244)."""
+    D = np.stack([O.seq_bfs(g, p) if g.w is None else O.dijkstra(g, p) for p in pivots],
+                 axis=1).astype(np.float64)
+    n, s = D.shape
+    if dbl:
+        sq = D * D
+        colsum = np.array([np.cumsum(sq[:, i])[-1] for i in range(s)])
+        rowsum = np.zeros(n)
+        for i in range(s):  # row j accumulates column by column
+            rowsum = rowsum + sq[:, i]
+        total = 0.0
+        for i in range(s):
+            total += colsum[i]
+        rN, rK = 1.0 / n, 1.0 / s
+        C = -0.5 * (((sq - rN * colsum[None, :]) - (rK * rowsum)[:, None]) + (rK * rN) * total)
+    else:
+        C = D - np.array([np.cumsum(D[:, i])[-1] for i in range(s)])[None, :] / n
+    K = np.empty((s, s))
+    for i in range(s):
+        for k in range(s):
+            K[i, k] = np.cumsum(C[:, i] * C[:, k])[-1]
+    return D, C, K
+
+
+@pytest.mark.parametrize("algo", ["pivotmds", "phde"])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_mds_matrix_bit_exact(algo, weighted):
+    """The s x s matrix the power method sees is bit-exact against the
+    sequential restatement, also where squared distances pass 2^53 and the
+    column sums round (weights up to 2^22)."""
+    g = Graph.grid3d(12, 11, 10, seed=8, wmax=1 << 22) if weighted else Graph.holed_grid(90, 80, 6, 12)
+    s = 10 if weighted else 16
+    fn = O.pivotmds if algo == "pivotmds" else O.phde
+    r = fn(g, s, 2, want_S=True)
+    assert r["rc"] == 0
+    D, C, K = _mds_sequential(g, r["pivots"], algo == "pivotmds")
+    assert np.array_equal(r["S"], C)
+    assert np.array_equal(r["SLS"].reshape(s, s), K)
+    if weighted:  # the case where the order of the roundings matters
+        assert float(np.max(np.sum(D * D, axis=0))) > 2.0 ** 53
