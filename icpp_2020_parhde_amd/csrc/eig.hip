@@ -18,6 +18,7 @@
 // Cholesky reduction B = L L^T of a generalised problem.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "phde_common.h"
 #include "eig.h"
@@ -461,4 +462,87 @@ void glibc_rand_sequence(unsigned seed, int count, int32_t *out) {
   for (int k = 0; k < count; k++) out[k] = (int32_t)(r[k + 344] >> 1);
 }
 
+// ---------------------------------------------------------------- no allocation
+// phde_power_eig_noalloc runs the power method above without allocating:
+// its scratch — the dim*dim start values and, when the power method's
+// matrices do not all fit in LDS, its global T (dim*dim) — is this
+// file-scope device array (64 KiB), which every device the library runs on
+// holds until the process ends.  Host code never passes `scratch` itself
+// (in host code the name is a host-side stand-in, not a device address):
+// each device's address comes from hipGetSymbolAddress, once per device.
+constexpr int kNoAllocMaxDim = 64;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxDim * kNoAllocMaxDim];
+
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices] = {};
+
+static int scratch_address(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("phde_power_eig_noalloc: device index out of range");
+    return PHDE_ERR_DEVICE;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    int cur = 0;
+    PHDE_HIP(hipGetDevice(&cur));
+    if (cur != device) PHDE_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) PHDE_HIP(hipSetDevice(cur));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("phde_power_eig_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+      return PHDE_ERR_DEVICE;
+    }
+    g_scratch_dev[device] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[device];
+  return PHDE_OK;
+}
+
+// The power method's start values on the device: srand(1), then
+// rand() / RAND_MAX for `count` draws (the sequence glibc_rand_sequence
+// gives the host), kept in a 34-entry ring: r[i] needs r[i-31] and r[i-3].
+__global__ void power_start_kernel(double *init, int count) {
+  if (threadIdx.x != 0) return;
+  uint32_t ring[34];
+  int64_t word = 1;
+  ring[0] = 1u;
+  for (int i = 1; i < 31; i++) {
+    const int64_t hi = word / 127773, lo = word % 127773;
+    word = 16807 * lo - 2836 * hi;
+    if (word < 0) word += 2147483647;
+    ring[i] = (uint32_t)word;
+  }
+  for (int i = 31; i < 34; i++) ring[i] = ring[i - 31];
+  for (int i = 34; i < 344 + count; i++) {
+    const uint32_t v = ring[(i - 31) % 34] + ring[(i - 3) % 34];
+    ring[i % 34] = v;
+    if (i >= 344) init[i - 344] = (double)(int32_t)(v >> 1) / 2147483647;
+  }
+}
+
 }  // namespace phde
+
+using namespace phde;
+
+extern "C" int phde_power_eig_noalloc(phde_ctx *c, double *K_dev, double *evals_dev, int dim,
+                                      int32_t *status_dev) {
+  if (!c || !K_dev || !evals_dev || !status_dev) {
+    set_error("phde_power_eig_noalloc: null argument");
+    return PHDE_ERR_ARG;
+  }
+  if (dim < 1 || dim > kNoAllocMaxDim) {
+    set_error("phde_power_eig_noalloc: dim " + std::to_string(dim) + " outside 1.." +
+              std::to_string(kNoAllocMaxDim) + " (the size of its static scratch)");
+    return PHDE_ERR_ARG;
+  }
+  double *scr = nullptr;
+  PHDE_TRY_RC(scratch_address(c->device, &scr));
+  double *init = scr, *work = scr + (size_t)dim * dim;
+  power_start_kernel<<<1, 64, 0, c->stream>>>(init, dim * dim);
+  PHDE_HIP(hipGetLastError());
+  PHDE_TRY_RC(power_eig_dev(c, K_dev, init, evals_dev, dim, 100000, work, reinterpret_cast<int *>(status_dev)));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  return PHDE_OK;
+}

@@ -173,6 +173,20 @@ int phde_layout(phde_ctx *ctx, const phde_options *opt, double *coords, phde_sta
  * the eigenvalue estimates.  iters optional. */
 int phde_power_eig(phde_ctx *ctx, double *K, double *evals, int dim, int32_t *iters);
 
+/* The power method of phde_power_eig on device-resident data, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc or
+ * hipFree: its scratch (start values and a work matrix, 64 KiB) is a
+ * static device array of the library, one per GPU, held until the process
+ * ends, so calls that overlap on one GPU race on it.  K_dev (dim*dim,
+ * column-major) is overwritten with the eigenvector matrix, evals_dev (dim)
+ * gets the eigenvalue estimates, status_dev (2 int32) the status (0 ok,
+ * 1 NaN/Inf, 2 iteration cap) and the iteration count, all device
+ * pointers.  dim <= 64, else PHDE_ERR_ARG.  Returns once the results are
+ * written (synchronises the context's stream). */
+int phde_power_eig_noalloc(phde_ctx *ctx, double *K_dev, double *evals_dev, int dim, int32_t *status_dev);
+
 /* Exact symmetric eigensolve (cyclic Jacobi, one workgroup) of the
  * symmetric part of A (dim*dim).  With B != NULL, the generalised problem
  * A x = lambda B x (B symmetric positive definite, Cholesky-reduced).
