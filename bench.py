@@ -32,13 +32,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bfs_kernel_name(deep: bool) -> str:
+def bfs_kernel_name(deep: bool, parts: int) -> str:
     """The traversal kernel the library launches: one cooperative launch per
-    traversal -- the multi-hop kernel for a graph it found deep (renumbered)
-    unless PHDE_BFS_HOPS=0, else the level-by-level one -- or one launch per
-    level with PHDE_BFS_PERSISTENT=0."""
+    traversal -- the partitioned kernel for a graph it found deep and cut
+    into parts, else the multi-hop kernel for a deep graph unless
+    PHDE_BFS_HOPS=0, else the level-by-level one -- or one launch per level
+    with PHDE_BFS_PERSISTENT=0."""
     if os.environ.get("PHDE_BFS_PERSISTENT", "1") == "0":
         return "bfs_level_kernel"
+    if deep and parts > 0:
+        return "pbfs_kernel"
     if deep and os.environ.get("PHDE_BFS_HOPS", "") not in ("0", "1"):
         return "bfs_hops_kernel"
     return "bfs_persistent_kernel"
@@ -151,7 +154,7 @@ def main():
     t0 = time.time()
     ctx = L.Context(g)
     h2d_s = time.time() - t0
-    kernel = bfs_kernel_name(ctx.vertex_order()[1])
+    kernel = bfs_kernel_name(ctx.vertex_order()[1], ctx.num_parts())
     log(f"[bench] graph resident in HBM ({h2d_s * 1e3:.1f} ms incl. validation and H2D)")
     prec = L.PREC_FP32 if args.precision == "fp32" else L.PREC_FP64
     opts = dict(s=args.s, num_coord=2, precision=prec, eager=1 if args.eager else 0)

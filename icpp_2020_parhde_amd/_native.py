@@ -103,6 +103,7 @@ PARHDE_SYMBOLS = {
     "phde_layout": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PhdeOptions), c_f64p,
                                    ctypes.POINTER(PhdeStats)]),
     "phde_power_eig": (ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, ctypes.c_int, c_i32p]),
+    "phde_num_parts": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "phde_power_eig_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                               ctypes.c_void_p]),
     "phde_sym_eig": (ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, ctypes.c_int, c_f64p, c_f64p]),

@@ -25,6 +25,7 @@
 #include "learned.h"
 #include "mds.h"
 #include "orth.h"
+#include "pbfs.h"
 #include "phde_common.h"
 #include "spmm.h"
 #include "traverse.h"
@@ -164,6 +165,8 @@ static int ctx_renumber(phde_ctx *c) {
   bool done = false;
   PHDE_TRY_RC(relabel_graph(c, &done));
   if (done) PHDE_TRY_RC(ctx_finish(c));
+  // unweighted graphs with a deep BFS: the partitioned traversal
+  if (c->bfs_hops > 1 && !c->g.w) PHDE_TRY_RC(pbfs_setup(c));
   return spmm_hubs(c);
 }
 
@@ -187,6 +190,7 @@ static void ctx_free(phde_ctx *c) {
   if (c->h_small) (void)hipHostFree(c->h_small);
   traverse_release(c);
   learned_release(c);
+  pbfs_release(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -995,6 +999,12 @@ int phde_vertex_order(const phde_ctx *c, uint32_t *order, int *is_renumbered) {
   if (is_renumbered) *is_renumbered = renumbered(c) ? 1 : 0;
   if (order)
     for (int64_t k = 0; k < c->g.n; k++) order[k] = (uint32_t)to_original(c, k);
+  return PHDE_OK;
+}
+
+int phde_num_parts(const phde_ctx *c, int64_t *parts) {
+  if (!c || !parts) return PHDE_ERR_ARG;
+  *parts = pbfs_num_parts(c);
   return PHDE_OK;
 }
 

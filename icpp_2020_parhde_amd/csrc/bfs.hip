@@ -30,6 +30,7 @@
 #include "device_util.h"
 #include "hops.h"
 #include "levels.h"
+#include "pbfs.h"
 #include "phde_common.h"
 
 namespace phde {
@@ -724,6 +725,15 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
   if (src < 0 || src >= c->g.n) {
     set_error("bfs source out of range");
     return PHDE_ERR_ARG;
+  }
+  if (c->pg) {
+    // deep graph cut into parts: the partitioned traversal (pbfs.hip)
+    bool ran = false;
+    hipEvent_t pev = nullptr;
+    c->timer.begin(c->stream, PHDE_KERNEL_BFS_LEVEL, &pev);
+    rc = pbfs_run(c, src, dist_out, min_dist, &ran);
+    c->timer.end(c->stream, PHDE_KERNEL_BFS_LEVEL, pev);
+    if (rc || ran) return rc;
   }
   BfsArgs a;
   a.g = c->g;

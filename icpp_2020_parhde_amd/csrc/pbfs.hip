@@ -1,0 +1,849 @@
+// pbfs.hip — partitioned breadth-first search for graphs with a deep BFS.
+//
+// Reference behaviour followed: DOBFS (gapbs/bfs.cc:144-185) — exact hop
+// levels from one source — and the k-centers step of ParHDE.cpp:124-140
+// (min over columns, first maximiser, lowest index on ties) fused at the end.
+//
+// A deep graph (thousands of narrow BFS levels: meshes, geometric graphs)
+// spends its traversal on the latency of each level's dependent global
+// round trips.  Here the graph is cut once, at context creation, into
+// compact parts whose local CSR, levels and work lists fit in one CU's LDS,
+// and one persistent cooperative kernel runs the search as a work queue of
+// parts: a workgroup takes a part some of whose vertices were lowered from
+// outside, runs level-synchronous rounds inside LDS from those vertices,
+// writes the part's lowered levels back and lowers the levels of its
+// neighbours in other parts (atomicMin), queueing their parts.  Every level
+// written is the length of a real path and levels only decrease, so when no
+// part is queued or running the levels are the exact BFS levels
+// (label-correcting across parts, breadth-first inside one); only the
+// amount of work depends on timing.
+//
+// Per-vertex state lives in part-major order (part p's vertices at
+// [vstart[p], vstart[p+1]), in the part's local order): `plevel`, the
+// part's own levels (written only by a run of the part), and `ilevel`, the
+// levels other parts lowered its boundary vertices to (atomicMin).  A run
+// reads ilevel for the part's boundary vertices only; a workgroup that
+// runs the same part again keeps its LDS image when no other workgroup ran
+// the part in between (version counter `pver`).
+//
+// Part states: 0 idle, 1 queued, 2 running, 3 running and lowered again
+// (the runner then runs it once more).  Lowering a vertex from outside
+// marks its part with atomicOr(1): 0 -> 1 queues it (and counts it in
+// `pending`), 2 -> 3 flags the runner.  `pending` counts parts queued or
+// running, so it reaches 0 only when the search is over.  Every wait is
+// bounded: a timeout sets an error word, every workgroup stops, and the
+// host repeats the traversal with the multi-hop kernel.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "device_util.h"
+#include "pbfs.h"
+
+namespace phde {
+
+constexpr int kPbfsBlock = 1024;         // 16 waves: a round's frontier spreads over all of them
+constexpr int kExpandLanes = 16;         // lanes per frontier vertex in a round
+constexpr int kClaims = 1;               // neighbours a lane claims per step
+constexpr int kPartBytes = 144 * 1024;   // LDS image of one part (see part_bytes)
+constexpr uint32_t kNoPart = 0xffffffffu;
+// Control words: the ticket counter and the error flag share a line; the
+// queue tail and the count of parts queued or running share one 64-bit word
+// on a line of its own (tail << 32 | pending), so that queueing a part
+// takes one atomic.
+constexpr int kCtlHead = 0, kCtlErr = 1, kCtlTP = 64, kCtlWords = 96;
+// Ring slots are 64-bit, (entry index + 1) << 32 | (part + 1), one per
+// 128-byte line: the workgroup holding ticket h waits on slot h alone.
+constexpr int kSlotStride = 16;
+constexpr uint32_t kDoneSlot = 0xffffffffu;  // low word of a slot: the search is over
+__device__ __forceinline__ unsigned long long *tp_word(uint32_t *ctl) {
+  return reinterpret_cast<unsigned long long *>(ctl + kCtlTP);
+}
+
+// A part's static data is one blob, copied into LDS as it is: local row
+// offsets (nv + 1), local adjacency (ne), boundary list (nb), external
+// edges' local sources (nx) as uint16, then the external edges' targets
+// (part-major index) and target parts (nx each) as uint32; 16-byte aligned.
+__host__ __device__ constexpr int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+__host__ __device__ constexpr int64_t blob_bytes(int64_t nv, int64_t ne, int64_t nb, int64_t nx) {
+  return align_up(align_up(2 * (nv + 1 + ne + nb + nx), 4) + 8 * nx, 16);
+}
+// LDS bytes of a part with nv vertices, ne internal and nx external
+// (directed) edges and nb boundary vertices: the blob, levels and seed
+// levels (int32), two frontier lists (nv + 1 entries: the source can be
+// seeded twice, as a boundary vertex and as the source) and the seed list
+// (uint16), changed flags.
+__host__ __device__ constexpr int64_t part_bytes(int64_t nv, int64_t ne, int64_t nb, int64_t nx) {
+  return blob_bytes(nv, ne, nb, nx) + 4 * nv + 4 * (nb + 1) + 4 * (nv + 1) + 2 * (nb + 1) + nv + 64;
+}
+
+struct PbfsArgs {
+  DevGraph g;
+  int32_t *level;   // output: the traversal's levels (internal ids)
+  int32_t *plevel;  // part-major: the part's own levels
+  int32_t *ilevel;  // part-major: levels lowered from outside
+  const uint32_t *vstart, *pmidx, *pcount;  // pcount: (ne, nb, nx) per part
+  const uint4 *blob;
+  const uint32_t *blob_off;  // per part, in 16-byte units
+  uint32_t *st, *pver, *ctl;
+  unsigned long long *ring;  // ring_n slots, kSlotStride words apart
+  uint32_t ring_n, src_part, src_loc;
+  int64_t k, src;
+  int32_t *min_dist;  // nullable: fused k-centers update
+  KcResult *kc;
+  unsigned int *arrive, *bar_err;
+  int backoff;
+  // with PHDE_PBFS_STATS: [0..3] per-phase wall time (100 MHz ticks) summed
+  // over workgroups — waiting for a part, seeding (and loading images),
+  // rounds, write-back and pushes; [4..6] activations, rounds, image loads;
+  // [7, 8] first and last tick; [9..11] expansion passes, jumps, seed scans;
+  // [13] frontier vertices; [14, 15] seedless activations, seeds;
+  // [16..19] queued-to-taken latency: sum, count, count above 10 us, max;
+  // [32 + slot] the tick at which each ring slot was filled
+  unsigned long long *stats;
+};
+
+__global__ void pbfs_init_kernel(PbfsArgs a) {
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
+  const uint32_t src_pm = a.vstart[a.src_part] + a.src_loc;
+  for (int64_t j = gtid; j < a.g.n; j += gsize) {
+    a.plevel[j] = kUnvisited;
+    a.ilevel[j] = j == src_pm ? 0 : kUnvisited;
+  }
+  for (int64_t p = gtid; p < a.k; p += gsize) a.st[p] = p == a.src_part ? 1u : 0u;
+  for (int64_t r = gtid; r < a.ring_n; r += gsize)
+    a.ring[r * kSlotStride] = r == 0 ? (1ull << 32) | (a.src_part + 1u) : 0ull;
+  if (gtid < kCtlWords) a.ctl[gtid] = gtid == kCtlTP || gtid == kCtlTP + 1 ? 1u : 0u;  // tail 1, pending 1
+  if (gtid == 0) {
+    a.kc->key = 0;
+    a.kc->sumsq[0] = a.kc->sumsq[1] = 0;
+    a.kc->unreached = 0;
+    a.kc->depth = 0;
+    a.arrive[0] = 0;
+    a.bar_err[0] = 0;
+  }
+}
+
+// A vertex of part q was lowered from outside: queue q or flag its runner.
+__device__ __forceinline__ void mark_lowered(const PbfsArgs &a, uint32_t q) {
+  const uint32_t old = atomicOr(&a.st[q], 1u);
+  if (old == 0u) {
+    const uint32_t t = (uint32_t)(atomicAdd(tp_word(a.ctl), (1ull << 32) | 1ull) >> 32);
+    if (a.stats) st_dev(&a.stats[32 + t % a.ring_n], __builtin_amdgcn_s_memrealtime());
+    st_dev(&a.ring[(uint64_t)(t % a.ring_n) * kSlotStride], ((unsigned long long)(t + 1u) << 32) | (q + 1u));
+  }
+}
+
+// After the last part is released (pending reached zero at tail `t`), the
+// slots of tickets t .. t + gridDim.x - 1 -- the only ones still to be
+// waited on, one per workgroup at most -- are marked done.
+__device__ __forceinline__ void mark_done(const PbfsArgs &a, uint32_t t) {
+  for (uint32_t j = 0; j < gridDim.x; j++) {
+    const uint32_t u = t + j;
+    st_dev(&a.ring[(uint64_t)(u % a.ring_n) * kSlotStride], ((unsigned long long)(u + 1u) << 32) | kDoneSlot);
+  }
+}
+
+// The next queued part (thread 0 only), kNoPart when the search is over or
+// an error stopped it.  Each call takes a ticket and waits on that ticket's
+// slot alone, so idle workgroups poll separate lines instead of contending
+// for one word.  Entry h of the queue lands in slot h % ring_n, tagged with
+// h + 1; ring_n exceeds the parts plus the workgroups, so a slot is not
+// reused while its ticket is still waiting (the bounded wait below turns a
+// violation into an error, never into a wrong part).
+__device__ uint32_t pop_part(const PbfsArgs &a) {
+  const uint32_t h = atomicAdd(&a.ctl[kCtlHead], 1u);
+  const unsigned long long *slot = &a.ring[(uint64_t)(h % a.ring_n) * kSlotStride];
+  long spins = 0;
+  for (;;) {
+    const unsigned long long v = ld_dev(slot);
+    if ((uint32_t)(v >> 32) == h + 1u) {
+      const uint32_t q = (uint32_t)v;
+      if (q == kDoneSlot) return kNoPart;
+      if (a.stats && h > 0) {
+        const unsigned long long tn = __builtin_amdgcn_s_memrealtime();
+        const unsigned long long te = ld_dev(&a.stats[32 + h % a.ring_n]);
+        const unsigned long long ql = tn > te ? tn - te : 0;
+        atomicAdd(&a.stats[16], ql);
+        atomicAdd(&a.stats[17], 1ull);
+        if (ql > 1000) atomicAdd(&a.stats[18], 1ull);
+        atomicMax(&a.stats[19], ql);
+      }
+      return q - 1u;
+    }
+    if ((spins & 63) == 63 && ld_dev(&a.ctl[kCtlErr])) return kNoPart;
+    __builtin_amdgcn_s_sleep(1);
+    if (++spins > (1L << 24)) {  // seconds with nothing queued: a lost wake-up
+      atomicOr(&a.ctl[kCtlErr], 2u);
+      return kNoPart;
+    }
+  }
+}
+
+// Wave-aggregated append of the lanes' `found` vertices to an LDS list;
+// every lane of the wave must call it.
+__device__ __forceinline__ void list_append(bool found, uint16_t v, uint16_t *list, int *count) {
+  const unsigned long long m = __ballot(found);
+  if (!m) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(m));
+  base = __builtin_amdgcn_readlane(base, leader);
+  if (found) list[base + __popcll(m & lanemask_lt())] = v;
+}
+
+__global__ __launch_bounds__(kPbfsBlock) void pbfs_kernel(PbfsArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ uint32_t s_part, s_ver;
+  __shared__ int s_cnt[3], s_ns[3], s_nseed, s_base, s_jns, s_rerun;
+  __shared__ int s_dummy[kPbfsBlock];  // targets of masked-off claims, one per thread
+  const int tid = threadIdx.x;
+  const int lane = tid % kWave;
+  uint32_t cached = kNoPart, cached_ver = 0;  // the part whose image is in LDS, and its version
+  unsigned long long t_wait = 0, t_load = 0, t_round = 0, t_push = 0, n_act = 0, n_round = 0, n_img = 0;
+  unsigned long long t_first = ~0ull, t_last = 0, t0 = 0, t1 = 0;
+  unsigned long long n_pass = 0, n_jump = 0, n_scan = 0, n_front = 0;
+  const bool stats = a.stats != nullptr;
+  for (;;) {
+    if (stats && tid == 0) t0 = __builtin_amdgcn_s_memrealtime();
+    if (tid == 0) {
+      const uint32_t p = pop_part(a);
+      if (p != kNoPart) {
+        // the exchange completes before the part's levels are read (every
+        // datum shared between workgroups is a device-scope access, so
+        // completing this wave's own accesses orders them; no cache
+        // write-back is needed)
+        atomicExch(&a.st[p], 2u);
+        s_ver = ld_dev(&a.pver[p]);
+        __builtin_amdgcn_s_waitcnt(0);
+      }
+      s_part = p;
+    }
+    __syncthreads();
+    const uint32_t p = s_part;
+    if (stats && tid == 0) {
+      t1 = __builtin_amdgcn_s_memrealtime();
+      t_wait += t1 - t0;
+      if (p != kNoPart && t1 < t_first) t_first = t1;
+    }
+    if (p == kNoPart) break;
+    const uint32_t v0 = a.vstart[p];
+    const int nv = (int)(a.vstart[p + 1] - v0);
+    const int ne = (int)a.pcount[3 * p], nb = (int)a.pcount[3 * p + 1], nx = (int)a.pcount[3 * p + 2];
+    const int nblob = (int)(a.blob_off[p + 1] - a.blob_off[p]);  // 16-byte words
+    // LDS: the blob, then the per-run state
+    uint16_t *lrow = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *ladj = lrow + nv + 1;
+    uint16_t *blist = ladj + ne;
+    uint16_t *xsrc = blist + nb;
+    const uint32_t *xdst = reinterpret_cast<const uint32_t *>(smem + align_up(2 * (nv + 1 + ne + nb + nx), 4));
+    const uint32_t *xpart = xdst + nx;
+    int32_t *lvl = reinterpret_cast<int32_t *>(smem + 16 * (int64_t)nblob);
+    int32_t *slev = lvl + nv;
+    uint16_t *F0 = reinterpret_cast<uint16_t *>(slev + nb + 1);
+    uint16_t *F1 = F0 + nv + 1;
+    uint16_t *seeds = F1 + nv + 1;
+    unsigned char *chg = reinterpret_cast<unsigned char *>(seeds + nb + 1);
+    for (;;) {
+      if (stats && tid == 0) {
+        t0 = __builtin_amdgcn_s_memrealtime();
+        n_act++;
+      }
+      if (cached != p || cached_ver != s_ver) {
+        // the part's image: its blob (read-only, 16-byte loads, four in
+        // flight per thread) and its own levels (device-scope loads)
+        const uint4 *gb = a.blob + a.blob_off[p];
+        uint4 *sb = reinterpret_cast<uint4 *>(smem);
+        for (int i = tid; i < nblob; i += 8 * kPbfsBlock) {
+          uint4 t[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) t[j] = gb[i + j * kPbfsBlock < nblob ? i + j * kPbfsBlock : i];
+#pragma unroll
+          for (int j = 0; j < 8; j++)
+            if (i + j * kPbfsBlock < nblob) sb[i + j * kPbfsBlock] = t[j];
+        }
+        for (int i0 = 0; i0 < nv; i0 += 8 * kPbfsBlock) {
+          int32_t t[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const int i = i0 + j * kPbfsBlock + tid;
+            t[j] = i < nv ? ld_dev(&a.plevel[v0 + i]) : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const int i = i0 + j * kPbfsBlock + tid;
+            if (i < nv) lvl[i] = t[j];
+          }
+        }
+        if (stats && tid == 0) n_img++;
+      }
+      for (int i = tid; i < nv; i += kPbfsBlock) chg[i] = 0;
+      if (tid == 0) {
+        s_nseed = 0;
+        s_base = kUnvisited;
+        s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
+        s_ns[0] = s_ns[1] = s_ns[2] = kUnvisited;
+        s_jns = kUnvisited;
+      }
+      __syncthreads();
+      // seeds: boundary vertices (and the source) lowered from outside
+      int mybase = kUnvisited;
+      for (int bb = 0; bb <= nb; bb += kPbfsBlock) {
+        const int b = bb + tid;
+        int u = -1;
+        if (b < nb)
+          u = blist[b];
+        else if (b == nb && p == a.src_part)
+          u = (int)a.src_loc;
+        bool sd = false;
+        int32_t l = kUnvisited;
+        if (u >= 0) {
+          l = ld_dev(&a.ilevel[v0 + u]);
+          sd = l < lvl[u];
+        }
+        const unsigned long long m = __ballot(sd);
+        if (m) {
+          const int leader = __ffsll((long long)m) - 1;
+          int base = 0;
+          if (lane == leader) base = atomicAdd(&s_nseed, __popcll(m));
+          if (stats && lane == leader) atomicAdd(&a.stats[15], (unsigned long long)__popcll(m));
+          base = __shfl(base, leader);
+          if (sd) {
+            const int k = base + __popcll(m & lanemask_lt());
+            seeds[k] = (uint16_t)u;
+            slev[k] = l;
+            lvl[u] = l;
+            chg[u] = 1;
+            mybase = min(mybase, l);
+          }
+        }
+      }
+      mybase = -wave_max_i32(-mybase);
+      if (lane == 0 && mybase < kUnvisited) atomicMin(&s_base, mybase);
+      __syncthreads();
+      const int nseed = s_nseed;
+      if (stats && tid == 0 && nseed == 0) atomicAdd(&a.stats[14], 1ull);
+      int cur = s_base;
+      int r = 0, ns = kUnvisited;
+      // Seeds at level L join the frontier list `list` (count *cnt); the
+      // lowest level of a seed still waiting above L goes to *next.
+      auto seed_scan = [&](int L, uint16_t *list, int *cnt, int *next) {
+        int mn = kUnvisited;
+        for (int kk = 0; kk < nseed; kk += kPbfsBlock) {
+          const int k = kk + tid;
+          bool f = false;
+          if (k < nseed) {
+            const int sl = slev[k];
+            const bool valid = lvl[seeds[k]] == sl;
+            f = valid && sl == L;
+            if (valid && sl > L) mn = min(mn, sl);
+          }
+          list_append(f, f ? seeds[k] : 0, list, cnt);
+        }
+        mn = -wave_max_i32(-mn);
+        if (lane == 0 && mn < kUnvisited) atomicMin(next, mn);
+      };
+      if (nseed > 0) {
+        seed_scan(cur, F0, &s_cnt[0], &s_jns);
+        __syncthreads();
+        ns = s_jns;
+      }
+      if (stats && tid == 0) {
+        t1 = __builtin_amdgcn_s_memrealtime();
+        t_load += t1 - t0;
+        t0 = t1;
+      }
+      while (nseed > 0) {
+        // round r: the frontier at level cur (list r & 1, count s_cnt[r % 3])
+        // expands into level cur + 1 (list (r + 1) & 1, count s_cnt[(r + 1) % 3])
+        const int nf = s_cnt[r % 3];
+        if (tid == 0) {
+          s_cnt[(r + 2) % 3] = 0;
+          s_ns[(r + 2) % 3] = kUnvisited;
+        }
+        uint16_t *Fc = (r & 1) ? F1 : F0;
+        if (nf == 0) {
+          // nothing at level cur: jump to the lowest seed still waiting
+          if (ns == kUnvisited) break;
+          if (stats && tid == 0) n_jump++;
+          cur = ns;
+          if (tid == 0) s_jns = kUnvisited;
+          __syncthreads();
+          seed_scan(cur, Fc, &s_cnt[r % 3], &s_jns);
+          __syncthreads();
+          ns = s_jns;
+          continue;
+        }
+        const int nl = cur + 1;
+        if (stats && tid == 0) {
+          n_pass += (nf + kPbfsBlock / kExpandLanes - 1) / (kPbfsBlock / kExpandLanes);
+          n_front += nf;
+        }
+        uint16_t *Fn = (r & 1) ? F0 : F1;
+        int *cn = &s_cnt[(r + 1) % 3];
+        // kExpandLanes lanes per frontier vertex; a lane claims up to
+        // kClaims neighbours (all reads and claims issued before any result
+        // is used), then the wave appends its winners with one atomic
+        for (int f0 = 0; f0 < nf; f0 += kPbfsBlock / kExpandLanes) {
+          const int f = f0 + tid / kExpandLanes;
+          int xb = 0, e = 0;
+          if (f < nf) {
+            const int u = Fc[f];
+            xb = lrow[u];
+            e = lrow[u + 1];
+          }
+          xb += tid % kExpandLanes;
+          while (__ballot(xb < e)) {
+            // unconditional reads and claims (clamped index, masked-off
+            // lanes claim their own dummy word), all issued before any
+            // result is used
+            int w[kClaims], old[kClaims];
+            bool ok[kClaims];
+#pragma unroll
+            for (int q = 0; q < kClaims; q++) {
+              const int x = xb + q * kExpandLanes;
+              ok[q] = x < e;
+              w[q] = ladj[ok[q] ? x : 0];
+            }
+#pragma unroll
+            for (int q = 0; q < kClaims; q++) old[q] = atomicMin(ok[q] ? &lvl[w[q]] : &s_dummy[tid], nl);
+            // positions from ballots: winner q of a lane goes after every
+            // winner q' < q of the wave and after winners q of lower lanes
+            unsigned long long mq[kClaims];
+            int pre[kClaims], total = 0;
+#pragma unroll
+            for (int q = 0; q < kClaims; q++) {
+              mq[q] = __ballot(ok[q] && old[q] > nl);
+              pre[q] = total;
+              total += __popcll(mq[q]);
+            }
+            if (total) {
+              int base = 0;
+              if (lane == 0) base = atomicAdd(cn, total);
+              base = __builtin_amdgcn_readfirstlane(base);
+              const unsigned long long lt = lanemask_lt();
+#pragma unroll
+              for (int q = 0; q < kClaims; q++)
+                if ((mq[q] >> lane) & 1ull) {
+                  chg[w[q]] = 1;
+                  Fn[base + pre[q] + __popcll(mq[q] & lt)] = (uint16_t)w[q];
+                }
+            }
+            xb += kClaims * kExpandLanes;
+          }
+        }
+        const bool scan = nl == ns;
+        if (scan) seed_scan(nl, Fn, cn, &s_ns[(r + 1) % 3]);
+        if (stats && tid == 0) n_scan += scan ? 1 : 0;
+        __syncthreads();
+        if (scan) ns = s_ns[(r + 1) % 3];
+        r++;
+        cur = nl;
+      }
+      if (stats && tid == 0) {
+        t1 = __builtin_amdgcn_s_memrealtime();
+        t_round += t1 - t0;
+        t0 = t1;
+        n_round += r;
+      }
+      // write back the lowered levels, then lower the neighbours in other
+      // parts (one external edge per thread)
+      for (int i = tid; i < nv; i += kPbfsBlock)
+        if (chg[i]) st_dev(&a.plevel[v0 + i], lvl[i]);
+      for (int x0 = 0; x0 < nx; x0 += 4 * kPbfsBlock) {
+        // up to four external edges per thread, their claims in flight together
+        int32_t nl[4], old[4];
+        bool on[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int x = x0 + j * kPbfsBlock + tid;
+          on[j] = false;
+          nl[j] = 0;
+          if (x < nx) {
+            const int u = xsrc[x];
+            on[j] = chg[u] != 0;
+            nl[j] = lvl[u] + 1;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          old[j] = on[j] ? atomicMin(&a.ilevel[xdst[x0 + j * kPbfsBlock + tid]], nl[j]) : 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (on[j] && old[j] > nl[j]) mark_lowered(a, xpart[x0 + j * kPbfsBlock + tid]);
+      }
+      // this run's stores and claims complete before the part is released
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (stats && tid == 0) {
+        t1 = __builtin_amdgcn_s_memrealtime();
+        t_push += t1 - t0;
+        if (t1 > t_last) t_last = t1;
+      }
+      if (tid == 0) {
+        // this run's image becomes the part's current version (only the
+        // part's runner changes it)
+        s_ver = s_ver + 1u;
+        st_dev(&a.pver[p], s_ver);
+        __builtin_amdgcn_s_waitcnt(0);  // the version lands before the part can be taken again
+        const uint32_t old = atomicCAS(&a.st[p], 2u, 0u);
+        if (old == 2u) {
+          s_rerun = 0;
+          const unsigned long long tp = atomicSub(tp_word(a.ctl), 1ull);
+          if ((uint32_t)tp == 1u) mark_done(a, (uint32_t)(tp >> 32));  // the last part: the search is over
+        } else {
+          // lowered again while it ran: run it once more
+          atomicExch(&a.st[p], 2u);
+          __builtin_amdgcn_s_waitcnt(0);
+          s_rerun = 1;
+        }
+      }
+      __syncthreads();
+      cached = p;
+      cached_ver = s_ver;
+      if (!s_rerun) break;
+    }
+  }
+  if (stats && tid == 0) {
+    atomicAdd(&a.stats[0], t_wait);
+    atomicAdd(&a.stats[1], t_load);
+    atomicAdd(&a.stats[2], t_round);
+    atomicAdd(&a.stats[3], t_push);
+    atomicAdd(&a.stats[4], n_act);
+    atomicAdd(&a.stats[5], n_round);
+    atomicAdd(&a.stats[6], n_img);
+    atomicMin(&a.stats[7], t_first);
+    atomicMax(&a.stats[8], t_last);
+    atomicAdd(&a.stats[9], n_pass);
+    atomicAdd(&a.stats[10], n_jump);
+    atomicAdd(&a.stats[11], n_scan);
+    atomicAdd(&a.stats[13], n_front);
+  }
+  // every workgroup has stopped taking parts; the levels are final unless
+  // an error stopped the search
+  if (grid_barrier(a.arrive, gridDim.x, a.bar_err, a.backoff)) return;
+  if (ld_dev(&a.ctl[kCtlErr])) return;
+  // levels to the output (internal ids) and the fused k-centers step:
+  // min_dist = min(min_dist, level), the first maximiser (caller's id on
+  // ties), exact sum of squares, unreached count, largest level
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + tid;
+  const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
+  unsigned long long key = 0, unreached = 0;
+  U128 sumsq{0, 0};
+  int32_t maxd = 0;
+  for (int64_t i = gtid; i < a.g.n; i += gsize) {
+    const int32_t d = ld_dev(&a.plevel[a.pmidx[i]]);
+    a.level[i] = d;
+    if (d == kUnvisited) {
+      unreached++;
+    } else {
+      u128_add(sumsq, (unsigned long long)d * (unsigned long long)d);
+      maxd = d > maxd ? d : maxd;
+    }
+    if (a.min_dist) {
+      int32_t md = a.min_dist[i];
+      if (d < md) {
+        md = d;
+        a.min_dist[i] = md;
+      }
+      const unsigned long long kk = ((unsigned long long)(uint32_t)md << 32) |
+                                    (0xffffffffull - (unsigned long long)original_id(a.g, (uint32_t)i));
+      key = kk > key ? kk : key;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long ok = __shfl_xor(key, o);
+    key = ok > key ? ok : key;
+    unreached += __shfl_xor(unreached, o);
+  }
+  sumsq = wave_sum_u128(sumsq);
+  maxd = wave_max_i32(maxd);
+  if (lane == 0) {
+    if (a.min_dist && key) atomicMax(&a.kc->key, key);
+    atomic_add_u128(a.kc->sumsq, sumsq);
+    if (unreached) atomicAdd(&a.kc->unreached, unreached);
+    if (maxd > 0) atomicMax(reinterpret_cast<unsigned long long *>(&a.kc->depth), (unsigned long long)maxd);
+  }
+}
+
+// ------------------------------------------------------------------ host side
+
+// The parts of a deep graph (device arrays) and the traversal's state.
+struct PartGraph {
+  int64_t k = 0;
+  uint32_t *vstart = nullptr, *pmidx = nullptr, *pcount = nullptr, *blob_off = nullptr;
+  uint4 *blob = nullptr;
+  std::vector<uint32_t> h_part_of, h_loc;
+  int32_t *plevel = nullptr, *ilevel = nullptr;
+  uint32_t *st = nullptr, *pver = nullptr, *ctl = nullptr;
+  unsigned long long *ring = nullptr;
+  uint32_t ring_n = 0;
+  bool disabled = false;
+  unsigned long long *stats = nullptr;  // PHDE_PBFS_STATS
+};
+
+void pbfs_release(phde_ctx *c) {
+  PartGraph *pg = c->pg;
+  if (!pg) return;
+  void *ptr[] = {pg->vstart, pg->pmidx, pg->pcount, pg->blob_off, pg->blob, pg->plevel, pg->ilevel,
+                 pg->st,     pg->pver,  pg->ring,   pg->ctl,      pg->stats};
+  for (void *x : ptr)
+    if (x) (void)hipFree(x);
+  delete pg;
+  c->pg = nullptr;
+}
+
+template <typename T>
+static int upload(T **dst, const std::vector<T> &src) {
+  PHDE_HIP(hipMalloc(dst, sizeof(T) * std::max<size_t>(src.size(), 1)));
+  if (!src.empty()) PHDE_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+  return PHDE_OK;
+}
+
+// Greedy graph growing: the lowest-numbered unassigned vertex seeds a part,
+// which grows breadth-first over unassigned vertices while its LDS image
+// fits the budget (internal and external edges counted exactly as the part
+// grows, every vertex counted as a boundary vertex).  Each part's vertices
+// are numbered locally in that growth order; an edge to another part
+// becomes an external edge of the part (local source, part-major index of
+// the target, target part) and makes its target a boundary vertex of the
+// target's part.
+int pbfs_setup(phde_ctx *c) {
+  if (c->pg) return PHDE_OK;
+  if (const char *e = getenv("PHDE_PBFS"))
+    if (atoi(e) == 0) return PHDE_OK;
+  const int64_t n = c->g.n, m = c->g.m;
+  if (n >= (int64_t)UINT32_MAX - 2) return PHDE_OK;
+  // part budget in LDS bytes (smaller budgets: more parts, for tests)
+  int64_t budget = kPartBytes;
+  if (const char *e = getenv("PHDE_PBFS_BYTES")) budget = std::max<int64_t>(128, std::min<int64_t>(kPartBytes, atoll(e)));
+  int resident = 0;
+  PHDE_HIP(hipFuncSetAttribute((const void *)pbfs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kPartBytes));
+  PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, pbfs_kernel, kPbfsBlock, kPartBytes));
+  if (resident < 1) return PHDE_OK;
+  std::vector<uint32_t> rp(n + 1), adj(std::max<int64_t>(m, 1));
+  PHDE_HIP(hipMemcpy(rp.data(), c->g.rowptr, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost));
+  if (m) PHDE_HIP(hipMemcpy(adj.data(), c->g.adj, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+  std::vector<uint32_t> part_of(n, kNoPart), vstart{0}, verts;
+  verts.reserve(n);
+  std::vector<uint32_t> q;
+  int64_t k = 0;
+  for (int64_t s = 0; s < n; s++) {
+    if (part_of[s] != kNoPart) continue;
+    const uint32_t p = (uint32_t)k++;
+    q.clear();
+    q.push_back((uint32_t)s);
+    part_of[s] = p;
+    int64_t nv = 1, ne = 0, nx = 0;
+    for (uint32_t x = rp[s]; x < rp[s + 1]; x++) {
+      if (adj[x] == (uint32_t)s) ne++;  // a self loop stays inside
+      else nx++;
+    }
+    if (part_bytes(nv, ne, nv, nx) > budget || ne + nx > 65535) return PHDE_OK;  // a vertex too wide for one part
+    for (size_t h = 0; h < q.size(); h++) {
+      const uint32_t u = q[h];
+      for (uint32_t x = rp[u]; x < rp[u + 1]; x++) {
+        const uint32_t w = adj[x];
+        if (part_of[w] != kNoPart) continue;
+        // w joins: its edges into the part turn internal from both sides
+        int64_t in = 0, out = 0;
+        for (uint32_t y = rp[w]; y < rp[w + 1]; y++) {
+          const uint32_t z = adj[y];
+          if (part_of[z] == p || z == w) in++;
+          else out++;
+        }
+        const int64_t ne2 = ne + 2 * in, nx2 = nx - in + out;
+        // 16-bit local ids and row offsets
+        if (part_bytes(nv + 1, ne2, nv + 1, nx2) > budget || nv + 1 > 65535 || ne2 + nv + 1 > 65535) continue;
+        nv++;
+        ne = ne2;
+        nx = nx2;
+        part_of[w] = p;
+        q.push_back(w);
+      }
+    }
+    verts.insert(verts.end(), q.begin(), q.end());
+    vstart.push_back((uint32_t)verts.size());
+  }
+  std::vector<uint16_t> loc(n);
+  std::vector<uint32_t> pmidx(n);
+  for (int64_t p = 0; p < k; p++)
+    for (uint32_t i = vstart[p]; i < vstart[p + 1]; i++) {
+      loc[verts[i]] = (uint16_t)(i - vstart[p]);
+      pmidx[verts[i]] = i;
+    }
+  std::vector<unsigned char> boundary(n, 0);
+  for (int64_t u = 0; u < n; u++)
+    for (uint32_t x = rp[u]; x < rp[u + 1]; x++)
+      if (part_of[adj[x]] != part_of[u]) boundary[adj[x]] = 1;
+  std::vector<uint32_t> pcount(3 * k), blob_off{0};
+  std::vector<uint16_t> b16;   // one part's uint16 section
+  std::vector<uint32_t> b32;   // and its uint32 section
+  std::vector<uint4> blob;
+  blob.reserve((size_t)(2 * m + 8 * n) / 16 + k);
+  int64_t mn = n, mx = 0, worst = 0, next = 0, nbtot = 0;
+  for (int64_t p = 0; p < k; p++) {
+    const uint32_t v0 = vstart[p], nv = vstart[p + 1] - v0;
+    std::vector<uint16_t> lrow, ladj, blist, xsrc;
+    std::vector<uint32_t> xdst, xpart;
+    for (uint32_t i = 0; i < nv; i++) {
+      const uint32_t u = verts[v0 + i];
+      lrow.push_back((uint16_t)ladj.size());
+      for (uint32_t x = rp[u]; x < rp[u + 1]; x++) {
+        const uint32_t w = adj[x];
+        if (part_of[w] == (uint32_t)p) {
+          ladj.push_back(loc[w]);
+        } else {
+          xsrc.push_back((uint16_t)i);
+          xdst.push_back(pmidx[w]);
+          xpart.push_back(part_of[w]);
+        }
+      }
+      if (boundary[u]) blist.push_back((uint16_t)i);
+    }
+    lrow.push_back((uint16_t)ladj.size());
+    const int64_t ne = ladj.size(), nb = blist.size(), nx = xsrc.size();
+    pcount[3 * p] = (uint32_t)ne;
+    pcount[3 * p + 1] = (uint32_t)nb;
+    pcount[3 * p + 2] = (uint32_t)nx;
+    b16.assign(lrow.begin(), lrow.end());
+    b16.insert(b16.end(), ladj.begin(), ladj.end());
+    b16.insert(b16.end(), blist.begin(), blist.end());
+    b16.insert(b16.end(), xsrc.begin(), xsrc.end());
+    const int64_t bytes = blob_bytes(nv, ne, nb, nx);
+    std::vector<unsigned char> raw(bytes, 0);
+    std::memcpy(raw.data(), b16.data(), 2 * b16.size());
+    const int64_t o32 = align_up(2 * (int64_t)b16.size(), 4);
+    std::memcpy(raw.data() + o32, xdst.data(), 4 * nx);
+    std::memcpy(raw.data() + o32 + 4 * nx, xpart.data(), 4 * nx);
+    const size_t at = blob.size();
+    blob.resize(at + bytes / 16);
+    std::memcpy(&blob[at], raw.data(), bytes);
+    next += bytes / 16;
+    if (next > (int64_t)UINT32_MAX) return PHDE_OK;
+    blob_off.push_back((uint32_t)next);
+    mn = std::min<int64_t>(mn, nv);
+    mx = std::max<int64_t>(mx, nv);
+    worst = std::max(worst, part_bytes(nv, ne, nb, nx));
+    nbtot += nb;
+  }
+  if (worst > kPartBytes) return PHDE_OK;  // cannot happen: the growth counted every vertex as boundary
+  PartGraph *pg = new PartGraph();
+  c->pg = pg;
+  pg->k = k;
+  pg->ring_n = (uint32_t)(k + std::max(1, c->num_cus) + 64);
+  PHDE_TRY_RC(upload(&pg->vstart, vstart));
+  PHDE_TRY_RC(upload(&pg->pmidx, pmidx));
+  PHDE_TRY_RC(upload(&pg->pcount, pcount));
+  PHDE_TRY_RC(upload(&pg->blob_off, blob_off));
+  PHDE_TRY_RC(upload(&pg->blob, blob));
+  PHDE_HIP(hipMalloc(&pg->plevel, sizeof(int32_t) * n));
+  PHDE_HIP(hipMalloc(&pg->ilevel, sizeof(int32_t) * n));
+  PHDE_HIP(hipMalloc(&pg->st, sizeof(uint32_t) * k));
+  PHDE_HIP(hipMalloc(&pg->pver, sizeof(uint32_t) * k));
+  PHDE_HIP(hipMemset(pg->pver, 0, sizeof(uint32_t) * k));
+  PHDE_HIP(hipMalloc(&pg->ring, sizeof(unsigned long long) * kSlotStride * pg->ring_n));
+  PHDE_HIP(hipMalloc(&pg->ctl, sizeof(uint32_t) * kCtlWords));
+  if (getenv("PHDE_PBFS_STATS")) {
+    PHDE_HIP(hipMalloc(&pg->stats, sizeof(unsigned long long) * (32 + pg->ring_n)));
+    std::fprintf(stderr,
+                 "[pbfs] %lld parts, vertices per part %lld..%lld, largest LDS image %lld bytes, %lld boundary "
+                 "vertices, blobs %.1f MB\n",
+                 (long long)k, (long long)mn, (long long)mx, (long long)worst, (long long)nbtot, 16.0 * next / 1e6);
+  }
+  pg->h_part_of.swap(part_of);
+  pg->h_loc.resize(n);
+  for (int64_t v = 0; v < n; v++) pg->h_loc[v] = loc[v];
+  return PHDE_OK;
+}
+
+int64_t pbfs_num_parts(const phde_ctx *c) { return c->pg && !c->pg->disabled ? c->pg->k : 0; }
+
+int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, bool *ran) {
+  *ran = false;
+  PartGraph *pg = c->pg;
+  if (!pg || pg->disabled || !c->bfs_persistent) return PHDE_OK;
+  PbfsArgs a;
+  a.g = c->g;
+  a.level = dist_out ? dist_out : c->d_dist;
+  a.plevel = pg->plevel;
+  a.ilevel = pg->ilevel;
+  a.vstart = pg->vstart;
+  a.pmidx = pg->pmidx;
+  a.pcount = pg->pcount;
+  a.blob = pg->blob;
+  a.blob_off = pg->blob_off;
+  a.st = pg->st;
+  a.pver = pg->pver;
+  a.ring = pg->ring;
+  a.ctl = pg->ctl;
+  a.ring_n = pg->ring_n;
+  a.src_part = pg->h_part_of[src];
+  a.src_loc = pg->h_loc[src];
+  a.k = pg->k;
+  a.src = src;
+  a.min_dist = min_dist;
+  a.kc = c->d_kc;
+  a.arrive = c->d_bar;
+  a.bar_err = c->d_bar + 1;
+  a.backoff = c->bar_backoff;
+  a.stats = pg->stats;
+  if (pg->stats) {
+    const unsigned long long init[32] = {0, 0, 0, 0, 0, 0, 0, ~0ull, 0};
+    PHDE_HIP(hipMemcpyAsync(pg->stats, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  }
+  const int ib = (int)std::max<int64_t>(1, std::min<int64_t>((c->g.n + 255) / 256, (int64_t)c->num_cus * 8));
+  pbfs_init_kernel<<<ib, 256, 0, c->stream>>>(a);
+  PHDE_HIP(hipGetLastError());
+  const int grid = std::max(1, c->num_cus);
+  if ((int64_t)pg->ring_n < pg->k + grid) {  // the ticket slots would be reused while waited on
+    pg->disabled = true;
+    return PHDE_OK;
+  }
+  void *args[] = {&a};
+  const hipError_t e = hipLaunchCooperativeKernel((const void *)pbfs_kernel, dim3(grid),
+                                                  dim3(kPbfsBlock), args, kPartBytes, c->stream);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    pg->disabled = true;
+    return PHDE_OK;
+  }
+  unsigned int flags[2] = {0, 0};
+  PHDE_HIP(hipMemcpyAsync(c->h_kc, c->d_kc, sizeof(KcResult), hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipMemcpyAsync(flags, c->d_bar + 1, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipMemcpyAsync(flags + 1, pg->ctl + kCtlErr, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  if (flags[0] || flags[1]) {
+    // a wait timed out: this context goes back to the multi-hop traversal
+    std::fprintf(stderr, "parhde: partitioned BFS stopped (barrier %u, queue %u); using the multi-hop traversal\n",
+                 flags[0], flags[1]);
+    pg->disabled = true;
+    return PHDE_OK;
+  }
+  if (pg->stats) {
+    unsigned long long s[32];
+    PHDE_HIP(hipMemcpy(s, pg->stats, sizeof(s), hipMemcpyDeviceToHost));
+    std::fprintf(stderr, "[pbfs]   queue latency (queued to taken): %llu parts, mean %.2f us, %llu above 10 us, max %.1f us\n",
+                 s[17], s[16] * 0.01 / (double)std::max(1ull, s[17]), s[18], s[19] * 0.01);
+    const double us = 0.01;  // ticks of 10 ns
+    const double na = (double)std::max(1ull, s[4]);
+    std::fprintf(stderr,
+                 "[pbfs] src %lld: span %.1f us; summed over workgroups: wait %.0f seed %.0f rounds %.0f push %.0f us; "
+                 "%llu activations, %llu rounds (%.1f per activation), %llu image loads; per activation: seed %.2f "
+                 "rounds %.2f push %.2f us, %.3f us per round\n",
+                 (long long)src, (s[8] - s[7]) * us, s[0] * us, s[1] * us, s[2] * us, s[3] * us, s[4], s[5],
+                 s[5] / na, s[6], s[1] * us / na, s[2] * us / na, s[3] * us / na,
+                 s[2] * us / (double)std::max(1ull, s[5]));
+    std::fprintf(stderr, "[pbfs]   %llu passes, %llu jumps, %llu seed scans, %.1f frontier vertices per round, %llu seedless "
+                 "activations, %.1f seeds per activation\n", s[9], s[10], s[11],
+                 (double)s[13] / (double)std::max(1ull, s[5]), s[14], s[15] / na);
+  }
+  *ran = true;
+  return PHDE_OK;
+}
+
+}  // namespace phde

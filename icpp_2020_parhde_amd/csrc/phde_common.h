@@ -72,6 +72,8 @@ __device__ __forceinline__ uint32_t internal_id(const DevGraph &g, uint32_t v) {
   return g.perm ? g.perm[v] : v;
 }
 
+struct PartGraph;  // pbfs.hip: the parts of a deep graph
+
 // Timing of kernel classes with HIP events on the context stream.
 struct Timer {
   bool enabled = false;
@@ -153,6 +155,7 @@ struct phde_ctx {
   int32_t *lp_d12 = nullptr, *lp_sd = nullptr;
   float *lp_score = nullptr, *lp_target = nullptr;
   phde_mpnn *net = nullptr;  // borrowed (phde_set_pivot_net)
+  phde::PartGraph *pg = nullptr;  // partitioned BFS (pbfs.hip); null when not in use
   phde::Timer timer;
 };
 

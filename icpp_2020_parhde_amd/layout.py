@@ -151,6 +151,12 @@ class Context:
                "phde_vertex_order")
         return order, bool(r.value)
 
+    def num_parts(self) -> int:
+        """Parts of the partitioned traversal (0: not in use)."""
+        k = ctypes.c_int64()
+        _check(self.lib.phde_num_parts(self.h, ctypes.byref(k)), "phde_num_parts")
+        return k.value
+
     def sssp(self, source: int, delta: int = 0) -> np.ndarray:
         """Exact distances (DeltaStep, gapbs/sssp.cc:53); 2^30-1 when unreached."""
         d = np.empty(self.n, dtype=np.int32)

@@ -147,6 +147,13 @@ int64_t phde_num_vertices(const phde_ctx *ctx);
  * ids in internal order; *renumbered (may be NULL) 1 or 0. */
 int phde_vertex_order(const phde_ctx *ctx, uint32_t *order, int *renumbered);
 
+/* Parts of the context's partitioned traversal (no reference counterpart):
+ * an unweighted graph with a deep BFS is cut at creation into parts that
+ * each fit one compute unit's local memory, and its traversals run part by
+ * part (same levels).  *parts = 0 when the context does not use it
+ * (low-diameter or weighted graphs, PHDE_PBFS=0). */
+int phde_num_parts(const phde_ctx *ctx, int64_t *parts);
+
 /* Breadth-first levels from one source: DOBFS (gapbs/bfs.cc:144-185).
  * dist[v] = hop count, -1 when unreached.  alpha/beta <= 0 use 15/18. */
 int phde_bfs(phde_ctx *ctx, int64_t source, int alpha, int beta, int32_t *dist);

@@ -1,0 +1,106 @@
+"""The partitioned traversal (csrc/pbfs.hip) of unweighted graphs with a
+deep BFS: exact levels against the oracle's textbook BFS from several
+sources, the k-centers chain and its distance field identical to the same
+context without it (PHDE_PBFS=0), and layouts against the oracle.  Small
+graphs are renumbered on request (PHDE_RELABEL=1, which also marks them
+deep) and cut with small part budgets (PHDE_PBFS_BYTES), so that a few
+thousand vertices make dozens to hundreds of parts and most levels cross a
+part boundary."""
+import os
+from contextlib import contextmanager
+
+import numpy as np
+import pytest
+
+from icpp_2020_parhde_amd import layout as L
+from icpp_2020_parhde_amd.graphs import Graph
+from oracle import pyoracle as O
+
+from test_gpu_core import FP64_TOL, aligned_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@contextmanager
+def context(g, part_bytes=None, pbfs=True):
+    env = {"PHDE_RELABEL": "1", "PHDE_PBFS": "1" if pbfs else "0"}
+    if part_bytes:
+        env["PHDE_PBFS_BYTES"] = str(part_bytes)
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        ctx = L.Context(g)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+    with ctx:
+        yield ctx
+
+
+def two_paths(n):
+    i = np.arange(n - 1)
+    return Graph.from_edges(2 * n, np.concatenate([i, i + n]), np.concatenate([i + 1, i + n + 1]))
+
+
+GRAPHS = {
+    "holed": lambda: Graph.holed_grid(120, 90, 10, 77),
+    "rgg": lambda: Graph.rgg(20000, 10.0, seed=2),
+    "mesh": lambda: Graph.mesh(96, 80),
+    "path": lambda: Graph.path(3000),
+    "two_paths": lambda: two_paths(400),
+}
+BUDGETS = [None, 4096, 600]
+
+
+@pytest.mark.parametrize("budget", BUDGETS)
+@pytest.mark.parametrize("name", list(GRAPHS))
+def test_levels_exact(gpu, name, budget):
+    g = GRAPHS[name]()
+    rng = np.random.default_rng(7)
+    srcs = [0, g.n - 1, g.n // 2] + [int(x) for x in rng.choice(g.n, 4, replace=False)]
+    with context(g, budget) as ctx:
+        parts = ctx.num_parts()
+        assert parts >= 1, name
+        for s in srcs:
+            want = O.seq_bfs(g, s)
+            assert np.array_equal(ctx.bfs(s), want), (name, budget, parts, s)
+    print(f"{name} budget {budget}: {parts} parts, {len(srcs)} sources exact")
+
+
+@pytest.mark.parametrize("budget", BUDGETS)
+def test_kcenters_same_as_without(gpu, budget):
+    g = GRAPHS["holed"]()
+    with context(g, budget) as ctx:
+        assert ctx.num_parts() >= 1
+        a = ctx.kcenters(30, first=11)
+    with context(g, pbfs=False) as ctx:
+        assert ctx.num_parts() == 0
+        b = ctx.kcenters(30, first=11)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("budget", [None, 4096])
+@pytest.mark.parametrize("name,s", [("holed", 30), ("rgg", 50), ("mesh", 20)])
+def test_layout_parity(gpu, name, s, budget):
+    g = GRAPHS[name]()
+    ref = O.parhde(g, s, 2, lazy=True)
+    with context(g, budget) as ctx:
+        assert ctx.num_parts() >= 1
+        X, st = ctx.layout(s=s, num_coord=2)
+        Xe, ste = ctx.layout(s=s, num_coord=2, eager=1)
+    assert st.pivots == ref["pivots"]
+    assert aligned_rel_err(X, ref["coords"]) < FP64_TOL
+    assert np.array_equal(X, Xe) and ste.n_traversals == 3 * s
+
+
+def test_unreached_component(gpu):
+    g = two_paths(400)
+    with context(g, 600) as ctx:
+        d = ctx.bfs(3)
+        assert d[400:].tolist() == [-1] * 400 and d[:400].tolist() == [abs(i - 3) for i in range(400)]
+        with pytest.raises(L.PhdeError) as e:
+            ctx.layout(s=4, num_coord=2)
+        assert e.value.code == -7
