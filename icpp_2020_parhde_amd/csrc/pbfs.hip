@@ -10,13 +10,17 @@
 // compact parts whose local CSR, levels and work lists fit in one CU's LDS,
 // and one persistent cooperative kernel runs the search as a work queue of
 // parts: a workgroup takes a part some of whose vertices were lowered from
-// outside, runs level-synchronous rounds inside LDS from those vertices,
-// writes the part's lowered levels back and lowers the levels of its
-// neighbours in other parts (atomicMin), queueing their parts.  Every level
-// written is the length of a real path and levels only decrease, so when no
-// part is queued or running the levels are the exact BFS levels
-// (label-correcting across parts, breadth-first inside one); only the
-// amount of work depends on timing.
+// outside, runs rounds inside LDS from those vertices (each listed vertex
+// offers its level + 1 to its neighbours; a neighbour whose level drops is
+// listed for the next round), writes the part's lowered levels back and
+// lowers the levels of its neighbours in other parts (atomicMin), queueing
+// their parts.  Every level written is the length of a real path and levels
+// only decrease, so when no part is queued or running the levels are the
+// exact BFS levels (label-correcting); only the amount of work depends on
+// timing.  A list may hold vertices of several levels: a lowering that
+// arrives while a part runs is taken up at once instead of after the run's
+// current wave (measured: 3.9 ms per traversal of RGG n=4.2M, 4.4 ms with
+// level-synchronous rounds that seed later arrivals level by level).
 //
 // Per-vertex state lives in part-major order (part p's vertices at
 // [vstart[p], vstart[p+1]), in the part's local order): `plevel`, the
@@ -24,7 +28,7 @@
 // levels other parts lowered its boundary vertices to (atomicMin).  A run
 // reads ilevel for the part's boundary vertices only; a workgroup that
 // runs the same part again keeps its LDS image when no other workgroup ran
-// the part in between (version counter `pver`).
+// the part in between (a version counter, kept in the part's state word).
 //
 // Part states: 0 idle, 1 queued, 2 running, 3 running and lowered again
 // (the runner then runs it once more).  Lowering a vertex from outside
@@ -43,9 +47,6 @@
 
 namespace phde {
 
-constexpr int kPbfsBlock = 1024;         // 16 waves: a round's frontier spreads over all of them
-constexpr int kExpandLanes = 16;         // lanes per frontier vertex in a round
-constexpr int kClaims = 1;               // neighbours a lane claims per step
 constexpr int kPartBytes = 144 * 1024;   // LDS image of one part (see part_bytes)
 constexpr uint32_t kNoPart = 0xffffffffu;
 // Control words: the ticket counter and the error flag share a line; the
@@ -63,30 +64,38 @@ __device__ __forceinline__ unsigned long long *tp_word(uint32_t *ctl) {
 
 // A part's static data is one blob, copied into LDS as it is: local row
 // offsets (nv + 1), local adjacency (ne), boundary list (nb), external
-// edges' local sources (nx) as uint16, then the external edges' targets
-// (part-major index) and target parts (nx each) as uint32; 16-byte aligned.
+// edges' local sources and target-part slots (nx each) as uint16, then the
+// external edges' targets (part-major index, nx) and the distinct
+// neighbouring parts (nn) as uint32; 16-byte aligned.
 __host__ __device__ constexpr int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
-__host__ __device__ constexpr int64_t blob_bytes(int64_t nv, int64_t ne, int64_t nb, int64_t nx) {
-  return align_up(align_up(2 * (nv + 1 + ne + nb + nx), 4) + 8 * nx, 16);
+__host__ __device__ constexpr int64_t blob_bytes(int64_t nv, int64_t ne, int64_t nb, int64_t nx, int64_t nn) {
+  return align_up(align_up(2 * (nv + 1 + ne + nb + 2 * nx), 4) + 4 * nx + 4 * nn, 16);
 }
 // LDS bytes of a part with nv vertices, ne internal and nx external
-// (directed) edges and nb boundary vertices: the blob, levels and seed
-// levels (int32), two frontier lists (nv + 1 entries: the source can be
-// seeded twice, as a boundary vertex and as the source) and the seed list
-// (uint16), changed flags.
-__host__ __device__ constexpr int64_t part_bytes(int64_t nv, int64_t ne, int64_t nb, int64_t nx) {
-  return blob_bytes(nv, ne, nb, nx) + 4 * nv + 4 * (nb + 1) + 4 * (nv + 1) + 2 * (nb + 1) + nv + 64;
+// (directed) edges, nb boundary vertices and nn neighbouring parts: the
+// blob, levels with their list stamps (uint64), two vertex lists (uint16,
+// nv + 1 entries), changed flags (nv) and lowered-neighbour flags (nn).
+__host__ __device__ constexpr int64_t part_bytes(int64_t nv, int64_t ne, int64_t nb, int64_t nx, int64_t nn) {
+  return blob_bytes(nv, ne, nb, nx, nn) + 8 * nv + 4 * (nv + 1) + nv + nn + 64;
 }
+// Rounds per run of a part (PHDE_PBFS_CAP): a run that reaches the cap
+// writes back and pushes what it has, then goes on in place from its list,
+// so that a part does not race ahead on levels that a neighbour is about to
+// lower.  Measured on RGG n=4.2M: 3.93 ms per traversal uncapped, 4.3 at 32
+// rounds, 4.5 at 16, 5.1 at 8 -- each extra run costs a write-back and push
+// (about 6 us) that the work it saves does not repay -- so runs are
+// uncapped by default.
+constexpr int kRoundCap = 1 << 30;
 
 struct PbfsArgs {
   DevGraph g;
   int32_t *level;   // output: the traversal's levels (internal ids)
   int32_t *plevel;  // part-major: the part's own levels
   int32_t *ilevel;  // part-major: levels lowered from outside
-  const uint32_t *vstart, *pmidx, *pcount;  // pcount: (ne, nb, nx) per part
+  const uint32_t *vstart, *pmidx, *pcount;  // pcount: (ne, nb, nx, nn) per part
   const uint4 *blob;
   const uint32_t *blob_off;  // per part, in 16-byte units
-  uint32_t *st, *pver, *ctl;
+  uint32_t *st, *ctl;  // st: version << 2 | state
   unsigned long long *ring;  // ring_n slots, kSlotStride words apart
   uint32_t ring_n, src_part, src_loc;
   int64_t k, src;
@@ -94,11 +103,12 @@ struct PbfsArgs {
   KcResult *kc;
   unsigned int *arrive, *bar_err;
   int backoff;
+  int cap;  // rounds per run of a part
   // with PHDE_PBFS_STATS: [0..3] per-phase wall time (100 MHz ticks) summed
   // over workgroups — waiting for a part, seeding (and loading images),
   // rounds, write-back and pushes; [4..6] activations, rounds, image loads;
-  // [7, 8] first and last tick; [9..11] expansion passes, jumps, seed scans;
-  // [13] frontier vertices; [14, 15] seedless activations, seeds;
+  // [7, 8] first and last tick; [9, 10] expansion passes, runs going on
+  // from the round cap; [13] listed vertices;
   // [16..19] queued-to-taken latency: sum, count, count above 10 us, max;
   // [32 + slot] the tick at which each ring slot was filled
   unsigned long long *stats;
@@ -129,7 +139,7 @@ __global__ void pbfs_init_kernel(PbfsArgs a) {
 // A vertex of part q was lowered from outside: queue q or flag its runner.
 __device__ __forceinline__ void mark_lowered(const PbfsArgs &a, uint32_t q) {
   const uint32_t old = atomicOr(&a.st[q], 1u);
-  if (old == 0u) {
+  if ((old & 3u) == 0u) {
     const uint32_t t = (uint32_t)(atomicAdd(tp_word(a.ctl), (1ull << 32) | 1ull) >> 32);
     if (a.stats) st_dev(&a.stats[32 + t % a.ring_n], __builtin_amdgcn_s_memrealtime());
     st_dev(&a.ring[(uint64_t)(t % a.ring_n) * kSlotStride], ((unsigned long long)(t + 1u) << 32) | (q + 1u));
@@ -195,17 +205,21 @@ __device__ __forceinline__ void list_append(bool found, uint16_t v, uint16_t *li
   if (found) list[base + __popcll(m & lanemask_lt())] = v;
 }
 
-__global__ __launch_bounds__(kPbfsBlock) void pbfs_kernel(PbfsArgs a) {
+template <int Block, int Lanes, int Claims>
+__global__ __launch_bounds__(Block) void pbfs_kernel(PbfsArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   __shared__ uint32_t s_part, s_ver;
-  __shared__ int s_cnt[3], s_ns[3], s_nseed, s_base, s_jns, s_rerun;
-  __shared__ int s_dummy[kPbfsBlock];  // targets of masked-off claims, one per thread
+  __shared__ int s_cnt[3], s_rerun;
+  __shared__ unsigned long long s_dummy[Block];  // targets of masked-off claims, one per thread
   const int tid = threadIdx.x;
   const int lane = tid % kWave;
   uint32_t cached = kNoPart, cached_ver = 0;  // the part whose image is in LDS, and its version
+  // Rounds so far in this workgroup: each round's list is stamped with its
+  // own epoch, so that a vertex joins it at most once.
+  uint32_t epoch = 0;
   unsigned long long t_wait = 0, t_load = 0, t_round = 0, t_push = 0, n_act = 0, n_round = 0, n_img = 0;
   unsigned long long t_first = ~0ull, t_last = 0, t0 = 0, t1 = 0;
-  unsigned long long n_pass = 0, n_jump = 0, n_scan = 0, n_front = 0;
+  unsigned long long n_pass = 0, n_carry = 0, n_front = 0;
   const bool stats = a.stats != nullptr;
   for (;;) {
     if (stats && tid == 0) t0 = __builtin_amdgcn_s_memrealtime();
@@ -216,8 +230,7 @@ __global__ __launch_bounds__(kPbfsBlock) void pbfs_kernel(PbfsArgs a) {
         // datum shared between workgroups is a device-scope access, so
         // completing this wave's own accesses orders them; no cache
         // write-back is needed)
-        atomicExch(&a.st[p], 2u);
-        s_ver = ld_dev(&a.pver[p]);
+        s_ver = atomicAdd(&a.st[p], 1u) >> 2;  // queued -> running; the part's version
         __builtin_amdgcn_s_waitcnt(0);
       }
       s_part = p;
@@ -232,191 +245,153 @@ __global__ __launch_bounds__(kPbfsBlock) void pbfs_kernel(PbfsArgs a) {
     if (p == kNoPart) break;
     const uint32_t v0 = a.vstart[p];
     const int nv = (int)(a.vstart[p + 1] - v0);
-    const int ne = (int)a.pcount[3 * p], nb = (int)a.pcount[3 * p + 1], nx = (int)a.pcount[3 * p + 2];
+    const int ne = (int)a.pcount[4 * p], nb = (int)a.pcount[4 * p + 1], nx = (int)a.pcount[4 * p + 2];
+    const int nn = (int)a.pcount[4 * p + 3];
     const int nblob = (int)(a.blob_off[p + 1] - a.blob_off[p]);  // 16-byte words
     // LDS: the blob, then the per-run state
     uint16_t *lrow = reinterpret_cast<uint16_t *>(smem);
     uint16_t *ladj = lrow + nv + 1;
     uint16_t *blist = ladj + ne;
     uint16_t *xsrc = blist + nb;
-    const uint32_t *xdst = reinterpret_cast<const uint32_t *>(smem + align_up(2 * (nv + 1 + ne + nb + nx), 4));
-    const uint32_t *xpart = xdst + nx;
-    int32_t *lvl = reinterpret_cast<int32_t *>(smem + 16 * (int64_t)nblob);
-    int32_t *slev = lvl + nv;
-    uint16_t *F0 = reinterpret_cast<uint16_t *>(slev + nb + 1);
+    uint16_t *xnbr = xsrc + nx;
+    const uint32_t *xdst = reinterpret_cast<const uint32_t *>(smem + align_up(2 * (nv + 1 + ne + nb + 2 * nx), 4));
+    const uint32_t *nbr = xdst + nx;
+    // level << 32 | epoch of the list the vertex last joined
+    unsigned long long *lvl = reinterpret_cast<unsigned long long *>(smem + 16 * (int64_t)nblob);
+    uint16_t *F0 = reinterpret_cast<uint16_t *>(lvl + nv);
     uint16_t *F1 = F0 + nv + 1;
-    uint16_t *seeds = F1 + nv + 1;
-    unsigned char *chg = reinterpret_cast<unsigned char *>(seeds + nb + 1);
+    unsigned char *chg = reinterpret_cast<unsigned char *>(F1 + nv + 1);
+    unsigned char *nflag = chg + nv;
+    // the list being built (F0 / F1) and its counter (s_cnt[ci]); a run that
+    // stops at the round cap leaves its list to the next run of the part
+    int li = 0, ci = 0;
+    bool carry = false;
     for (;;) {
       if (stats && tid == 0) {
         t0 = __builtin_amdgcn_s_memrealtime();
         n_act++;
+        n_carry += carry ? 1 : 0;
       }
       if (cached != p || cached_ver != s_ver) {
-        // the part's image: its blob (read-only, 16-byte loads, four in
+        // the part's image: its blob (read-only, 16-byte loads, eight in
         // flight per thread) and its own levels (device-scope loads)
         const uint4 *gb = a.blob + a.blob_off[p];
         uint4 *sb = reinterpret_cast<uint4 *>(smem);
-        for (int i = tid; i < nblob; i += 8 * kPbfsBlock) {
+        for (int i = tid; i < nblob; i += 8 * Block) {
           uint4 t[8];
 #pragma unroll
-          for (int j = 0; j < 8; j++) t[j] = gb[i + j * kPbfsBlock < nblob ? i + j * kPbfsBlock : i];
+          for (int j = 0; j < 8; j++) t[j] = gb[i + j * Block < nblob ? i + j * Block : i];
 #pragma unroll
           for (int j = 0; j < 8; j++)
-            if (i + j * kPbfsBlock < nblob) sb[i + j * kPbfsBlock] = t[j];
+            if (i + j * Block < nblob) sb[i + j * Block] = t[j];
         }
-        for (int i0 = 0; i0 < nv; i0 += 8 * kPbfsBlock) {
+        for (int i0 = 0; i0 < nv; i0 += 8 * Block) {
           int32_t t[8];
 #pragma unroll
           for (int j = 0; j < 8; j++) {
-            const int i = i0 + j * kPbfsBlock + tid;
+            const int i = i0 + j * Block + tid;
             t[j] = i < nv ? ld_dev(&a.plevel[v0 + i]) : 0;
           }
 #pragma unroll
           for (int j = 0; j < 8; j++) {
-            const int i = i0 + j * kPbfsBlock + tid;
-            if (i < nv) lvl[i] = t[j];
+            const int i = i0 + j * Block + tid;
+            if (i < nv) lvl[i] = (unsigned long long)(uint32_t)t[j] << 32;
           }
         }
         if (stats && tid == 0) n_img++;
       }
-      for (int i = tid; i < nv; i += kPbfsBlock) chg[i] = 0;
-      if (tid == 0) {
-        s_nseed = 0;
-        s_base = kUnvisited;
-        s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
-        s_ns[0] = s_ns[1] = s_ns[2] = kUnvisited;
-        s_jns = kUnvisited;
+      for (int i = tid; i < nv; i += Block) chg[i] = 0;
+      for (int i = tid; i < nn; i += Block) nflag[i] = 0;
+      if (!carry) {
+        // a new list: stamped with a new epoch
+        epoch++;
+        li = 0;
+        ci = 0;
+        if (tid == 0) s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
       }
       __syncthreads();
-      // seeds: boundary vertices (and the source) lowered from outside
-      int mybase = kUnvisited;
-      for (int bb = 0; bb <= nb; bb += kPbfsBlock) {
+      // seeds: boundary vertices (and the source) lowered from outside join
+      // the list, after any vertices carried over from the last run
+      uint16_t *Fs = li ? F1 : F0;
+      for (int bb = 0; bb <= nb; bb += Block) {
         const int b = bb + tid;
         int u = -1;
         if (b < nb)
           u = blist[b];
         else if (b == nb && p == a.src_part)
           u = (int)a.src_loc;
-        bool sd = false;
-        int32_t l = kUnvisited;
+        bool add = false;
         if (u >= 0) {
-          l = ld_dev(&a.ilevel[v0 + u]);
-          sd = l < lvl[u];
-        }
-        const unsigned long long m = __ballot(sd);
-        if (m) {
-          const int leader = __ffsll((long long)m) - 1;
-          int base = 0;
-          if (lane == leader) base = atomicAdd(&s_nseed, __popcll(m));
-          if (stats && lane == leader) atomicAdd(&a.stats[15], (unsigned long long)__popcll(m));
-          base = __shfl(base, leader);
-          if (sd) {
-            const int k = base + __popcll(m & lanemask_lt());
-            seeds[k] = (uint16_t)u;
-            slev[k] = l;
-            lvl[u] = l;
-            chg[u] = 1;
-            mybase = min(mybase, l);
+          const int32_t l = ld_dev(&a.ilevel[v0 + u]);
+          if ((uint32_t)l < (uint32_t)(lvl[u] >> 32)) {
+            const unsigned long long old = atomicMin(&lvl[u], ((unsigned long long)(uint32_t)l << 32) | epoch);
+            if ((uint32_t)(old >> 32) > (uint32_t)l) {
+              chg[u] = 1;
+              add = (uint32_t)old != epoch;
+            }
           }
         }
+        list_append(add, add ? (uint16_t)u : 0, Fs, &s_cnt[ci]);
       }
-      mybase = -wave_max_i32(-mybase);
-      if (lane == 0 && mybase < kUnvisited) atomicMin(&s_base, mybase);
       __syncthreads();
-      const int nseed = s_nseed;
-      if (stats && tid == 0 && nseed == 0) atomicAdd(&a.stats[14], 1ull);
-      int cur = s_base;
-      int r = 0, ns = kUnvisited;
-      // Seeds at level L join the frontier list `list` (count *cnt); the
-      // lowest level of a seed still waiting above L goes to *next.
-      auto seed_scan = [&](int L, uint16_t *list, int *cnt, int *next) {
-        int mn = kUnvisited;
-        for (int kk = 0; kk < nseed; kk += kPbfsBlock) {
-          const int k = kk + tid;
-          bool f = false;
-          if (k < nseed) {
-            const int sl = slev[k];
-            const bool valid = lvl[seeds[k]] == sl;
-            f = valid && sl == L;
-            if (valid && sl > L) mn = min(mn, sl);
-          }
-          list_append(f, f ? seeds[k] : 0, list, cnt);
-        }
-        mn = -wave_max_i32(-mn);
-        if (lane == 0 && mn < kUnvisited) atomicMin(next, mn);
-      };
-      if (nseed > 0) {
-        seed_scan(cur, F0, &s_cnt[0], &s_jns);
-        __syncthreads();
-        ns = s_jns;
-      }
       if (stats && tid == 0) {
         t1 = __builtin_amdgcn_s_memrealtime();
         t_load += t1 - t0;
         t0 = t1;
       }
-      while (nseed > 0) {
-        // round r: the frontier at level cur (list r & 1, count s_cnt[r % 3])
-        // expands into level cur + 1 (list (r + 1) & 1, count s_cnt[(r + 1) % 3])
-        const int nf = s_cnt[r % 3];
-        if (tid == 0) {
-          s_cnt[(r + 2) % 3] = 0;
-          s_ns[(r + 2) % 3] = kUnvisited;
-        }
-        uint16_t *Fc = (r & 1) ? F1 : F0;
-        if (nf == 0) {
-          // nothing at level cur: jump to the lowest seed still waiting
-          if (ns == kUnvisited) break;
-          if (stats && tid == 0) n_jump++;
-          cur = ns;
-          if (tid == 0) s_jns = kUnvisited;
-          __syncthreads();
-          seed_scan(cur, Fc, &s_cnt[r % 3], &s_jns);
-          __syncthreads();
-          ns = s_jns;
-          continue;
-        }
-        const int nl = cur + 1;
+      // Rounds: every listed vertex offers its own level + 1 to its
+      // neighbours (label-correcting: the list may hold several levels);
+      // a neighbour whose level drops joins the next list once.
+      int nf = s_cnt[ci], r = 0;
+      while (nf > 0 && r < a.cap) {
+        const uint32_t en = epoch + 1u;  // the stamp of the list this round builds
+        if (tid == 0) s_cnt[(ci + 2) % 3] = 0;
+        const uint16_t *Fc = li ? F1 : F0;
+        uint16_t *Fn = li ? F0 : F1;
+        int *cn = &s_cnt[(ci + 1) % 3];
         if (stats && tid == 0) {
-          n_pass += (nf + kPbfsBlock / kExpandLanes - 1) / (kPbfsBlock / kExpandLanes);
+          n_pass += (nf + Block / Lanes - 1) / (Block / Lanes);
           n_front += nf;
         }
-        uint16_t *Fn = (r & 1) ? F0 : F1;
-        int *cn = &s_cnt[(r + 1) % 3];
-        // kExpandLanes lanes per frontier vertex; a lane claims up to
-        // kClaims neighbours (all reads and claims issued before any result
-        // is used), then the wave appends its winners with one atomic
-        for (int f0 = 0; f0 < nf; f0 += kPbfsBlock / kExpandLanes) {
-          const int f = f0 + tid / kExpandLanes;
+        // Lanes lanes per listed vertex; a lane claims up to Claims
+        // neighbours (all reads and claims issued before any result is
+        // used), then the wave appends its new entries with one atomic
+        for (int f0 = 0; f0 < nf; f0 += Block / Lanes) {
+          const int f = f0 + tid / Lanes;
           int xb = 0, e = 0;
+          uint32_t nl = 0;
           if (f < nf) {
             const int u = Fc[f];
             xb = lrow[u];
             e = lrow[u + 1];
+            nl = (uint32_t)(lvl[u] >> 32) + 1u;
           }
-          xb += tid % kExpandLanes;
+          xb += tid % Lanes;
+          const unsigned long long offer = ((unsigned long long)nl << 32) | en;
           while (__ballot(xb < e)) {
             // unconditional reads and claims (clamped index, masked-off
             // lanes claim their own dummy word), all issued before any
             // result is used
-            int w[kClaims], old[kClaims];
-            bool ok[kClaims];
+            int w[Claims];
+            unsigned long long old[Claims];
+            bool ok[Claims];
 #pragma unroll
-            for (int q = 0; q < kClaims; q++) {
-              const int x = xb + q * kExpandLanes;
+            for (int q = 0; q < Claims; q++) {
+              const int x = xb + q * Lanes;
               ok[q] = x < e;
               w[q] = ladj[ok[q] ? x : 0];
             }
 #pragma unroll
-            for (int q = 0; q < kClaims; q++) old[q] = atomicMin(ok[q] ? &lvl[w[q]] : &s_dummy[tid], nl);
-            // positions from ballots: winner q of a lane goes after every
-            // winner q' < q of the wave and after winners q of lower lanes
-            unsigned long long mq[kClaims];
-            int pre[kClaims], total = 0;
+            for (int q = 0; q < Claims; q++) old[q] = atomicMin(ok[q] ? &lvl[w[q]] : &s_dummy[tid], offer);
+            // positions from ballots: new entry q of a lane goes after every
+            // new entry q' < q of the wave and after entries q of lower lanes
+            unsigned long long mq[Claims];
+            int pre[Claims], total = 0;
 #pragma unroll
-            for (int q = 0; q < kClaims; q++) {
-              mq[q] = __ballot(ok[q] && old[q] > nl);
+            for (int q = 0; q < Claims; q++) {
+              const bool low = ok[q] && (uint32_t)(old[q] >> 32) > nl;  // the level dropped
+              if (low) chg[w[q]] = 1;
+              mq[q] = __ballot(low && (uint32_t)old[q] != en);
               pre[q] = total;
               total += __popcll(mq[q]);
             }
@@ -426,22 +401,18 @@ __global__ __launch_bounds__(kPbfsBlock) void pbfs_kernel(PbfsArgs a) {
               base = __builtin_amdgcn_readfirstlane(base);
               const unsigned long long lt = lanemask_lt();
 #pragma unroll
-              for (int q = 0; q < kClaims; q++)
-                if ((mq[q] >> lane) & 1ull) {
-                  chg[w[q]] = 1;
-                  Fn[base + pre[q] + __popcll(mq[q] & lt)] = (uint16_t)w[q];
-                }
+              for (int q = 0; q < Claims; q++)
+                if ((mq[q] >> lane) & 1ull) Fn[base + pre[q] + __popcll(mq[q] & lt)] = (uint16_t)w[q];
             }
-            xb += kClaims * kExpandLanes;
+            xb += Claims * Lanes;
           }
         }
-        const bool scan = nl == ns;
-        if (scan) seed_scan(nl, Fn, cn, &s_ns[(r + 1) % 3]);
-        if (stats && tid == 0) n_scan += scan ? 1 : 0;
         __syncthreads();
-        if (scan) ns = s_ns[(r + 1) % 3];
+        epoch = en;
+        li ^= 1;
+        ci = (ci + 1) % 3;
+        nf = s_cnt[ci];
         r++;
-        cur = nl;
       }
       if (stats && tid == 0) {
         t1 = __builtin_amdgcn_s_memrealtime();
@@ -450,31 +421,35 @@ __global__ __launch_bounds__(kPbfsBlock) void pbfs_kernel(PbfsArgs a) {
         n_round += r;
       }
       // write back the lowered levels, then lower the neighbours in other
-      // parts (one external edge per thread)
-      for (int i = tid; i < nv; i += kPbfsBlock)
-        if (chg[i]) st_dev(&a.plevel[v0 + i], lvl[i]);
-      for (int x0 = 0; x0 < nx; x0 += 4 * kPbfsBlock) {
-        // up to four external edges per thread, their claims in flight together
+      // parts (up to four external edges per thread, their claims in flight
+      // together) and flag their parts; one thread per flagged part then
+      // queues it (or flags its runner)
+      for (int i = tid; i < nv; i += Block)
+        if (chg[i]) st_dev(&a.plevel[v0 + i], (int32_t)(lvl[i] >> 32));
+      for (int x0 = 0; x0 < nx; x0 += 4 * Block) {
         int32_t nl[4], old[4];
         bool on[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-          const int x = x0 + j * kPbfsBlock + tid;
+          const int x = x0 + j * Block + tid;
           on[j] = false;
           nl[j] = 0;
           if (x < nx) {
             const int u = xsrc[x];
             on[j] = chg[u] != 0;
-            nl[j] = lvl[u] + 1;
+            nl[j] = (int32_t)(lvl[u] >> 32) + 1;
           }
         }
 #pragma unroll
         for (int j = 0; j < 4; j++)
-          old[j] = on[j] ? atomicMin(&a.ilevel[xdst[x0 + j * kPbfsBlock + tid]], nl[j]) : 0;
+          old[j] = on[j] ? atomicMin(&a.ilevel[xdst[x0 + j * Block + tid]], nl[j]) : 0;
 #pragma unroll
         for (int j = 0; j < 4; j++)
-          if (on[j] && old[j] > nl[j]) mark_lowered(a, xpart[x0 + j * kPbfsBlock + tid]);
+          if (on[j] && old[j] > nl[j]) nflag[xnbr[x0 + j * Block + tid]] = 1;
       }
+      __syncthreads();
+      for (int i = tid; i < nn; i += Block)
+        if (nflag[i]) mark_lowered(a, nbr[i]);
       // this run's stores and claims complete before the part is released
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
@@ -484,27 +459,35 @@ __global__ __launch_bounds__(kPbfsBlock) void pbfs_kernel(PbfsArgs a) {
         if (t1 > t_last) t_last = t1;
       }
       if (tid == 0) {
-        // this run's image becomes the part's current version (only the
-        // part's runner changes it)
-        s_ver = s_ver + 1u;
-        st_dev(&a.pver[p], s_ver);
-        __builtin_amdgcn_s_waitcnt(0);  // the version lands before the part can be taken again
-        const uint32_t old = atomicCAS(&a.st[p], 2u, 0u);
-        if (old == 2u) {
-          s_rerun = 0;
-          const unsigned long long tp = atomicSub(tp_word(a.ctl), 1ull);
-          if ((uint32_t)tp == 1u) mark_done(a, (uint32_t)(tp >> 32));  // the last part: the search is over
-        } else {
-          // lowered again while it ran: run it once more
-          atomicExch(&a.st[p], 2u);
+        // this run's image becomes the part's next version, published with
+        // the state (only the part's runner changes the version)
+        const uint32_t v = s_ver;
+        s_ver = v + 1u;
+        if (nf > 0) {
+          // stopped at the round cap: go on in place with the list; a
+          // lowering from now on flags the part again
+          atomicExch(&a.st[p], (v + 1u) << 2 | 2u);
           __builtin_amdgcn_s_waitcnt(0);
-          s_rerun = 1;
+          s_rerun = 2;
+        } else {
+          const uint32_t old = atomicCAS(&a.st[p], v << 2 | 2u, (v + 1u) << 2);
+          if (old == (v << 2 | 2u)) {
+            s_rerun = 0;
+            const unsigned long long tp = atomicSub(tp_word(a.ctl), 1ull);
+            if ((uint32_t)tp == 1u) mark_done(a, (uint32_t)(tp >> 32));  // the last part: the search is over
+          } else {
+            // lowered again while it ran: run it once more
+            atomicExch(&a.st[p], (v + 1u) << 2 | 2u);
+            __builtin_amdgcn_s_waitcnt(0);
+            s_rerun = 1;
+          }
         }
       }
       __syncthreads();
       cached = p;
       cached_ver = s_ver;
       if (!s_rerun) break;
+      carry = s_rerun == 2;
     }
   }
   if (stats && tid == 0) {
@@ -518,8 +501,7 @@ __global__ __launch_bounds__(kPbfsBlock) void pbfs_kernel(PbfsArgs a) {
     atomicMin(&a.stats[7], t_first);
     atomicMax(&a.stats[8], t_last);
     atomicAdd(&a.stats[9], n_pass);
-    atomicAdd(&a.stats[10], n_jump);
-    atomicAdd(&a.stats[11], n_scan);
+    atomicAdd(&a.stats[10], n_carry);
     atomicAdd(&a.stats[13], n_front);
   }
   // every workgroup has stopped taking parts; the levels are final unless
@@ -572,6 +554,18 @@ __global__ __launch_bounds__(kPbfsBlock) void pbfs_kernel(PbfsArgs a) {
 
 // ------------------------------------------------------------------ host side
 
+// Workgroup shapes: threads, lanes per frontier vertex, claims per lane.
+struct PbfsShape {
+  int block, lanes, claims;
+  void (*fn)(PbfsArgs);
+};
+static const PbfsShape kShapes[] = {
+    {1024, 16, 1, pbfs_kernel<1024, 16, 1>}, {512, 8, 2, pbfs_kernel<512, 8, 2>},
+    {512, 16, 1, pbfs_kernel<512, 16, 1>},   {256, 4, 3, pbfs_kernel<256, 4, 3>},
+    {256, 8, 2, pbfs_kernel<256, 8, 2>},     {512, 4, 3, pbfs_kernel<512, 4, 3>},
+    {256, 16, 1, pbfs_kernel<256, 16, 1>},
+};
+
 // The parts of a deep graph (device arrays) and the traversal's state.
 struct PartGraph {
   int64_t k = 0;
@@ -579,10 +573,11 @@ struct PartGraph {
   uint4 *blob = nullptr;
   std::vector<uint32_t> h_part_of, h_loc;
   int32_t *plevel = nullptr, *ilevel = nullptr;
-  uint32_t *st = nullptr, *pver = nullptr, *ctl = nullptr;
+  uint32_t *st = nullptr, *ctl = nullptr;
   unsigned long long *ring = nullptr;
   uint32_t ring_n = 0;
   bool disabled = false;
+  PbfsShape shape;
   unsigned long long *stats = nullptr;  // PHDE_PBFS_STATS
 };
 
@@ -590,7 +585,7 @@ void pbfs_release(phde_ctx *c) {
   PartGraph *pg = c->pg;
   if (!pg) return;
   void *ptr[] = {pg->vstart, pg->pmidx, pg->pcount, pg->blob_off, pg->blob, pg->plevel, pg->ilevel,
-                 pg->st,     pg->pver,  pg->ring,   pg->ctl,      pg->stats};
+                 pg->st,     pg->ring,  pg->ctl,    pg->stats};
   for (void *x : ptr)
     if (x) (void)hipFree(x);
   delete pg;
@@ -621,9 +616,12 @@ int pbfs_setup(phde_ctx *c) {
   // part budget in LDS bytes (smaller budgets: more parts, for tests)
   int64_t budget = kPartBytes;
   if (const char *e = getenv("PHDE_PBFS_BYTES")) budget = std::max<int64_t>(128, std::min<int64_t>(kPartBytes, atoll(e)));
+  PbfsShape shape = kShapes[0];
+  if (const char *e = getenv("PHDE_PBFS_SHAPE"))
+    shape = kShapes[std::max(0, std::min((int)(sizeof(kShapes) / sizeof(kShapes[0])) - 1, atoi(e)))];
   int resident = 0;
-  PHDE_HIP(hipFuncSetAttribute((const void *)pbfs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, kPartBytes));
-  PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, pbfs_kernel, kPbfsBlock, kPartBytes));
+  PHDE_HIP(hipFuncSetAttribute((const void *)shape.fn, hipFuncAttributeMaxDynamicSharedMemorySize, kPartBytes));
+  PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, shape.fn, shape.block, kPartBytes));
   if (resident < 1) return PHDE_OK;
   std::vector<uint32_t> rp(n + 1), adj(std::max<int64_t>(m, 1));
   PHDE_HIP(hipMemcpy(rp.data(), c->g.rowptr, sizeof(uint32_t) * (n + 1), hipMemcpyDeviceToHost));
@@ -643,7 +641,7 @@ int pbfs_setup(phde_ctx *c) {
       if (adj[x] == (uint32_t)s) ne++;  // a self loop stays inside
       else nx++;
     }
-    if (part_bytes(nv, ne, nv, nx) > budget || ne + nx > 65535) return PHDE_OK;  // a vertex too wide for one part
+    if (part_bytes(nv, ne, nv, nx, nx) > budget || ne + nx > 65535) return PHDE_OK;  // a vertex too wide for one part
     for (size_t h = 0; h < q.size(); h++) {
       const uint32_t u = q[h];
       for (uint32_t x = rp[u]; x < rp[u + 1]; x++) {
@@ -658,7 +656,7 @@ int pbfs_setup(phde_ctx *c) {
         }
         const int64_t ne2 = ne + 2 * in, nx2 = nx - in + out;
         // 16-bit local ids and row offsets
-        if (part_bytes(nv + 1, ne2, nv + 1, nx2) > budget || nv + 1 > 65535 || ne2 + nv + 1 > 65535) continue;
+        if (part_bytes(nv + 1, ne2, nv + 1, nx2, nx2) > budget || nv + 1 > 65535 || ne2 + nv + 1 > 65535) continue;
         nv++;
         ne = ne2;
         nx = nx2;
@@ -680,7 +678,8 @@ int pbfs_setup(phde_ctx *c) {
   for (int64_t u = 0; u < n; u++)
     for (uint32_t x = rp[u]; x < rp[u + 1]; x++)
       if (part_of[adj[x]] != part_of[u]) boundary[adj[x]] = 1;
-  std::vector<uint32_t> pcount(3 * k), blob_off{0};
+  std::vector<uint32_t> pcount(4 * k), blob_off{0};
+  std::vector<int32_t> slot_of(k, -1);  // a part's slot in the current part's neighbour list
   std::vector<uint16_t> b16;   // one part's uint16 section
   std::vector<uint32_t> b32;   // and its uint32 section
   std::vector<uint4> blob;
@@ -688,8 +687,8 @@ int pbfs_setup(phde_ctx *c) {
   int64_t mn = n, mx = 0, worst = 0, next = 0, nbtot = 0;
   for (int64_t p = 0; p < k; p++) {
     const uint32_t v0 = vstart[p], nv = vstart[p + 1] - v0;
-    std::vector<uint16_t> lrow, ladj, blist, xsrc;
-    std::vector<uint32_t> xdst, xpart;
+    std::vector<uint16_t> lrow, ladj, blist, xsrc, xnbr;
+    std::vector<uint32_t> xdst, nbr;
     for (uint32_t i = 0; i < nv; i++) {
       const uint32_t u = verts[v0 + i];
       lrow.push_back((uint16_t)ladj.size());
@@ -700,26 +699,34 @@ int pbfs_setup(phde_ctx *c) {
         } else {
           xsrc.push_back((uint16_t)i);
           xdst.push_back(pmidx[w]);
-          xpart.push_back(part_of[w]);
+          int32_t &slot = slot_of[part_of[w]];
+          if (slot < 0) {
+            slot = (int32_t)nbr.size();
+            nbr.push_back(part_of[w]);
+          }
+          xnbr.push_back((uint16_t)slot);
         }
       }
       if (boundary[u]) blist.push_back((uint16_t)i);
     }
     lrow.push_back((uint16_t)ladj.size());
-    const int64_t ne = ladj.size(), nb = blist.size(), nx = xsrc.size();
-    pcount[3 * p] = (uint32_t)ne;
-    pcount[3 * p + 1] = (uint32_t)nb;
-    pcount[3 * p + 2] = (uint32_t)nx;
+    for (uint32_t q : nbr) slot_of[q] = -1;
+    const int64_t ne = ladj.size(), nb = blist.size(), nx = xsrc.size(), nn = nbr.size();
+    pcount[4 * p] = (uint32_t)ne;
+    pcount[4 * p + 1] = (uint32_t)nb;
+    pcount[4 * p + 2] = (uint32_t)nx;
+    pcount[4 * p + 3] = (uint32_t)nn;
     b16.assign(lrow.begin(), lrow.end());
     b16.insert(b16.end(), ladj.begin(), ladj.end());
     b16.insert(b16.end(), blist.begin(), blist.end());
     b16.insert(b16.end(), xsrc.begin(), xsrc.end());
-    const int64_t bytes = blob_bytes(nv, ne, nb, nx);
+    b16.insert(b16.end(), xnbr.begin(), xnbr.end());
+    const int64_t bytes = blob_bytes(nv, ne, nb, nx, nn);
     std::vector<unsigned char> raw(bytes, 0);
     std::memcpy(raw.data(), b16.data(), 2 * b16.size());
     const int64_t o32 = align_up(2 * (int64_t)b16.size(), 4);
     std::memcpy(raw.data() + o32, xdst.data(), 4 * nx);
-    std::memcpy(raw.data() + o32 + 4 * nx, xpart.data(), 4 * nx);
+    std::memcpy(raw.data() + o32 + 4 * nx, nbr.data(), 4 * nn);
     const size_t at = blob.size();
     blob.resize(at + bytes / 16);
     std::memcpy(&blob[at], raw.data(), bytes);
@@ -728,13 +735,14 @@ int pbfs_setup(phde_ctx *c) {
     blob_off.push_back((uint32_t)next);
     mn = std::min<int64_t>(mn, nv);
     mx = std::max<int64_t>(mx, nv);
-    worst = std::max(worst, part_bytes(nv, ne, nb, nx));
+    worst = std::max(worst, part_bytes(nv, ne, nb, nx, nn));
     nbtot += nb;
   }
   if (worst > kPartBytes) return PHDE_OK;  // cannot happen: the growth counted every vertex as boundary
   PartGraph *pg = new PartGraph();
   c->pg = pg;
   pg->k = k;
+  pg->shape = shape;
   pg->ring_n = (uint32_t)(k + std::max(1, c->num_cus) + 64);
   PHDE_TRY_RC(upload(&pg->vstart, vstart));
   PHDE_TRY_RC(upload(&pg->pmidx, pmidx));
@@ -744,8 +752,6 @@ int pbfs_setup(phde_ctx *c) {
   PHDE_HIP(hipMalloc(&pg->plevel, sizeof(int32_t) * n));
   PHDE_HIP(hipMalloc(&pg->ilevel, sizeof(int32_t) * n));
   PHDE_HIP(hipMalloc(&pg->st, sizeof(uint32_t) * k));
-  PHDE_HIP(hipMalloc(&pg->pver, sizeof(uint32_t) * k));
-  PHDE_HIP(hipMemset(pg->pver, 0, sizeof(uint32_t) * k));
   PHDE_HIP(hipMalloc(&pg->ring, sizeof(unsigned long long) * kSlotStride * pg->ring_n));
   PHDE_HIP(hipMalloc(&pg->ctl, sizeof(uint32_t) * kCtlWords));
   if (getenv("PHDE_PBFS_STATS")) {
@@ -778,7 +784,6 @@ int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, boo
   a.blob = pg->blob;
   a.blob_off = pg->blob_off;
   a.st = pg->st;
-  a.pver = pg->pver;
   a.ring = pg->ring;
   a.ctl = pg->ctl;
   a.ring_n = pg->ring_n;
@@ -791,6 +796,8 @@ int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, boo
   a.arrive = c->d_bar;
   a.bar_err = c->d_bar + 1;
   a.backoff = c->bar_backoff;
+  a.cap = kRoundCap;
+  if (const char *e = getenv("PHDE_PBFS_CAP")) a.cap = std::max(1, atoi(e));
   a.stats = pg->stats;
   if (pg->stats) {
     const unsigned long long init[32] = {0, 0, 0, 0, 0, 0, 0, ~0ull, 0};
@@ -805,8 +812,8 @@ int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, boo
     return PHDE_OK;
   }
   void *args[] = {&a};
-  const hipError_t e = hipLaunchCooperativeKernel((const void *)pbfs_kernel, dim3(grid),
-                                                  dim3(kPbfsBlock), args, kPartBytes, c->stream);
+  const hipError_t e = hipLaunchCooperativeKernel((const void *)pg->shape.fn, dim3(grid),
+                                                  dim3(pg->shape.block), args, kPartBytes, c->stream);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     pg->disabled = true;
@@ -838,9 +845,8 @@ int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, boo
                  (long long)src, (s[8] - s[7]) * us, s[0] * us, s[1] * us, s[2] * us, s[3] * us, s[4], s[5],
                  s[5] / na, s[6], s[1] * us / na, s[2] * us / na, s[3] * us / na,
                  s[2] * us / (double)std::max(1ull, s[5]));
-    std::fprintf(stderr, "[pbfs]   %llu passes, %llu jumps, %llu seed scans, %.1f frontier vertices per round, %llu seedless "
-                 "activations, %.1f seeds per activation\n", s[9], s[10], s[11],
-                 (double)s[13] / (double)std::max(1ull, s[5]), s[14], s[15] / na);
+    std::fprintf(stderr, "[pbfs]   %llu passes, %llu runs going on from the round cap, %.1f listed vertices per round\n",
+                 s[9], s[10], (double)s[13] / (double)std::max(1ull, s[5]));
   }
   *ran = true;
   return PHDE_OK;
