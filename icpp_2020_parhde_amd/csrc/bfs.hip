@@ -20,7 +20,7 @@
 // a bitmap.  Queue appends are wave-aggregated (one atomic per wave).
 // Vertices of degree >= kHugeDegree are expanded by the whole grid, those of
 // degree >= kBigDegree by one wave, the rest one lane each.  The same levels
-// run in one cooperative launch with a grid barrier between them
+// run in one persistent launch (every block resident) with a grid barrier between them
 // (bfs_persistent_kernel), and graphs with a deep BFS take the multi-hop
 // traversal (bfs_hops_kernel: several hops per barrier, label-correcting,
 // exact levels).
@@ -545,7 +545,7 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_level_kernel(BfsArgs a, int L) 
   }
 }
 
-// The whole traversal in one cooperative launch: levels separated by the
+// The whole traversal in one persistent launch (every block resident): levels separated by the
 // grid barrier above.  *arrive must be 0 at launch; *err receives 1 if a
 // barrier wait timed out (a block whose wait timed out leaves the loop, the
 // others then time out too, so the kernel always ends).
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_persistent_kernel(BfsArgs a, un
 // queue holds, *overflow is set and the host repeats the traversal level by
 // level.
 
-// The multi-hop traversal in one cooperative launch.  *arrive must be 0 at
+// The multi-hop traversal in one persistent launch.  *arrive must be 0 at
 // launch; *err receives 1 if a barrier wait timed out.
 __global__ __launch_bounds__(kBfsBlock) void bfs_hops_kernel(BfsArgs a, unsigned int *arrive, unsigned int *err) {
   if (blockIdx.x % a.bstride) return;
@@ -772,13 +772,11 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
     PHDE_TRY_RC(fill_u32(c, c->d_bar, 3, 0u));
     unsigned int *arrive = c->d_bar, *err = c->d_bar + 1;
     void *args[] = {&a, &arrive, &err};
-    const hipError_t e =
-        hipLaunchCooperativeKernel(hops ? (const void *)bfs_hops_kernel : (const void *)bfs_persistent_kernel,
-                                   dim3(grid), dim3(kBfsBlock), args, 0, c->stream);
+    const hipError_t e = launch_resident(hops ? (const void *)bfs_hops_kernel : (const void *)bfs_persistent_kernel,
+                                         grid, kBfsBlock, args, 0, c->stream);
     if (e != hipSuccess) {
-      // cooperative launches can be unavailable (measured: in a second
-      // process sharing the GPU); this context then launches one kernel
-      // per level, which needs no co-residency
+      // this context then launches one kernel per level, which needs no
+      // co-residency
       (void)hipGetLastError();
       c->bfs_persistent = false;
       a.bstride = 1;
@@ -803,8 +801,13 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
                               c->stream));
     PHDE_HIP(hipStreamSynchronize(c->stream));
     if (c->bfs_persistent && c->h_bar[0]) {
-      set_error("BFS grid barrier timed out (blocks not co-resident?)");
-      return PHDE_ERR_DEVICE;
+      // the grid barrier timed out (blocks kept from becoming resident):
+      // this context goes on level by level.  Every level the partial run
+      // wrote is the length of a real path, so min_dist ends the same
+      // after the exact pass.
+      std::fprintf(stderr, "parhde: BFS grid barrier timed out; traversing level by level from now on\n");
+      c->bfs_persistent = false;
+      return bfs_run(c, src, alpha, beta, dist_out, min_dist, sync_result);
     }
     if (hops && c->h_bar[1]) {
       // the multi-hop traversal outgrew its queue: repeat it level by level.

@@ -79,7 +79,7 @@ __device__ __forceinline__ void st_dev(unsigned int *p, unsigned int v) {
 __device__ __forceinline__ void st_dev(unsigned long long *p, unsigned long long v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// Grid barrier for co-resident blocks (cooperative launch): every block
+// Grid barrier for co-resident blocks (a persistent launch): every block
 // adds one to a monotone arrival counter and waits until it reaches
 // `target` (= blocks x barriers so far).  All of this thread's memory
 // operations complete before the arrival (s_waitcnt), and every

@@ -40,7 +40,7 @@ int net_score(phde_ctx *c, phde_mpnn *net);
 // within r), with c->lp_sd holding the distances to picks[0..first-1] on
 // entry.  picks (host) must hold K entries; picks[0..first-1] are inputs.
 int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, int64_t *picks);
-// greedy_select's status when the device cannot run its cooperative launch
+// greedy_select's status when the device cannot run its persistent kernel
 // (the learned layout then falls back to k-centers)
 constexpr int kSelectUnavailable = -100;
 

@@ -697,7 +697,7 @@ __device__ __forceinline__ void sel_push(bool push, uint32_t v, uint32_t *qn, un
   if (push) st_dev(&qn[base + __popcll(mask & lanemask_lt())], v);
 }
 
-// All picks in one cooperative launch, in rounds of up to kSelRound picks.
+// All picks in one persistent launch (every block resident), in rounds of up to kSelRound picks.
 // Within a round each pick maximises (min(sd, r+1), score, -index) over the
 // vertices that are provably more than r hops from the round's earlier
 // picks — max(|d1_v - d1_c|, |d2_v - d2_c|) > r, a lower bound of the hop
@@ -1091,11 +1091,10 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
   void *args[] = {&a, &arrive, &err};
   hipEvent_t ev;
   c->timer.begin(c->stream, PHDE_KERNEL_OTHER, &ev);
-  const hipError_t e = hipLaunchCooperativeKernel((const void *)select_kernel, dim3(c->bfs_grid), dim3(256),
-                                                  args, 0, c->stream);
+  const hipError_t e = launch_resident((const void *)select_kernel, c->bfs_grid, 256, args, 0, c->stream);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    set_error(std::string("pivot selection needs a cooperative launch: ") + hipGetErrorString(e));
+    set_error(std::string("pivot selection needs a persistent kernel: ") + hipGetErrorString(e));
     return kSelectUnavailable;
   }
   c->timer.end(c->stream, PHDE_KERNEL_OTHER, ev);
@@ -1105,8 +1104,10 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
   PHDE_HIP(hipMemcpyAsync(herr, err, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
   if (herr[0]) {
+    // blocks kept from becoming resident: the caller selects without the
+    // persistent kernel
     set_error("pivot selection grid barrier timed out");
-    return PHDE_ERR_DEVICE;
+    return kSelectUnavailable;
   }
   if (a.hops > 1 && herr[1]) {
     PHDE_HIP(hipMemcpyAsync(c->lp_sd, sd0.p, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));

@@ -8,7 +8,7 @@
 // spends its traversal on the latency of each level's dependent global
 // round trips.  Here the graph is cut once, at context creation, into
 // compact parts whose local CSR, levels and work lists fit in one CU's LDS,
-// and one persistent cooperative kernel runs the search as a work queue of
+// and one persistent kernel (every block resident) runs the search as a work queue of
 // parts: a workgroup takes a part some of whose vertices were lowered from
 // outside, runs rounds inside LDS from those vertices (each listed vertex
 // offers its level + 1 to its neighbours; a neighbour whose level drops is
@@ -812,8 +812,8 @@ int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, boo
     return PHDE_OK;
   }
   void *args[] = {&a};
-  const hipError_t e = hipLaunchCooperativeKernel((const void *)pg->shape.fn, dim3(grid),
-                                                  dim3(pg->shape.block), args, kPartBytes, c->stream);
+  const hipError_t e = launch_resident((const void *)pg->shape.fn, grid, pg->shape.block, args, kPartBytes,
+                                       c->stream);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     pg->disabled = true;

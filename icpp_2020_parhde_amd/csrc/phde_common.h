@@ -207,6 +207,20 @@ int bfs_run(phde_ctx *c, int64_t src, int alpha, int beta, int32_t *dist_out, in
 int dist_finalize(phde_ctx *c, const int32_t *dist_in, int32_t *dist_out, int32_t unreached_val,
                   int32_t unreached_out);
 
+// Launch of a persistent kernel (blocks that wait on one another through a
+// grid barrier or a work queue).  Its grid is sized from the occupancy query
+// so that every block is resident at once on an otherwise idle GPU, and
+// every wait in it is bounded: a grid that other kernels keep from becoming
+// resident ends with a timeout flag, and the caller repeats the work
+// without a persistent kernel.  This is an ordinary launch:
+// hipLaunchCooperativeKernel places the blocks no differently here, and
+// after one the HIP runtime's exit handler faults inside libhsa-runtime64
+// whenever the process runs under rocprofv3.
+inline hipError_t launch_resident(const void *fn, int grid, int block, void **args, size_t shmem,
+                                  hipStream_t stream) {
+  return hipLaunchKernel(fn, dim3(grid), dim3(block), args, shmem, stream);
+}
+
 // orth.hip — D-orthogonalisation and reductions
 int fill_i32(phde_ctx *c, int32_t *p, int64_t n, int32_t v);
 int fill_u32(phde_ctx *c, unsigned int *p, int64_t n, unsigned int v);

@@ -722,7 +722,7 @@ __global__ __launch_bounds__(kTBlock) void msbfs_level_kernel(MsArgs a, int L) {
 }
 
 // ---- multi-source BFS of a graph with a deep BFS: the k searches run as
-// concurrent multi-hop searches (hops.h) in one cooperative launch, search
+// concurrent multi-hop searches (hops.h) in one persistent launch, search
 // j's levels in column j of D; the same levels as the bit-parallel sweep.
 struct MsHopArgs {
   DevGraph g;
@@ -950,8 +950,7 @@ int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
     int resident = 0;
     PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, msbfs_hops_kernel, kTBlock, 0));
     const int hgrid = std::max(1, c->num_cus) * std::max(1, std::min(per_cu, resident));
-    const hipError_t e = hipLaunchCooperativeKernel((const void *)msbfs_hops_kernel, dim3(hgrid),
-                                                    dim3(kTBlock), args, 0, c->stream);
+    const hipError_t e = launch_resident((const void *)msbfs_hops_kernel, hgrid, kTBlock, args, 0, c->stream);
     if (e == hipSuccess) {
       mshop_finish_kernel<<<fgrid, kTBlock, 0, c->stream>>>(D, n * k);
       PHDE_HIP(hipGetLastError());
@@ -959,11 +958,8 @@ int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
       PHDE_HIP(hipMemcpyAsync(c->h_bar, c->d_bar + 1, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost,
                               c->stream));
       PHDE_HIP(hipStreamSynchronize(c->stream));
-      if (c->h_bar[0]) {
-        set_error("multi-source BFS grid barrier timed out");
-        return PHDE_ERR_DEVICE;
-      }
-      if (!c->h_bar[1]) return PHDE_OK;
+      if (!c->h_bar[0] && !c->h_bar[1]) return PHDE_OK;
+      // the grid barrier timed out (blocks kept from becoming resident) or
       // the queue overflowed: the bit-parallel sweep below recomputes D
     } else {
       (void)hipGetLastError();
