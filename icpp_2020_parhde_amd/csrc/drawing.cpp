@@ -10,6 +10,7 @@
 #include <cstring>
 #include <iostream>
 #include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "DrawingBase.h"
@@ -25,16 +26,51 @@ namespace {
   std::exit(code);
 }
 
+// Fingerprint of an array's contents: four independent lanes of 64-bit
+// multiply-rotate mixing over 8-byte words, folded together (not
+// cryptographic; a changed word changes the result with overwhelming
+// probability).  About 10 GB/s on one core.
+uint64_t fingerprint(const void *p, size_t bytes) {
+  const unsigned char *b = static_cast<const unsigned char *>(p);
+  const uint64_t K = 0x9E3779B97F4A7C15ull;
+  uint64_t h[4] = {K, K ^ 1, K ^ 2, K ^ 3};
+  auto mix = [&](uint64_t &x, uint64_t v) {
+    x ^= v * 0xC2B2AE3D27D4EB4Full;
+    x = (x << 31 | x >> 33) * K;
+  };
+  size_t i = 0;
+  for (; i + 32 <= bytes; i += 32) {
+    uint64_t v[4];
+    std::memcpy(v, b + i, 32);
+    for (int j = 0; j < 4; j++) mix(h[j], v[j]);
+  }
+  uint64_t tail = bytes;
+  for (; i < bytes; i++) tail = tail * 131 + b[i];
+  mix(h[0], tail);
+  return h[0] ^ (h[1] << 1 | h[1] >> 63) ^ (h[2] << 2 | h[2] >> 62) ^ (h[3] << 3 | h[3] >> 61);
+}
+
 // A context for callers that hand raw arrays to the free functions: the last
 // graph is kept resident so repeated DOBFS / DeltaStep calls on the same
-// arrays do not re-upload it.
+// arrays do not re-upload it.  The reference reads the arrays it is given on
+// every call, so the cache is keyed on their contents as well as their
+// addresses: arrays changed in place, or freed and allocated again at the
+// same addresses, are uploaded again.
 struct GraphCache {
   std::mutex mu;
   const void *row = nullptr, *col = nullptr, *w = nullptr;
   long n = -1, m = -1;
+  uint64_t fp = 0;
   phde_ctx *ctx = nullptr;
+  static uint64_t contents(const unsigned *r, const unsigned *c, const int32_t *wt, long nn, long mm) {
+    uint64_t f = fingerprint(r, sizeof(unsigned) * (size_t)(nn + 1));
+    f = f * 31 + fingerprint(c, sizeof(unsigned) * (size_t)mm);
+    if (wt) f = f * 31 + fingerprint(wt, sizeof(int32_t) * (size_t)mm);
+    return f;
+  }
   phde_ctx *get(const unsigned *r, const unsigned *c, const int32_t *wt, long nn, long mm) {
-    if (ctx && r == row && c == col && wt == w && nn == n && mm == m) return ctx;
+    const uint64_t f = nn >= 0 && mm >= 0 && r && (c || mm == 0) ? contents(r, c, wt, nn, mm) : 0;
+    if (ctx && r == row && c == col && wt == w && nn == n && mm == m && f == fp) return ctx;
     if (ctx) phde_destroy(ctx);
     ctx = nullptr;
     int rc = phde_create(r, c, wt, nn, mm, &ctx);
@@ -44,12 +80,36 @@ struct GraphCache {
     w = wt;
     n = nn;
     m = mm;
+    fp = f;
     return ctx;
   }
 };
 GraphCache &cache() {
   static GraphCache g;
   return g;
+}
+
+// The GPU state of each layout object, keyed by its address: the objects
+// keep the reference's layout (variants/DrawingBase.h:13-48), which has no
+// room for it.  Allocated once and never destroyed, so that objects that
+// outlive static destruction still find it.
+struct DrawState {
+  phde_ctx *ctx = nullptr;
+  phde_options opts;
+  phde_stats stats;
+};
+std::mutex &states_mu() {
+  static std::mutex *mu = new std::mutex;
+  return *mu;
+}
+std::unordered_map<const DrawingBase *, DrawState> &states() {
+  static auto *m = new std::unordered_map<const DrawingBase *, DrawState>;
+  return *m;
+}
+// References to map elements stay valid while other objects come and go.
+DrawState &state_of(const DrawingBase *b) {
+  std::lock_guard<std::mutex> lk(states_mu());
+  return states().at(b);
 }
 
 // Context for the eigen helpers, which need a device but no graph.
@@ -123,28 +183,41 @@ DrawingBase::DrawingBase(unsigned int *rowPtrs, unsigned int *adj, int32_t *weig
       m(m),
       numSources(numSources),
       numCoord(numCoord),
-      doRand(doRand),
-      ctx(nullptr) {
+      doRand(doRand) {
   eigenVectors = (double *)malloc(sizeof(double) * numSources * numCoord);
   coordinates = (double *)malloc(sizeof(double) * n * numCoord);
-  std::memset(&lastStats, 0, sizeof(lastStats));
-  phde_default_options(&opts);
-  opts.s = numSources;
-  opts.num_coord = numCoord;
+  DrawState st;
+  std::memset(&st.stats, 0, sizeof(st.stats));
+  phde_default_options(&st.opts);
+  st.opts.s = numSources;
+  st.opts.num_coord = numCoord;
   // doRand as spectralDrawing.cpp:88-95 maps -p (0 RandCoarse, 1 RandFine,
   // 2 Kcenters); 3 selects the learned proposal (setPivotNet)
-  opts.strategy = doRand == 3   ? PHDE_PIVOTS_LEARNED
-                  : doRand == 2 ? PHDE_PIVOTS_KCENTERS
-                                : (doRand == 1 ? PHDE_PIVOTS_RANDFINE : PHDE_PIVOTS_RANDCOARSE);
-  int rc = phde_create(rowPtrs, adj, weights, n, m, &ctx);
+  st.opts.strategy = doRand == 3   ? PHDE_PIVOTS_LEARNED
+                     : doRand == 2 ? PHDE_PIVOTS_KCENTERS
+                                   : (doRand == 1 ? PHDE_PIVOTS_RANDFINE : PHDE_PIVOTS_RANDCOARSE);
+  int rc = phde_create(rowPtrs, adj, weights, n, m, &st.ctx);
   if (rc) die("graph upload", rc, 1);
+  std::lock_guard<std::mutex> lk(states_mu());
+  states()[this] = st;
 }
 
-int DrawingBase::setPivotNet(phde_mpnn *net) { return phde_set_pivot_net(ctx, net); }
+phde_options &DrawingBase::options() { return state_of(this).opts; }
+const phde_stats &DrawingBase::stats() const { return state_of(this).stats; }
+int DrawingBase::setPivotNet(phde_mpnn *net) { return phde_set_pivot_net(state_of(this).ctx, net); }
 
 DrawingBase::~DrawingBase() {
   free(eigenVectors);
   free(coordinates);
+  phde_ctx *ctx = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(states_mu());
+    auto it = states().find(this);
+    if (it != states().end()) {
+      ctx = it->second.ctx;
+      states().erase(it);
+    }
+  }
   if (ctx) phde_destroy(ctx);
 }
 
@@ -152,7 +225,7 @@ void DrawingBase::compute_eigens(double *matrix, int dimension, double *eigVecs,
   // DrawingBase.cpp:39-108 (non-MKL): power method, then the first or last
   // numCoord columns.
   std::vector<double> w(dimension);
-  if (phde_power_eig(ctx, matrix, w.data(), dimension, nullptr) != PHDE_OK) {
+  if (phde_power_eig(state_of(this).ctx, matrix, w.data(), dimension, nullptr) != PHDE_OK) {
     printf("The algorithm failed to compute eigenvalues.\n");
     std::exit(1);
   }
@@ -174,7 +247,7 @@ void DrawingBase::DistanceAlgorithm(unsigned int *row, unsigned int *col, long s
 
 void DrawingBase::seqBfs(unsigned int start, double *distCol) {
   std::vector<int32_t> d(n);
-  int rc = phde_bfs(ctx, start, 15, 18, d.data());
+  int rc = phde_bfs(state_of(this).ctx, start, 15, 18, d.data());
   if (rc) die("seqBfs", rc, 1);
   for (long i = 0; i < n; i++)
     if (d[i] >= 0) distCol[i] = (double)d[i];
@@ -242,6 +315,9 @@ void DrawingBase::matrixTransMultiplication(double *A, double *B, double *C, lon
 }
 
 int DrawingBase::runLayout(int algo, int tries, int alpha, int beta, int32_t delta) {
+  DrawState &S = state_of(this);
+  phde_options &opts = S.opts;
+  phde_stats &lastStats = S.stats;
   opts.algorithm = algo;
   opts.alpha = alpha > 0 ? alpha : 15;
   opts.beta = 18;
@@ -251,7 +327,7 @@ int DrawingBase::runLayout(int algo, int tries, int alpha, int beta, int32_t del
   double tHDE = 0, tDist = 0, tOrth = 0, tMM = 0;
   int rc = PHDE_OK;
   for (int t = 0; t < tries; t++) {
-    rc = phde_layout(ctx, &opts, X.data(), &lastStats);
+    rc = phde_layout(S.ctx, &opts, X.data(), &lastStats);
     if (rc == PHDE_ERR_WEIGHTS) {
       std::cerr << (algo == PHDE_ALGO_WPARHDE ? "Weighted ParHDE code requires the weighted edges"
                                               : "ParHDE code doesn't requires the weighted edges")
@@ -296,13 +372,19 @@ int DrawingBase::runLayout(int algo, int tries, int alpha, int beta, int32_t del
 
 ParHDE::ParHDE(unsigned int *rowPtrs, unsigned int *adj, int32_t *weights, long n, long m,
                int numSources, int doRand, int numCoord)
-    : DrawingBase(rowPtrs, adj, weights, n, m, numSources, doRand, numCoord) {
+    : DrawingBase(rowPtrs, adj, weights, n, m, numSources, doRand, numCoord),
+      SLSmatrix(nullptr),
+      LSmatrix(nullptr) {
   if (weights) {  // ParHDE.cpp:27-31
     std::cerr << "ParHDE code doesn't requires the weighted edges" << std::endl;
     std::exit(0);
   }
 }
-ParHDE::~ParHDE() {}
+ParHDE::~ParHDE() {
+  // null here (the products live on the GPU); freed as ParHDE.cpp:34-38 does
+  free(SLSmatrix);
+  free(LSmatrix);
+}
 int ParHDE::run(int tries, int alpha, int beta, int32_t delta) {
   printf("Running ParHDE\n");
   std::cout << "Performing ParHDE with Number of sources: " << numSources << " and ";
@@ -312,13 +394,19 @@ int ParHDE::run(int tries, int alpha, int beta, int32_t delta) {
 
 WParHDE::WParHDE(unsigned int *rowPtrs, unsigned int *adj, int32_t *weights, long n, long m,
                  int numSources, int numCoord)
-    : DrawingBase(rowPtrs, adj, weights, n, m, numSources, 2, numCoord) {
+    : DrawingBase(rowPtrs, adj, weights, n, m, numSources, 2, numCoord),
+      SLSmatrix(nullptr),
+      LSmatrix(nullptr) {
   if (!weights) {  // WParHDE.cpp:22-26
     std::cerr << "Weighted ParHDE code requires the weighted edges" << std::endl;
     std::exit(0);
   }
 }
-WParHDE::~WParHDE() {}
+WParHDE::~WParHDE() {
+  // null here (the products live on the GPU); freed as WParHDE.cpp:29-33 does
+  free(SLSmatrix);
+  free(LSmatrix);
+}
 int WParHDE::run(int tries, int alpha, int beta, int32_t delta) {
   printf("Running Weighted ParHDE\n");
   std::cout << "Performing parHDE with Number of sources: " << numSources << " and ";
@@ -328,8 +416,11 @@ int WParHDE::run(int tries, int alpha, int beta, int32_t delta) {
 
 PivotMDS::PivotMDS(unsigned int *rowPtrs, unsigned int *adj, int32_t *weights, long n, long m,
                    int numSources, int doRand, int numCoord)
-    : DrawingBase(rowPtrs, adj, weights, n, m, numSources, doRand, numCoord) {}
-PivotMDS::~PivotMDS() {}
+    : DrawingBase(rowPtrs, adj, weights, n, m, numSources, doRand, numCoord), matrixC(nullptr) {}
+PivotMDS::~PivotMDS() {
+  // null here (the products live on the GPU); freed as PivotMDS.cpp:21-24 does
+  free(matrixC);
+}
 int PivotMDS::run(int tries, int alpha, int beta, int32_t delta) {
   std::cout << "Performing PivotMDS with " << tries << " tries and pivots: " << numSources << std::endl;
   return runLayout(PHDE_ALGO_PIVOTMDS, tries, alpha, beta, delta);
@@ -337,8 +428,11 @@ int PivotMDS::run(int tries, int alpha, int beta, int32_t delta) {
 
 PHDE::PHDE(unsigned int *rowPtrs, unsigned int *adj, int32_t *weights, long n, long m, int numSources,
            int doRand, int numCoord)
-    : DrawingBase(rowPtrs, adj, weights, n, m, numSources, doRand, numCoord) {}
-PHDE::~PHDE() {}
+    : DrawingBase(rowPtrs, adj, weights, n, m, numSources, doRand, numCoord), matrixC(nullptr) {}
+PHDE::~PHDE() {
+  // null here (the products live on the GPU); freed as PHDE.cpp:21-24 does
+  free(matrixC);
+}
 int PHDE::run(int tries, int alpha, int beta, int32_t delta) {
   std::cout << "Performing PHDE with " << tries << " tries and pivots: " << numSources << std::endl;
   return runLayout(PHDE_ALGO_PHDE, tries, alpha, beta, delta);

@@ -205,7 +205,18 @@ int main(int argc, char **argv) {
   elt = std::chrono::high_resolution_clock::now() - startTimer;
   std::cout << "Overall time: " << elt.count() << " s." << std::endl;
   if (net) {
-    delete draw;  // the context that borrows the network goes first
+    // the context that borrows the network goes first; deleted through its
+    // own class, which is the object's dynamic type (the destructor is
+    // non-virtual, as in the reference)
+#pragma GCC diagnostic push
+#pragma GCC diagnostic ignored "-Wdelete-non-virtual-dtor"
+    switch (drawingAlgorithm) {
+      case 0: delete static_cast<PivotMDS *>(draw); break;
+      case 1: delete static_cast<PHDE *>(draw); break;
+      case 2: delete static_cast<WParHDE *>(draw); break;
+      default: delete static_cast<ParHDE *>(draw);
+    }
+#pragma GCC diagnostic pop
     phde_net_destroy(net);
   }
   return 0;

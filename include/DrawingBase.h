@@ -1,7 +1,12 @@
 // DrawingBase.h — drop-in C++ interface of the reference's layout classes
 // (variants/DrawingBase.h:1-68), implemented on the MI355X library
 // (parhde.h).  Same class names, constructor and run() signatures, enum and
-// free functions, so code written against the reference compiles unchanged.
+// free functions, so code written against the reference compiles unchanged;
+// the objects also have the reference's layout (data members, their order,
+// a non-virtual destructor, run() in vtable slot 0), so code compiled
+// against the reference's headers links against this library.  The GPU
+// state of an object (context, options, statistics) lives in a table keyed
+// by the object's address, not in the object.
 //
 // Differences, all deliberate:
 //  - the constructor uploads the graph to the GPU once; the caller's arrays
@@ -37,9 +42,6 @@ class DrawingBase {
   double *coordinates;
   const int ALIGN = 64;
   int doRand;
-  phde_ctx *ctx;
-  phde_options opts;
-  phde_stats lastStats;
 
   // DrawingBase.cpp:221-244: C[k + i*m] = sum_j A[j + i*k] B[j + k*k]
   void matrixTransMultiplication(double *A, double *B, double *C, long m, long k, long n);
@@ -60,18 +62,18 @@ class DrawingBase {
 
  public:
   DrawingBase(unsigned int *, unsigned int *, int32_t *, long, long, int, int, int numCoord = 2);
-  virtual ~DrawingBase();
+  ~DrawingBase();  // non-virtual, as variants/DrawingBase.h:42
 
   double *getCoordinates();
 
   void compute_eigens(double *, int, double *, EigenChoice);
   virtual int run(int tries = 11, int alpha = 15, int beta = 18, int32_t delta = 1) = 0;
 
-  phde_options &options() { return opts; }
+  phde_options &options();
   // Network used by the learned pivot strategy (doRand 3 or
   // options().strategy = PHDE_PIVOTS_LEARNED); borrowed.
   int setPivotNet(phde_mpnn *net);
-  const phde_stats &stats() const { return lastStats; }
+  const phde_stats &stats() const;
 };
 
 struct Compare {

@@ -4,6 +4,11 @@
 #include "DrawingBase.h"
 
 class PivotMDS : public DrawingBase {
+ private:
+  // the reference's members (variants/PivotMDS.h:13), kept so that the object has its
+  // size and layout; the products they held live on the GPU here
+  double *matrixC;
+
  public:
   PivotMDS(unsigned int *rowPtrs, unsigned int *adj, int32_t *weights, long n, long m, int s, int r,
        int numCoord);

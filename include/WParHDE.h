@@ -6,6 +6,12 @@
 #include "DrawingBase.h"
 
 class WParHDE : public DrawingBase {
+ private:
+  // the reference's members (variants/WParHDE.h:13-14), kept so that the object has its
+  // size and layout; the products they held live on the GPU here
+  double *SLSmatrix;
+  double *LSmatrix;
+
  public:
   WParHDE(unsigned int *rowPtrs, unsigned int *adj, int32_t *weights, long n, long m, int s,
           int numCoord);

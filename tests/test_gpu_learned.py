@@ -229,7 +229,10 @@ def test_trained_network_gains_over_no_network(gpu):
     """What the trained network adds: on held-out graphs (seeds outside the
     training range), the layout from the trained network's scores has a
     lower Hall energy, relative to k-centers, than the same selection on
-    constant scores (no network); both stay within the 2% bar."""
+    constant scores (no network), on every graph and by at least 0.1
+    percentage point on average (measured: +0.07% trained against +0.26%
+    without the network); both stay within the 2% bar.  The selection's
+    speed does not depend on the scores."""
     held = [Graph.rgg(60000, 10.0, seed=6001), Graph.holed_grid(220, 220, 10, 6002),
             Graph.rgg(90000, 10.0, seed=6003)]
     with L.PivotNet(path=str(LP.DEFAULT_WEIGHTS)) as net:
@@ -237,7 +240,8 @@ def test_trained_network_gains_over_no_network(gpu):
     for (gt, g0), g in zip(gaps, held):
         print(f"n={g.n}: trained {100 * gt:+.3f}%, no network {100 * g0:+.3f}%")
     assert all(gt <= 0.02 for gt, _ in gaps)
-    assert np.mean([gt for gt, _ in gaps]) < np.mean([g0 for _, g0 in gaps])
+    assert all(gt < g0 for gt, g0 in gaps)
+    assert np.mean([g0 for _, g0 in gaps]) - np.mean([gt for gt, _ in gaps]) >= 0.001
 
 
 def _chunk_boundary_graph():

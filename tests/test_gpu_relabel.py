@@ -231,3 +231,22 @@ def test_learned_entry_points(gpu):
     assert aligned_rel_err(b[6], a[6]) < FP64_TOL
     assert b[8] == pytest.approx(a[8], rel=1e-12)
     assert b[7].hall_energy == pytest.approx(a[7].hall_energy, rel=1e-12)
+
+
+@pytest.mark.parametrize("name,s", [("path7", 7), ("cycle8", 5), ("mesh3x3", 8), ("star9", 10)])
+def test_dedupe_branch_matches_oracle(gpu, name, s):
+    """More traversals than vertices: every vertex becomes a source and the
+    dedupe rule (ParHDE.cpp:142-155) repeats; the pivot sequence is the
+    oracle's (or_parhde), and the layout stops with the reference's
+    too-few-pivots error (ParHDE.cpp:281-285)."""
+    g = {"path7": lambda: Graph.path(7), "cycle8": lambda: Graph.cycle(8),
+         "mesh3x3": lambda: Graph.mesh(3, 3), "star9": lambda: Graph.star(9)}[name]()
+    ref = O.parhde(g, s, 2)
+    assert ref["n_bfs"] > g.n and ref["rc"] != 0
+    for r in (False, True):
+        with context(g, r) as ctx:
+            src, _ = ctx.kcenters(ref["n_bfs"])
+            assert src.tolist() == ref["pivots"]
+            with pytest.raises(L.PhdeError) as e:
+                ctx.layout(s=s, num_coord=2)
+            assert e.value.code == -4
