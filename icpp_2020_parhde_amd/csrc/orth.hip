@@ -24,7 +24,6 @@ namespace phde {
 
 constexpr int kOrthBlock = 256;
 constexpr int kColChunk = 16;
-constexpr int kCgsCols = 4;  // columns per block of the blocked dot products
 
 __device__ __forceinline__ double block_sum(double v, double *sh) {
 #pragma unroll
@@ -248,33 +247,35 @@ __global__ void mgs_axpy_kernel(double *a, const TY *y, const double *c, int64_t
 
 // Dot products of the candidates A_b (b < nb) with columns [0, m) and the
 // ones vector.  part layout: [b][slot][rowblock] with slot = column or
-// kOnesSlot.  Columns are taken kCgsCols per blockIdx.y.
-template <typename T>
+// kOnesSlot.  Columns are taken Cols per blockIdx.y; every (candidate,
+// column) sum runs over the same rows in the same order whatever Cols is,
+// so the choice changes time, not bits.
+template <typename T, int Cols>
 __global__ __launch_bounds__(kOrthBlock) void cgs_dots_multi_kernel(const double *A, const double *deg,
                                                                    const T *S, int64_t n, int m, int nb,
                                                                    double u0, double *part) {
-  __shared__ double sh[kCgsBlock * (kCgsCols + 1)][kOrthBlock / kWave];
-  const int k0 = blockIdx.y * kCgsCols;
-  const int kc = max(0, min(kCgsCols, m - k0));
+  __shared__ double sh[kCgsBlock * (Cols + 1)][kOrthBlock / kWave];
+  const int k0 = blockIdx.y * Cols;
+  const int kc = max(0, min(Cols, m - k0));
   const bool ones = blockIdx.y == 0;
-  double acc[kCgsBlock][kCgsCols + 1];
+  double acc[kCgsBlock][Cols + 1];
 #pragma unroll
   for (int b = 0; b < kCgsBlock; b++)
 #pragma unroll
-    for (int c = 0; c <= kCgsCols; c++) acc[b][c] = 0;
+    for (int c = 0; c <= Cols; c++) acc[b][c] = 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const double dg = deg[i];
-    double sv[kCgsCols];
+    double sv[Cols];
 #pragma unroll
-    for (int c = 0; c < kCgsCols; c++) sv[c] = c < kc ? (double)S[(int64_t)(k0 + c) * n + i] : 0.0;
+    for (int c = 0; c < Cols; c++) sv[c] = c < kc ? (double)S[(int64_t)(k0 + c) * n + i] : 0.0;
 #pragma unroll
     for (int b = 0; b < kCgsBlock; b++) {
       if (b < nb) {
         const double ad = A[(int64_t)b * n + i] * dg;
-        acc[b][kCgsCols] = fma(ad, u0, acc[b][kCgsCols]);
+        acc[b][Cols] = fma(ad, u0, acc[b][Cols]);
 #pragma unroll
-        for (int c = 0; c < kCgsCols; c++) acc[b][c] = fma(ad, sv[c], acc[b][c]);
+        for (int c = 0; c < Cols; c++) acc[b][c] = fma(ad, sv[c], acc[b][c]);
       }
     }
   }
@@ -283,21 +284,21 @@ __global__ __launch_bounds__(kOrthBlock) void cgs_dots_multi_kernel(const double
 #pragma unroll
   for (int b = 0; b < kCgsBlock; b++)
 #pragma unroll
-    for (int c = 0; c <= kCgsCols; c++) {
+    for (int c = 0; c <= Cols; c++) {
       double v = acc[b][c];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-      if (lane == 0) sh[b * (kCgsCols + 1) + c][w] = v;
+      if (lane == 0) sh[b * (Cols + 1) + c][w] = v;
     }
   __syncthreads();
-  for (int t = threadIdx.x; t < kCgsBlock * (kCgsCols + 1); t += blockDim.x) {
-    const int b = t / (kCgsCols + 1), c = t % (kCgsCols + 1);
+  for (int t = threadIdx.x; t < kCgsBlock * (Cols + 1); t += blockDim.x) {
+    const int b = t / (Cols + 1), c = t % (Cols + 1);
     if (b >= nb) continue;
-    if (c < kCgsCols && c >= kc) continue;
-    if (c == kCgsCols && !ones) continue;
+    if (c < Cols && c >= kc) continue;
+    if (c == Cols && !ones) continue;
     double tot = 0;
     for (int q = 0; q < (int)(blockDim.x / kWave); q++) tot += sh[t][q];
-    const int slot = c == kCgsCols ? kOnesSlot : k0 + c;
+    const int slot = c == Cols ? kOnesSlot : k0 + c;
     part[((int64_t)b * (kOnesSlot + 1) + slot) * gridDim.x + blockIdx.x] = tot;
   }
 }
@@ -597,6 +598,20 @@ static int accept_t(Orth &o, int j) {
 
 int Orth::accept(int j) { return fp32 ? accept_t<float>(*this, j) : accept_t<double>(*this, j); }
 
+// Columns per block of the blocked dot products.  Each block re-reads the
+// candidates, but wider blocks hold more accumulators and run fewer waves:
+// measured on RGG n=4.2M, s=50 (bit-identical results), orth 12.4 / 12.8 /
+// 15.0 ms (FP64) and 20.9 / 23.9 / 32.1 ms (FP32 storage, CGS2) at 4 / 8 /
+// 16 columns.
+constexpr int kCgsCols = 4;
+
+template <typename T>
+static void dots_multi(Orth &o, const T *S, int m, int nc) {
+  const int nchunk = std::max(1, (m + kCgsCols - 1) / kCgsCols);
+  cgs_dots_multi_kernel<T, kCgsCols><<<dim3(o.nblk, nchunk), kOrthBlock, 0, o.ctx->stream>>>(
+      o.blkA, o.deg_, S, o.n, m, nc, o.u0, o.blkPart);
+}
+
 template <typename T>
 static int cgs_block_t(Orth &o, const int32_t *const *cols, const double *scales, int nc, int s_target,
                        int &accepted, std::vector<bool> &taken, std::vector<double> &norms) {
@@ -606,9 +621,7 @@ static int cgs_block_t(Orth &o, const int32_t *const *cols, const double *scales
   const int m = accepted;
   for (int b = 0; b < nc; b++)
     cand_from_dist_kernel<<<grid_for(n, 4096), kOrthBlock, 0, st>>>(cols[b], scales[b], o.blkA + (int64_t)b * n, n);
-  const int nchunk = std::max(1, (m + kCgsCols - 1) / kCgsCols);
-  cgs_dots_multi_kernel<T><<<dim3(o.nblk, nchunk), kOrthBlock, 0, st>>>(o.blkA, o.deg_, S, n, m, nc, o.u0,
-                                                                        o.blkPart);
+  dots_multi<T>(o, S, m, nc);
   cgs_coef_multi_kernel<<<dim3(m + 1, nc), kOrthBlock, 0, st>>>(o.blkPart, o.nblk, m, o.den, o.blkCoef);
   cgs_update_old_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(S, n, m, nc, o.blkCoef, o.blkAcc);
   PHDE_HIP(hipGetLastError());
@@ -647,10 +660,8 @@ static int cgs2_block_t(Orth &o, const int32_t *const *cols, const double *scale
   const int m = accepted;
   for (int b = 0; b < nc; b++)
     cand_from_dist_kernel<<<grid_for(n, 4096), kOrthBlock, 0, st>>>(cols[b], scales[b], o.blkA + (int64_t)b * n, n);
-  const int nchunk = std::max(1, (m + kCgsCols - 1) / kCgsCols);
   for (int pass = 0; pass < 2; pass++) {
-    cgs_dots_multi_kernel<T><<<dim3(o.nblk, nchunk), kOrthBlock, 0, st>>>(o.blkA, o.deg_, S, n, m, nc, o.u0,
-                                                                          o.blkPart);
+    dots_multi<T>(o, S, m, nc);
     cgs_coef_multi_kernel<<<dim3(m + 1, nc), kOrthBlock, 0, st>>>(o.blkPart, o.nblk, m, o.den, o.blkCoef);
     cgs_block_sub_kernel<T><<<o.nblk, kOrthBlock, 0, st>>>(o.blkA, S, n, m, nc, o.u0, o.blkCoef);
   }
