@@ -104,6 +104,7 @@ PARHDE_SYMBOLS = {
                                    ctypes.POINTER(PhdeStats)]),
     "phde_power_eig": (ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, ctypes.c_int, c_i32p]),
     "phde_num_parts": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "phde_num_boundary": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "phde_power_eig_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                               ctypes.c_void_p]),
     "phde_sym_eig": (ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, ctypes.c_int, c_f64p, c_f64p]),

@@ -1008,6 +1008,12 @@ int phde_num_parts(const phde_ctx *c, int64_t *parts) {
   return PHDE_OK;
 }
 
+int phde_num_boundary(const phde_ctx *c, int64_t *boundary) {
+  if (!c || !boundary) return PHDE_ERR_ARG;
+  *boundary = pbfs_num_boundary(c);
+  return PHDE_OK;
+}
+
 int phde_bfs_dev(phde_ctx *c, int64_t source, int alpha, int beta, int32_t *dist_dev) {
   if (!c || !dist_dev || source < 0 || source >= c->g.n) return PHDE_ERR_ARG;
   PHDE_TRY(bfs_run(c, to_internal(c, source), alpha, beta, c->d_dist, nullptr, false));

@@ -11,6 +11,9 @@ int pbfs_setup(phde_ctx *c);
 void pbfs_release(phde_ctx *c);
 // Parts of the context's graph, 0 when the partitioned BFS is not in use.
 int64_t pbfs_num_parts(const phde_ctx *c);
+// Boundary vertices when the traversals run by the boundary-distance
+// method, -1 otherwise.
+int64_t pbfs_num_boundary(const phde_ctx *c);
 // One traversal from src (internal id) with the fused k-centers step (as
 // bfs_run; c->h_kc holds the result).  *ran is false when the context does
 // not use the partitioned BFS, or a wait in it timed out (it then stays

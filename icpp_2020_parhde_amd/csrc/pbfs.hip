@@ -38,6 +38,7 @@
 // bounded: a timeout sets an error word, every workgroup stops, and the
 // host repeats the traversal with the multi-hop kernel.
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -137,7 +138,8 @@ __global__ void pbfs_init_kernel(PbfsArgs a) {
 }
 
 // A vertex of part q was lowered from outside: queue q or flag its runner.
-__device__ __forceinline__ void mark_lowered(const PbfsArgs &a, uint32_t q) {
+template <class A>
+__device__ __forceinline__ void mark_lowered(const A &a, uint32_t q) {
   const uint32_t old = atomicOr(&a.st[q], 1u);
   if ((old & 3u) == 0u) {
     const uint32_t t = (uint32_t)(atomicAdd(tp_word(a.ctl), (1ull << 32) | 1ull) >> 32);
@@ -149,7 +151,8 @@ __device__ __forceinline__ void mark_lowered(const PbfsArgs &a, uint32_t q) {
 // After the last part is released (pending reached zero at tail `t`), the
 // slots of tickets t .. t + gridDim.x - 1 -- the only ones still to be
 // waited on, one per workgroup at most -- are marked done.
-__device__ __forceinline__ void mark_done(const PbfsArgs &a, uint32_t t) {
+template <class A>
+__device__ __forceinline__ void mark_done(const A &a, uint32_t t) {
   for (uint32_t j = 0; j < gridDim.x; j++) {
     const uint32_t u = t + j;
     st_dev(&a.ring[(uint64_t)(u % a.ring_n) * kSlotStride], ((unsigned long long)(u + 1u) << 32) | kDoneSlot);
@@ -163,7 +166,8 @@ __device__ __forceinline__ void mark_done(const PbfsArgs &a, uint32_t t) {
 // h + 1; ring_n exceeds the parts plus the workgroups, so a slot is not
 // reused while its ticket is still waiting (the bounded wait below turns a
 // violation into an error, never into a wrong part).
-__device__ uint32_t pop_part(const PbfsArgs &a) {
+template <class A>
+__device__ uint32_t pop_part(const A &a) {
   const uint32_t h = atomicAdd(&a.ctl[kCtlHead], 1u);
   const unsigned long long *slot = &a.ring[(uint64_t)(h % a.ring_n) * kSlotStride];
   long spins = 0;
@@ -552,6 +556,694 @@ __global__ __launch_bounds__(Block) void pbfs_kernel(PbfsArgs a) {
   }
 }
 
+// ------------------------------------------------- boundary-distance traversal
+//
+// The same parts, searched at two scales.  At context creation every part's
+// boundary-to-boundary distances inside the part (paths that stay in the
+// part) are computed once, as a byte matrix Dp (255: no such path).  A
+// shortest path from the source to a boundary vertex is a chain of such
+// in-part segments joined by edges between parts, so the boundary levels of
+// the whole graph are the shortest-path distances of the small graph of
+// boundary vertices whose edges are the parts' Dp entries and the edges
+// between parts.  A traversal then has three steps:
+//   1. the source's part: a BFS inside it from the source gives its
+//      boundary vertices their first levels (lowered "from outside");
+//   2. the queue of parts as in pbfs_kernel, but a run of part p takes only
+//      its lowered boundary vertices (the seeds) and sets every boundary
+//      level to min(own, seed level + Dp[seed][b]) — one min-plus pass over
+//      rows of Dp instead of rounds of a BFS — then lowers the neighbours
+//      across its external edges.  Dp obeys the triangle inequality, so a
+//      vertex lowered by the pass needs no pass of its own: what it would
+//      offer inside p its seed already offered;
+//   3. when the queue is empty every boundary level is exact, and each part
+//      computes its interior levels in one BFS inside LDS that starts each
+//      boundary vertex at its exact level (and the source at 0): level by
+//      level, a seed joining the frontier at its own level, so every vertex
+//      is reached once and at its exact level.  The k-centers step is fused
+//      into the write-out.
+// Step 2 is still label-correcting (a part can run before all its seeds have
+// their final levels), but a run costs one pass over |seeds| rows of Dp
+// instead of tens of dependent rounds, and the interior of a part is
+// traversed once per traversal instead of at every run.
+
+constexpr unsigned char kDInf = 255;  // Dp entry: no path inside the part
+
+// The part's compact image for in-part BFS: row offsets (nv + 1), adjacency
+// (ne) and boundary list (nb), uint16, 16-byte aligned.
+__host__ __device__ constexpr int64_t cblob_bytes(int64_t nv, int64_t ne, int64_t nb) {
+  return align_up(2 * (nv + 1 + ne + nb), 16);
+}
+// LDS of part_bfs: the image, levels (int32 nv), two frontier lists (uint16,
+// nv + 1 each), the seed list (uint16 vertices and int32 levels, nb + 2).
+__host__ __device__ constexpr int64_t final_bytes(int64_t nv, int64_t ne, int64_t nb) {
+  return cblob_bytes(nv, ne, nb) + 4 * nv + align_up(4 * (nv + 1), 4) + align_up(2 * (nb + 2), 4) + 4 * (nb + 2);
+}
+// LDS of a run in step 2: boundary levels and seed levels (int32 nb each),
+// seed indices (uint16 nb), changed flags (nb) and neighbouring-part flags (nn).
+__host__ __device__ constexpr int64_t orun_bytes(int64_t nb, int64_t nn) {
+  return 8 * nb + align_up(2 * nb, 4) + align_up(nb, 4) + nn + 16;
+}
+// Row stride of Dp: whole 16-byte groups.
+__host__ __device__ constexpr int64_t drow(int64_t nb) { return align_up(nb, 16); }
+
+struct ObfsArgs {
+  DevGraph g;
+  int32_t *level;     // output: levels (internal ids)
+  int32_t *min_dist;  // nullable: fused k-centers update
+  KcResult *kc;
+  int64_t k;
+  const uint32_t *vstart, *pverts;  // part-major vertex ranges; part-major index -> internal id
+  const uint32_t *bstart;           // boundary-major ranges of the parts' boundary vertices
+  const uint32_t *xoff, *noff;      // external edges and neighbouring parts of each part
+  const uint16_t *xsrc, *xnbr;      // external edge: source (boundary index in its part), neighbour slot
+  const uint32_t *xdst, *nbr;       // external edge target (boundary-major); neighbouring parts
+  const uint4 *cblob;
+  const uint32_t *cblob_off;  // per part, in 16-byte units
+  const unsigned char *dmat;
+  const unsigned long long *doff;  // per part, bytes
+  int32_t *blevel;   // boundary-major: a part's boundary levels (written by its runs)
+  int32_t *bilevel;  // boundary-major: levels lowered from outside (atomicMin)
+  uint32_t *st, *ctl;
+  unsigned long long *ring;
+  uint32_t ring_n, src_part, src_loc;
+  // with PHDE_PBFS_STATS: [0..3] wait, seed, min-plus, push (ticks summed
+  // over workgroups); [4] runs, [5] seeds, [6] level loads, [7, 8] first
+  // and last tick, [9] interior time, [10] interior rounds, [11] boundary
+  // levels lowered by the min-plus passes, [12] end of step 2;
+  // [16..19] queued-to-taken latency; [32 + slot] fill ticks
+  unsigned long long *stats;
+};
+constexpr int kCtlFin = 32;  // ctl word: next part of step 3
+
+__global__ void obfs_init_kernel(ObfsArgs a, int64_t nbtot) {
+  const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t j = gtid; j < nbtot; j += gsize) {
+    a.blevel[j] = kUnvisited;
+    a.bilevel[j] = kUnvisited;
+  }
+  for (int64_t p = gtid; p < a.k; p += gsize) a.st[p] = p == a.src_part ? 1u : 0u;
+  for (int64_t r = gtid; r < a.ring_n; r += gsize)
+    a.ring[r * kSlotStride] = r == 0 ? (1ull << 32) | (a.src_part + 1u) : 0ull;
+  if (gtid < kCtlWords) a.ctl[gtid] = gtid == kCtlTP || gtid == kCtlTP + 1 ? 1u : 0u;  // tail 1, pending 1
+  if (gtid == 0) {
+    a.kc->key = 0;
+    a.kc->sumsq[0] = a.kc->sumsq[1] = 0;
+    a.kc->unreached = 0;
+    a.kc->depth = 0;
+  }
+}
+
+// Copies part p's compact image into LDS (16-byte loads, eight in flight
+// per thread); returns its size in 16-byte words.
+template <int Block>
+__device__ __forceinline__ void load_cblob(const uint4 *cblob, const uint32_t *cblob_off, uint32_t p,
+                                           unsigned char *smem) {
+  const uint4 *gb = cblob + cblob_off[p];
+  const int nblob = (int)(cblob_off[p + 1] - cblob_off[p]);
+  uint4 *sb = reinterpret_cast<uint4 *>(smem);
+  for (int i = threadIdx.x; i < nblob; i += 8 * Block) {
+    uint4 t[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) t[j] = gb[i + j * Block < nblob ? i + j * Block : i];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (i + j * Block < nblob) sb[i + j * Block] = t[j];
+  }
+}
+
+// Running totals of the fused k-centers step over the vertices one thread
+// wrote out.
+struct KcAcc {
+  unsigned long long key = 0, unreached = 0;
+  U128 sumsq{0, 0};
+  int32_t maxd = 0;
+};
+
+// BFS inside part p (steps 1 and 3).  Source mode (final == false): from
+// the source alone; the levels of the part's boundary vertices go to
+// bilevel.  Final mode: every boundary vertex starts at its exact level
+// (blevel) and the source, when in p, at 0; the levels are written out
+// (internal ids) with the fused k-centers step.  Level by level from the
+// lowest seed level: a seed joins the frontier at its own level, and a
+// level with an empty frontier jumps to the next seed level, so every vertex
+// is listed once.  Returns the rounds taken.
+template <int Block, int Lanes, int Claims>
+__device__ int part_bfs(const ObfsArgs &a, uint32_t p, bool final_mode, unsigned char *smem, int *s_cnt,
+                        int *s_min, int32_t *s_dummy, KcAcc &acc) {
+  const int tid = threadIdx.x;
+  const int lane = tid % kWave;
+  const uint32_t v0 = a.vstart[p];
+  const int nv = (int)(a.vstart[p + 1] - v0);
+  const uint32_t b0 = a.bstart[p];
+  const int nb = (int)(a.bstart[p + 1] - b0);
+  load_cblob<Block>(a.cblob, a.cblob_off, p, smem);
+  if (tid == 0) {
+    s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
+    *s_min = kUnvisited;
+  }
+  __syncthreads();
+  const uint16_t *lrow = reinterpret_cast<const uint16_t *>(smem);
+  const int ne = lrow[nv];
+  const uint16_t *ladj = lrow + nv + 1;
+  const uint16_t *blist = ladj + ne;
+  int32_t *lvl = reinterpret_cast<int32_t *>(smem + cblob_bytes(nv, ne, nb));
+  uint16_t *F0 = reinterpret_cast<uint16_t *>(lvl + nv);
+  uint16_t *F1 = F0 + nv + 1;
+  uint16_t *sv = reinterpret_cast<uint16_t *>(smem + cblob_bytes(nv, ne, nb) + 4 * nv + align_up(4 * (nv + 1), 4));
+  int32_t *sl = reinterpret_cast<int32_t *>(reinterpret_cast<unsigned char *>(sv) + align_up(2 * (nb + 2), 4));
+  for (int v = tid; v < nv; v += Block) lvl[v] = kUnvisited;
+  __syncthreads();
+  // seeds: the boundary at its exact levels (final mode), the source
+  for (int jj = 0; jj <= nb; jj += Block) {
+    const int j = jj + tid;
+    int u = 0;
+    int32_t L = kUnvisited;
+    if (j < nb) {
+      if (final_mode) {
+        u = blist[j];
+        L = ld_dev(&a.blevel[b0 + j]);
+      }
+    } else if (j == nb && p == a.src_part) {
+      u = (int)a.src_loc;
+      L = 0;
+    }
+    const bool add = L != kUnvisited;
+    if (add) {
+      atomicMin(&lvl[u], L);
+      atomicMin(s_min, L);
+    }
+    const unsigned long long m = __ballot(add);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&s_cnt[2], __popcll(m));
+      base = __builtin_amdgcn_readlane(base, leader);
+      if (add) {
+        const int at = base + __popcll(m & lanemask_lt());
+        sv[at] = (uint16_t)u;
+        sl[at] = L;
+      }
+    }
+  }
+  __syncthreads();
+  const int ns = s_cnt[2];
+  int32_t Lc = *s_min;
+  __syncthreads();  // every thread has read s_min before the rounds reset it
+  int cur = 0, rounds = 0;
+  if (ns > 0) {
+    for (;;) {
+      uint16_t *Fc = cur ? F1 : F0;
+      uint16_t *Fn = cur ? F0 : F1;
+      // seeds at this level join the frontier (unless a shorter path
+      // reached them, which exact seeds never allow)
+      for (int ii = 0; ii < ns; ii += Block) {
+        const int i = ii + tid;
+        bool add = false;
+        int u = 0;
+        if (i < ns && sl[i] == Lc) {
+          u = sv[i];
+          add = lvl[u] == Lc;
+        }
+        const unsigned long long m = __ballot(add);
+        if (m) {
+          const int leader = __ffsll((long long)m) - 1;
+          int base = 0;
+          if (lane == leader) base = atomicAdd(&s_cnt[cur], __popcll(m));
+          base = __builtin_amdgcn_readlane(base, leader);
+          if (add) Fc[base + __popcll(m & lanemask_lt())] = (uint16_t)u;
+        }
+      }
+      if (tid == 0) {
+        s_cnt[cur ^ 1] = 0;
+        *s_min = kUnvisited;
+      }
+      __syncthreads();
+      const int nf = s_cnt[cur];
+      if (nf == 0) {
+        // nothing at this level: jump to the next seed level, if any
+        for (int i = tid; i < ns; i += Block)
+          if (sl[i] > Lc) atomicMin(s_min, sl[i]);
+        __syncthreads();
+        const int32_t nL = *s_min;
+        __syncthreads();
+        if (nL == kUnvisited) break;
+        Lc = nL;
+        continue;
+      }
+      rounds++;
+      const int32_t nl = Lc + 1;
+      int *cn = &s_cnt[cur ^ 1];
+      for (int f0 = 0; f0 < nf; f0 += Block / Lanes) {
+        const int f = f0 + tid / Lanes;
+        int xb = 0, e = 0;
+        if (f < nf) {
+          const int u = Fc[f];
+          xb = lrow[u];
+          e = lrow[u + 1];
+        }
+        xb += tid % Lanes;
+        while (__ballot(xb < e)) {
+          int w[Claims];
+          int32_t old[Claims];
+          bool ok[Claims];
+#pragma unroll
+          for (int q = 0; q < Claims; q++) {
+            const int x = xb + q * Lanes;
+            ok[q] = x < e;
+            w[q] = ladj[ok[q] ? x : 0];
+          }
+#pragma unroll
+          for (int q = 0; q < Claims; q++) old[q] = atomicMin(ok[q] ? &lvl[w[q]] : &s_dummy[tid], nl);
+          unsigned long long mq[Claims];
+          int pre[Claims], total = 0;
+#pragma unroll
+          for (int q = 0; q < Claims; q++) {
+            mq[q] = __ballot(ok[q] && old[q] > nl);
+            pre[q] = total;
+            total += __popcll(mq[q]);
+          }
+          if (total) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(cn, total);
+            base = __builtin_amdgcn_readfirstlane(base);
+            const unsigned long long lt = lanemask_lt();
+#pragma unroll
+            for (int q = 0; q < Claims; q++)
+              if ((mq[q] >> lane) & 1ull) Fn[base + pre[q] + __popcll(mq[q] & lt)] = (uint16_t)w[q];
+          }
+          xb += Claims * Lanes;
+        }
+      }
+      __syncthreads();
+      cur ^= 1;
+      Lc = nl;
+    }
+  }
+  if (!final_mode) {
+    for (int j = tid; j < nb; j += Block) {
+      const int32_t L = lvl[blist[j]];
+      if (L != kUnvisited) st_dev(&a.bilevel[b0 + j], L);
+    }
+    return rounds;
+  }
+  // write-out with the fused k-centers step: min_dist = min(min_dist,
+  // level), the first maximiser (caller's id on ties), exact sum of
+  // squares, unreached count, largest level
+  for (int v = tid; v < nv; v += Block) {
+    const int32_t d = lvl[v];
+    const uint32_t i = a.pverts[v0 + v];
+    a.level[i] = d;
+    if (d == kUnvisited) {
+      acc.unreached++;
+    } else {
+      u128_add(acc.sumsq, (unsigned long long)d * (unsigned long long)d);
+      acc.maxd = d > acc.maxd ? d : acc.maxd;
+    }
+    if (a.min_dist) {
+      int32_t md = a.min_dist[i];
+      if (d < md) {
+        md = d;
+        a.min_dist[i] = md;
+      }
+      const unsigned long long kk = ((unsigned long long)(uint32_t)md << 32) |
+                                    (0xffffffffull - (unsigned long long)original_id(a.g, i));
+      acc.key = kk > acc.key ? kk : acc.key;
+    }
+  }
+  return rounds;
+}
+
+// Step 1: one workgroup, the source's part.
+template <int Block, int Lanes, int Claims>
+__global__ __launch_bounds__(Block) void obfs_source_kernel(ObfsArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ int s_cnt[3], s_min;
+  __shared__ int32_t s_dummy[Block];
+  KcAcc acc;
+  part_bfs<Block, Lanes, Claims>(a, a.src_part, false, smem, s_cnt, &s_min, s_dummy, acc);
+}
+
+// Steps 2 and 3: every workgroup resident; the queue of parts, then the
+// parts' interiors.
+template <int Block, int Lanes, int Claims>
+__global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ uint32_t s_part, s_ver;
+  __shared__ int s_ns, s_rerun, s_cnt[3], s_min;
+  __shared__ int32_t s_dummy[Block];
+  const int tid = threadIdx.x;
+  const int lane = tid % kWave;
+  const bool stats = a.stats != nullptr;
+  uint32_t cached = kNoPart, cached_ver = 0;
+  unsigned long long t_wait = 0, t_seed = 0, t_mp = 0, t_push = 0, n_act = 0, n_seed = 0, n_img = 0;
+  unsigned long long t_first = ~0ull, t_last = 0, t0 = 0, t1 = 0;
+  for (;;) {
+    if (stats && tid == 0) t0 = __builtin_amdgcn_s_memrealtime();
+    if (tid == 0) {
+      const uint32_t p = pop_part(a);
+      if (p != kNoPart) {
+        s_ver = atomicAdd(&a.st[p], 1u) >> 2;  // queued -> running; the part's version
+        __builtin_amdgcn_s_waitcnt(0);
+      }
+      s_part = p;
+    }
+    __syncthreads();
+    const uint32_t p = s_part;
+    if (stats && tid == 0) {
+      t1 = __builtin_amdgcn_s_memrealtime();
+      t_wait += t1 - t0;
+      if (p != kNoPart && t1 < t_first) t_first = t1;
+    }
+    if (p == kNoPart) break;
+    const uint32_t b0 = a.bstart[p];
+    const int nb = (int)(a.bstart[p + 1] - b0);
+    const int nbp = (int)drow(nb);
+    const uint32_t x0 = a.xoff[p];
+    const int nx = (int)(a.xoff[p + 1] - x0);
+    const uint32_t n0 = a.noff[p];
+    const int nn = (int)(a.noff[p + 1] - n0);
+    int32_t *lv = reinterpret_cast<int32_t *>(smem);
+    int32_t *sl = lv + nb;
+    uint16_t *sj = reinterpret_cast<uint16_t *>(sl + nb);
+    unsigned char *chg = reinterpret_cast<unsigned char *>(smem + 8 * nb + align_up(2 * nb, 4));
+    unsigned char *nflag = chg + align_up(nb, 4);
+    const unsigned char *D = a.dmat + a.doff[p];
+    for (;;) {
+      if (stats && tid == 0) {
+        t0 = __builtin_amdgcn_s_memrealtime();
+        n_act++;
+      }
+      if (cached != p || cached_ver != s_ver) {
+        for (int j = tid; j < nb; j += Block) lv[j] = ld_dev(&a.blevel[b0 + j]);
+        if (stats && tid == 0) n_img++;
+      }
+      for (int j = tid; j < nb; j += Block) chg[j] = 0;
+      for (int i = tid; i < nn; i += Block) nflag[i] = 0;
+      if (tid == 0) s_ns = 0;
+      __syncthreads();
+      // seeds: boundary vertices lowered from outside
+      for (int jj = 0; jj < nb; jj += Block) {
+        const int j = jj + tid;
+        bool add = false;
+        int32_t il = 0;
+        if (j < nb) {
+          il = ld_dev(&a.bilevel[b0 + j]);
+          if (il < lv[j]) {
+            lv[j] = il;
+            chg[j] = 1;
+            add = true;
+          }
+        }
+        const unsigned long long m = __ballot(add);
+        if (m) {
+          const int leader = __ffsll((long long)m) - 1;
+          int base = 0;
+          if (lane == leader) base = atomicAdd(&s_ns, __popcll(m));
+          base = __builtin_amdgcn_readlane(base, leader);
+          if (add) {
+            const int at = base + __popcll(m & lanemask_lt());
+            sj[at] = (uint16_t)j;
+            sl[at] = il;
+          }
+        }
+      }
+      __syncthreads();
+      const int ns = s_ns;
+      if (stats && tid == 0) {
+        t1 = __builtin_amdgcn_s_memrealtime();
+        t_seed += t1 - t0;
+        t0 = t1;
+        n_seed += ns;
+      }
+      // min-plus: lv[b] = min(lv[b], sl[i] + Dp[sj[i]][b]) over the seeds;
+      // a thread takes one 16-column group of the rows of every R-th seed
+      const int G = nbp / 16;
+      if (ns > 0 && G <= Block) {
+        const int R = Block / G;
+        const int g = tid % G, r = tid / G;
+        if (r < R) {
+          int32_t mn[16];
+#pragma unroll
+          for (int c = 0; c < 16; c++) mn[c] = kUnvisited;
+          for (int i0 = r; i0 < ns; i0 += 4 * R) {
+            uint4 w[4];
+            int32_t L[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              const int i = i0 + q * R;
+              const int ii = i < ns ? i : i0;
+              w[q] = *reinterpret_cast<const uint4 *>(D + (size_t)sj[ii] * nbp + 16 * g);
+              L[q] = sl[ii];
+              if (i >= ns) w[q] = make_uint4(~0u, ~0u, ~0u, ~0u);  // past the last seed: no path
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              const uint32_t wq[4] = {w[q].x, w[q].y, w[q].z, w[q].w};
+#pragma unroll
+              for (int c = 0; c < 16; c++) {
+                const int32_t b = (int32_t)((wq[c / 4] >> (8 * (c % 4))) & 0xffu);
+                const int32_t cand = b == kDInf ? kUnvisited : L[q] + b;
+                mn[c] = cand < mn[c] ? cand : mn[c];
+              }
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < 16; c++) {
+            const int col = 16 * g + c;
+            if (col < nb && mn[c] < lv[col]) {
+              const int32_t old = atomicMin(&lv[col], mn[c]);
+              if (old > mn[c]) chg[col] = 1;
+            }
+          }
+        }
+      } else if (ns > 0) {
+        for (int64_t it = tid; it < (int64_t)ns * G; it += Block) {
+          const int i = (int)(it / G), g = (int)(it % G);
+          const uint4 w = *reinterpret_cast<const uint4 *>(D + (size_t)sj[i] * nbp + 16 * g);
+          const uint32_t wq[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+          for (int c = 0; c < 16; c++) {
+            const int col = 16 * g + c;
+            const int32_t b = (int32_t)((wq[c / 4] >> (8 * (c % 4))) & 0xffu);
+            if (col < nb && b != kDInf) {
+              const int32_t cand = sl[i] + b;
+              if (cand < lv[col]) {
+                const int32_t old = atomicMin(&lv[col], cand);
+                if (old > cand) chg[col] = 1;
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (stats && tid == 0) {
+        t1 = __builtin_amdgcn_s_memrealtime();
+        t_mp += t1 - t0;
+        t0 = t1;
+      }
+      // write back the lowered levels, lower the neighbours across the
+      // external edges (four in flight per thread) and flag their parts
+      for (int j = tid; j < nb; j += Block)
+        if (chg[j]) {
+          st_dev(&a.blevel[b0 + j], lv[j]);
+          if (stats) atomicAdd(&a.stats[11], 1ull);
+        }
+      for (int xx = 0; xx < nx; xx += 4 * Block) {
+        int32_t nl[4], old[4];
+        bool on[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int x = xx + q * Block + tid;
+          on[q] = false;
+          nl[q] = 0;
+          if (x < nx) {
+            const int j = a.xsrc[x0 + x];
+            on[q] = chg[j] != 0;
+            nl[q] = lv[j] + 1;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          old[q] = on[q] ? atomicMin(&a.bilevel[a.xdst[x0 + xx + q * Block + tid]], nl[q]) : 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (on[q] && old[q] > nl[q]) nflag[a.xnbr[x0 + xx + q * Block + tid]] = 1;
+      }
+      __syncthreads();
+      for (int i = tid; i < nn; i += Block)
+        if (nflag[i]) mark_lowered(a, a.nbr[n0 + i]);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (stats && tid == 0) {
+        t1 = __builtin_amdgcn_s_memrealtime();
+        t_push += t1 - t0;
+        if (t1 > t_last) t_last = t1;
+      }
+      if (tid == 0) {
+        const uint32_t v = s_ver;
+        s_ver = v + 1u;
+        const uint32_t old = atomicCAS(&a.st[p], v << 2 | 2u, (v + 1u) << 2);
+        if (old == (v << 2 | 2u)) {
+          s_rerun = 0;
+          const unsigned long long tp = atomicSub(tp_word(a.ctl), 1ull);
+          if ((uint32_t)tp == 1u) mark_done(a, (uint32_t)(tp >> 32));  // the last part: step 2 is over
+        } else {
+          // lowered again while it ran: run it once more
+          atomicExch(&a.st[p], (v + 1u) << 2 | 2u);
+          __builtin_amdgcn_s_waitcnt(0);
+          s_rerun = 1;
+        }
+      }
+      __syncthreads();
+      cached = p;
+      cached_ver = s_ver;
+      if (!s_rerun) break;
+    }
+  }
+  if (stats && tid == 0) {
+    atomicAdd(&a.stats[0], t_wait);
+    atomicAdd(&a.stats[1], t_seed);
+    atomicAdd(&a.stats[2], t_mp);
+    atomicAdd(&a.stats[3], t_push);
+    atomicAdd(&a.stats[4], n_act);
+    atomicAdd(&a.stats[5], n_seed);
+    atomicAdd(&a.stats[6], n_img);
+    atomicMin(&a.stats[7], t_first);
+    atomicMax(&a.stats[8], t_last);
+    atomicMax(&a.stats[12], __builtin_amdgcn_s_memrealtime());
+  }
+  // the queue is empty and every run has completed: the boundary levels
+  // are exact (unless an error stopped the search; the host then repeats
+  // the traversal another way)
+  if (ld_dev(&a.ctl[kCtlErr])) return;
+  KcAcc acc;
+  unsigned long long t_fin = 0, n_fr = 0;
+  if (stats && tid == 0) t0 = __builtin_amdgcn_s_memrealtime();
+  for (;;) {
+    if (tid == 0) s_part = atomicAdd(&a.ctl[kCtlFin], 1u);
+    __syncthreads();
+    const uint32_t p = s_part;
+    if ((int64_t)p >= a.k) break;
+    n_fr += part_bfs<Block, Lanes, Claims>(a, p, true, smem, s_cnt, &s_min, s_dummy, acc);
+    __syncthreads();
+  }
+  if (stats && tid == 0) {
+    t_fin = __builtin_amdgcn_s_memrealtime() - t0;
+    atomicAdd(&a.stats[9], t_fin);
+    atomicAdd(&a.stats[10], n_fr);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long ok = __shfl_xor(acc.key, o);
+    acc.key = ok > acc.key ? ok : acc.key;
+    acc.unreached += __shfl_xor(acc.unreached, o);
+  }
+  const U128 sumsq = wave_sum_u128(acc.sumsq);
+  const int32_t maxd = wave_max_i32(acc.maxd);
+  if (lane == 0) {
+    if (a.min_dist && acc.key) atomicMax(&a.kc->key, acc.key);
+    atomic_add_u128(a.kc->sumsq, sumsq);
+    if (acc.unreached) atomicAdd(&a.kc->unreached, acc.unreached);
+    if (maxd > 0) atomicMax(reinterpret_cast<unsigned long long *>(&a.kc->depth), (unsigned long long)maxd);
+  }
+}
+
+// Dp of every part (context creation): bit-parallel BFS inside the part
+// from W = 8 sizeof(M) of its boundary vertices at a time (bit i of a
+// vertex's mask: reached from boundary vertex s0 + i), level-synchronous
+// and pulled (a vertex ORs its neighbours' newly reached bits).  A boundary
+// vertex newly reached at level L gets L in the rows of those sources.
+struct DmatArgs {
+  int64_t k;
+  const uint32_t *vstart, *bstart, *cblob_off;
+  const uint4 *cblob;
+  unsigned char *dmat;
+  const unsigned long long *doff;
+  unsigned int *err;  // set when an in-part distance reaches kDInf
+};
+
+template <typename M, int Block>
+__global__ __launch_bounds__(Block) void dmat_kernel(DmatArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ int s_any[2];
+  constexpr int W = 8 * (int)sizeof(M);
+  const int tid = threadIdx.x;
+  for (int64_t pp = blockIdx.x; pp < a.k; pp += gridDim.x) {
+    const uint32_t p = (uint32_t)pp;
+    const uint32_t v0 = a.vstart[p];
+    const int nv = (int)(a.vstart[p + 1] - v0);
+    const int nb = (int)(a.bstart[p + 1] - a.bstart[p]);
+    const int nbp = (int)drow(nb);
+    load_cblob<Block>(a.cblob, a.cblob_off, p, smem);
+    __syncthreads();
+    const uint16_t *lrow = reinterpret_cast<const uint16_t *>(smem);
+    const int ne = lrow[nv];
+    const uint16_t *ladj = lrow + nv + 1;
+    const uint16_t *blist = ladj + ne;
+    uint16_t *bpos = reinterpret_cast<uint16_t *>(smem + cblob_bytes(nv, ne, nb));
+    M *vis = reinterpret_cast<M *>(smem + cblob_bytes(nv, ne, nb) + align_up(2 * nv, 16));
+    M *cur = vis + nv;
+    M *nxt = cur + nv;
+    for (int v = tid; v < nv; v += Block) bpos[v] = 0xffff;
+    __syncthreads();
+    for (int j = tid; j < nb; j += Block) bpos[blist[j]] = (uint16_t)j;
+    unsigned char *D = a.dmat + a.doff[p];
+    for (int s0 = 0; s0 < nb; s0 += W) {
+      const int sw = nb - s0 < W ? nb - s0 : W;
+      const M full = (M)(sw == W ? ~0ull : ((1ull << sw) - 1ull));
+      for (int v = tid; v < nv; v += Block) vis[v] = cur[v] = 0;
+      __syncthreads();
+      for (int i = tid; i < sw; i += Block) {
+        const int v = blist[s0 + i];
+        vis[v] = cur[v] = (M)(1ull << i);
+        D[(size_t)(s0 + i) * nbp + s0 + i] = 0;
+      }
+      if (tid == 0) s_any[0] = s_any[1] = 0;
+      __syncthreads();
+      for (int L = 1;; L++) {
+        for (int v = tid; v < nv; v += Block) {
+          M nw = 0;
+          const M vv = vis[v];
+          if (vv != full) {
+            M acc = 0;
+            for (int x = lrow[v]; x < lrow[v + 1]; x++) acc |= cur[ladj[x]];
+            nw = (M)(acc & (M)~vv);
+          }
+          nxt[v] = nw;
+        }
+        __syncthreads();
+        for (int v = tid; v < nv; v += Block) {
+          M nw = nxt[v];
+          cur[v] = nw;
+          if (nw) {
+            vis[v] |= nw;
+            s_any[L & 1] = 1;
+            const int j = bpos[v];
+            if (j != 0xffff) {
+              if (L >= (int)kDInf) {
+                atomicOr(a.err, 1u);
+              } else {
+                unsigned long long bits = (unsigned long long)nw;
+                while (bits) {
+                  const int i = __ffsll((long long)bits) - 1;
+                  bits &= bits - 1ull;
+                  D[(size_t)(s0 + i) * nbp + j] = (unsigned char)L;
+                }
+              }
+            }
+          }
+        }
+        if (tid == 0) s_any[(L + 1) & 1] = 0;
+        __syncthreads();
+        if (!s_any[L & 1]) break;
+      }
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ host side
 
 // Workgroup shapes: threads, lanes per frontier vertex, claims per lane.
@@ -579,11 +1271,38 @@ struct PartGraph {
   bool disabled = false;
   PbfsShape shape;
   unsigned long long *stats = nullptr;  // PHDE_PBFS_STATS
+  // the boundary-distance traversal (ovl false: not built, or PHDE_OBFS=0)
+  bool ovl = false;
+  int64_t nbtot = 0;
+  uint32_t *pverts = nullptr, *bstart = nullptr, *xoff = nullptr, *noff = nullptr, *xdst = nullptr, *nbr = nullptr,
+           *cblob_off = nullptr;
+  uint16_t *xsrc = nullptr, *xnbr = nullptr;
+  uint4 *cblob = nullptr;
+  unsigned char *dmat = nullptr;
+  unsigned long long *doff = nullptr;
+  int32_t *blevel = nullptr, *bilevel = nullptr;
 };
+
+// Frees the boundary-distance traversal's arrays (the partitioned traversal
+// stays usable).
+static void obfs_release(PartGraph *pg) {
+  void *ptr[] = {pg->pverts, pg->bstart, pg->xoff,  pg->noff, pg->xdst,   pg->nbr,    pg->cblob_off,
+                 pg->xsrc,   pg->xnbr,   pg->cblob, pg->dmat, pg->doff,   pg->blevel, pg->bilevel};
+  for (void *x : ptr)
+    if (x) (void)hipFree(x);
+  pg->pverts = pg->bstart = pg->xoff = pg->noff = pg->xdst = pg->nbr = pg->cblob_off = nullptr;
+  pg->xsrc = pg->xnbr = nullptr;
+  pg->cblob = nullptr;
+  pg->dmat = nullptr;
+  pg->doff = nullptr;
+  pg->blevel = pg->bilevel = nullptr;
+  pg->ovl = false;
+}
 
 void pbfs_release(phde_ctx *c) {
   PartGraph *pg = c->pg;
   if (!pg) return;
+  obfs_release(pg);
   void *ptr[] = {pg->vstart, pg->pmidx, pg->pcount, pg->blob_off, pg->blob, pg->plevel, pg->ilevel,
                  pg->st,     pg->ring,  pg->ctl,    pg->stats};
   for (void *x : ptr)
@@ -596,6 +1315,123 @@ template <typename T>
 static int upload(T **dst, const std::vector<T> &src) {
   PHDE_HIP(hipMalloc(dst, sizeof(T) * std::max<size_t>(src.size(), 1)));
   if (!src.empty()) PHDE_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+  return PHDE_OK;
+}
+
+// Host-side arrays of the boundary-distance traversal, built part by part
+// with the partition.
+struct ObfsHost {
+  std::vector<uint32_t> bmidx, bstart, xoff{0}, xdst, noff{0}, nbr, coff{0};
+  std::vector<uint16_t> xsrc, xnbr;
+  std::vector<uint4> cblob;
+  std::vector<unsigned long long> doff{0};
+  // largest LDS of step 3 (and 1), of a run of step 2, and of dmat_kernel
+  // with 32-, 16- and 8-bit masks
+  int64_t max_final = 0, max_orun = 0, max_dm[3] = {0, 0, 0}, max_nb = 0;
+  bool ok = true;
+  void add_part(int64_t p, uint32_t v0, uint32_t nv, const std::vector<uint32_t> &verts,
+                const std::vector<uint16_t> &lrow, const std::vector<uint16_t> &ladj,
+                const std::vector<uint16_t> &blist, const std::vector<uint16_t> &pxsrc,
+                const std::vector<uint32_t> &pxdst, const std::vector<uint16_t> &pxnbr,
+                const std::vector<uint32_t> &pnbr) {
+    if (!ok) return;
+    const uint32_t b0 = bstart[p];
+    const int64_t ne = ladj.size(), nb = blist.size(), nn = pnbr.size();
+    // external edges come only from boundary vertices; pxdst holds the
+    // target's part-major index
+    for (size_t x = 0; x < pxsrc.size(); x++) {
+      xsrc.push_back((uint16_t)(bmidx[verts[v0 + pxsrc[x]]] - b0));
+      xdst.push_back(bmidx[verts[pxdst[x]]]);
+      xnbr.push_back(pxnbr[x]);
+    }
+    xoff.push_back((uint32_t)xsrc.size());
+    nbr.insert(nbr.end(), pnbr.begin(), pnbr.end());
+    noff.push_back((uint32_t)nbr.size());
+    const int64_t cb = cblob_bytes(nv, ne, nb);
+    std::vector<uint16_t> b16(cb / 2, 0);
+    std::copy(lrow.begin(), lrow.end(), b16.begin());
+    std::copy(ladj.begin(), ladj.end(), b16.begin() + lrow.size());
+    std::copy(blist.begin(), blist.end(), b16.begin() + lrow.size() + ladj.size());
+    const size_t at = cblob.size();
+    cblob.resize(at + cb / 16);
+    std::memcpy(&cblob[at], b16.data(), cb);
+    coff.push_back((uint32_t)cblob.size());
+    doff.push_back(doff.back() + (unsigned long long)(nb * drow(nb)));
+    max_final = std::max(max_final, final_bytes(nv, ne, nb));
+    max_orun = std::max(max_orun, orun_bytes(nb, nn));
+    max_nb = std::max(max_nb, nb);
+    const int64_t w[3] = {4, 2, 1};
+    for (int i = 0; i < 3; i++) max_dm[i] = std::max(max_dm[i], cb + align_up(2 * nv, 16) + 3 * w[i] * nv);
+    if (xsrc.size() >= UINT32_MAX || nbr.size() >= UINT32_MAX || cblob.size() >= UINT32_MAX) ok = false;
+  }
+};
+
+// Builds the boundary-distance traversal over the parts just cut: uploads
+// its arrays and computes every part's Dp on the GPU.  Leaves pg->ovl false
+// (the partitioned traversal alone) when PHDE_OBFS=0, when a part's LDS
+// would not fit, or when an in-part distance reaches 255.
+static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vector<uint32_t> &verts) {
+  if (const char *e = getenv("PHDE_OBFS"))
+    if (atoi(e) == 0) return PHDE_OK;
+  // a level plus an in-part distance (< 255) must stay below kUnvisited
+  if (c->g.n >= (int64_t)kUnvisited - 1024) return PHDE_OK;
+  if (!oh.ok || std::max(oh.max_final, oh.max_orun) > kPartBytes) return PHDE_OK;
+  int mask = -1;
+  for (int i = 0; i < 3 && mask < 0; i++)
+    if (oh.max_dm[i] <= kPartBytes) mask = i;
+  if (mask < 0) return PHDE_OK;
+  PHDE_TRY_RC(upload(&pg->pverts, verts));
+  PHDE_TRY_RC(upload(&pg->bstart, oh.bstart));
+  PHDE_TRY_RC(upload(&pg->xoff, oh.xoff));
+  PHDE_TRY_RC(upload(&pg->noff, oh.noff));
+  PHDE_TRY_RC(upload(&pg->xdst, oh.xdst));
+  PHDE_TRY_RC(upload(&pg->nbr, oh.nbr));
+  PHDE_TRY_RC(upload(&pg->cblob_off, oh.coff));
+  PHDE_TRY_RC(upload(&pg->xsrc, oh.xsrc));
+  PHDE_TRY_RC(upload(&pg->xnbr, oh.xnbr));
+  PHDE_TRY_RC(upload(&pg->cblob, oh.cblob));
+  PHDE_TRY_RC(upload(&pg->doff, oh.doff));
+  pg->nbtot = oh.bstart.back();
+  const size_t dbytes = std::max<size_t>(oh.doff.back(), 16);
+  PHDE_HIP(hipMalloc(&pg->dmat, dbytes));
+  PHDE_HIP(hipMalloc(&pg->blevel, sizeof(int32_t) * std::max<int64_t>(pg->nbtot, 1)));
+  PHDE_HIP(hipMalloc(&pg->bilevel, sizeof(int32_t) * std::max<int64_t>(pg->nbtot, 1)));
+  PHDE_HIP(hipMemsetAsync(pg->dmat, kDInf, dbytes, c->stream));
+  DBuf<unsigned int> err;
+  PHDE_TRY_RC(err.alloc(1));
+  PHDE_HIP(hipMemsetAsync(err.p, 0, sizeof(unsigned int), c->stream));
+  DmatArgs da;
+  da.k = pg->k;
+  da.vstart = pg->vstart;
+  da.bstart = pg->bstart;
+  da.cblob_off = pg->cblob_off;
+  da.cblob = pg->cblob;
+  da.dmat = pg->dmat;
+  da.doff = pg->doff;
+  da.err = err.p;
+  void (*dk)(DmatArgs) = mask == 0 ? dmat_kernel<uint32_t, 1024> : mask == 1 ? dmat_kernel<uint16_t, 1024>
+                                                                             : dmat_kernel<uint8_t, 1024>;
+  const size_t dl = (size_t)oh.max_dm[mask];
+  PHDE_HIP(hipFuncSetAttribute((const void *)dk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dl));
+  const auto t0 = std::chrono::steady_clock::now();
+  const int dgrid = (int)std::min<int64_t>(pg->k, (int64_t)1 << 20);
+  dk<<<dgrid, 1024, dl, c->stream>>>(da);
+  PHDE_HIP(hipGetLastError());
+  unsigned int h_err = 0;
+  PHDE_HIP(hipMemcpyAsync(&h_err, err.p, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (const void *f : {(const void *)obfs_kernel<1024, 16, 1>, (const void *)obfs_source_kernel<1024, 16, 1>})
+    PHDE_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kPartBytes));
+  if (pg->stats)
+    std::fprintf(stderr, "[obfs] boundary %lld (largest part %lld), Dp %.1f MB in %.1f ms (%d-bit masks)%s\n",
+                 (long long)pg->nbtot, (long long)oh.max_nb, oh.doff.back() / 1e6, ms, 32 >> mask,
+                 h_err ? ": an in-part distance reached 255, not used" : "");
+  if (h_err) {
+    obfs_release(pg);
+    return PHDE_OK;
+  }
+  pg->ovl = true;
   return PHDE_OK;
 }
 
@@ -685,6 +1521,21 @@ int pbfs_setup(phde_ctx *c) {
   std::vector<uint4> blob;
   blob.reserve((size_t)(2 * m + 8 * n) / 16 + k);
   int64_t mn = n, mx = 0, worst = 0, next = 0, nbtot = 0;
+  // the boundary-distance traversal's arrays: boundary-major numbering (part
+  // by part, each part's boundary vertices in local order), external edges
+  // from boundary index to boundary-major index, compact images, and the
+  // byte offsets of the parts' Dp
+  ObfsHost oh;
+  oh.bmidx.assign(n, 0xffffffffu);
+  oh.bstart.push_back(0);
+  {
+    uint32_t nbt = 0;
+    for (int64_t p = 0; p < k; p++) {
+      for (uint32_t i = vstart[p]; i < vstart[p + 1]; i++)
+        if (boundary[verts[i]]) oh.bmidx[verts[i]] = nbt++;
+      oh.bstart.push_back(nbt);
+    }
+  }
   for (int64_t p = 0; p < k; p++) {
     const uint32_t v0 = vstart[p], nv = vstart[p + 1] - v0;
     std::vector<uint16_t> lrow, ladj, blist, xsrc, xnbr;
@@ -737,13 +1588,14 @@ int pbfs_setup(phde_ctx *c) {
     mx = std::max<int64_t>(mx, nv);
     worst = std::max(worst, part_bytes(nv, ne, nb, nx, nn));
     nbtot += nb;
+    oh.add_part(p, v0, nv, verts, lrow, ladj, blist, xsrc, xdst, xnbr, nbr);
   }
   if (worst > kPartBytes) return PHDE_OK;  // cannot happen: the growth counted every vertex as boundary
   PartGraph *pg = new PartGraph();
   c->pg = pg;
   pg->k = k;
   pg->shape = shape;
-  pg->ring_n = (uint32_t)(k + std::max(1, c->num_cus) + 64);
+  pg->ring_n = (uint32_t)(k + 4 * std::max(1, c->num_cus) + 64);
   PHDE_TRY_RC(upload(&pg->vstart, vstart));
   PHDE_TRY_RC(upload(&pg->pmidx, pmidx));
   PHDE_TRY_RC(upload(&pg->pcount, pcount));
@@ -764,15 +1616,107 @@ int pbfs_setup(phde_ctx *c) {
   pg->h_part_of.swap(part_of);
   pg->h_loc.resize(n);
   for (int64_t v = 0; v < n; v++) pg->h_loc[v] = loc[v];
-  return PHDE_OK;
+  return obfs_setup(c, pg, oh, verts);
 }
 
 int64_t pbfs_num_parts(const phde_ctx *c) { return c->pg && !c->pg->disabled ? c->pg->k : 0; }
+int64_t pbfs_num_boundary(const phde_ctx *c) {
+  return c->pg && !c->pg->disabled && c->pg->ovl ? c->pg->nbtot : -1;
+}
+
+// One traversal by the boundary-distance method (steps 1-3 above).
+static int obfs_run(phde_ctx *c, PartGraph *pg, int64_t src, int32_t *dist_out, int32_t *min_dist, bool *ran) {
+  ObfsArgs a;
+  a.g = c->g;
+  a.level = dist_out ? dist_out : c->d_dist;
+  a.min_dist = min_dist;
+  a.kc = c->d_kc;
+  a.k = pg->k;
+  a.vstart = pg->vstart;
+  a.pverts = pg->pverts;
+  a.bstart = pg->bstart;
+  a.xoff = pg->xoff;
+  a.noff = pg->noff;
+  a.xsrc = pg->xsrc;
+  a.xnbr = pg->xnbr;
+  a.xdst = pg->xdst;
+  a.nbr = pg->nbr;
+  a.cblob = pg->cblob;
+  a.cblob_off = pg->cblob_off;
+  a.dmat = pg->dmat;
+  a.doff = pg->doff;
+  a.blevel = pg->blevel;
+  a.bilevel = pg->bilevel;
+  a.st = pg->st;
+  a.ctl = pg->ctl;
+  a.ring = pg->ring;
+  a.ring_n = pg->ring_n;
+  a.src_part = pg->h_part_of[src];
+  a.src_loc = pg->h_loc[src];
+  a.stats = pg->stats;
+  if (pg->stats) {
+    const unsigned long long init[32] = {0, 0, 0, 0, 0, 0, 0, ~0ull, 0};
+    PHDE_HIP(hipMemcpyAsync(pg->stats, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  }
+  const auto fn = obfs_kernel<1024, 16, 1>;
+  int resident = 0;
+  PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, 1024, kPartBytes));
+  const int grid = std::max(1, c->num_cus) * std::max(1, resident);
+  if (resident < 1 || (int64_t)pg->ring_n < pg->k + grid) {
+    obfs_release(pg);
+    return PHDE_OK;
+  }
+  const int ib = (int)std::max<int64_t>(
+      1, std::min<int64_t>((std::max<int64_t>(pg->nbtot, pg->ring_n) + 255) / 256, (int64_t)c->num_cus * 8));
+  obfs_init_kernel<<<ib, 256, 0, c->stream>>>(a, pg->nbtot);
+  PHDE_HIP(hipGetLastError());
+  obfs_source_kernel<1024, 16, 1><<<1, 1024, kPartBytes, c->stream>>>(a);
+  PHDE_HIP(hipGetLastError());
+  void *args[] = {&a};
+  const hipError_t e = launch_resident((const void *)fn, grid, 1024, args, kPartBytes, c->stream);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    obfs_release(pg);
+    return PHDE_OK;
+  }
+  unsigned int flag = 0;
+  PHDE_HIP(hipMemcpyAsync(c->h_kc, c->d_kc, sizeof(KcResult), hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipMemcpyAsync(&flag, pg->ctl + kCtlErr, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  if (flag) {
+    // a wait timed out: this context goes back to the partitioned traversal
+    std::fprintf(stderr, "parhde: boundary-distance BFS stopped (queue %u); using the partitioned traversal\n", flag);
+    obfs_release(pg);
+    return PHDE_OK;
+  }
+  if (pg->stats) {
+    unsigned long long s[32];
+    PHDE_HIP(hipMemcpy(s, pg->stats, sizeof(s), hipMemcpyDeviceToHost));
+    const double us = 0.01;  // ticks of 10 ns
+    const double na = (double)std::max(1ull, s[4]);
+    std::fprintf(stderr,
+                 "[obfs] src %lld: step 2 %.1f us (to last release %.1f us); summed over workgroups: wait %.0f seed "
+                 "%.0f min-plus %.0f push %.0f us; %llu runs (%.2f per part), %.1f seeds and %.1f lowered per run, "
+                 "%llu level loads; per run: seed %.2f min-plus %.2f push %.2f us; step 3 %.0f us summed, %llu "
+                 "rounds (%.1f per part)\n",
+                 (long long)src, (s[12] - s[7]) * us, (s[8] - s[7]) * us, s[0] * us, s[1] * us, s[2] * us, s[3] * us,
+                 s[4], s[4] / (double)pg->k, s[5] / na, s[11] / na, s[6], s[1] * us / na, s[2] * us / na,
+                 s[3] * us / na, s[9] * us, s[10], s[10] / (double)pg->k);
+    std::fprintf(stderr, "[obfs]   queue latency (queued to taken): %llu parts, mean %.2f us, %llu above 10 us, max %.1f us\n",
+                 s[17], s[16] * 0.01 / (double)std::max(1ull, s[17]), s[18], s[19] * 0.01);
+  }
+  *ran = true;
+  return PHDE_OK;
+}
 
 int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, bool *ran) {
   *ran = false;
   PartGraph *pg = c->pg;
   if (!pg || pg->disabled || !c->bfs_persistent) return PHDE_OK;
+  if (pg->ovl) {
+    PHDE_TRY_RC(obfs_run(c, pg, src, dist_out, min_dist, ran));
+    if (*ran) return PHDE_OK;
+  }
   PbfsArgs a;
   a.g = c->g;
   a.level = dist_out ? dist_out : c->d_dist;

@@ -157,6 +157,12 @@ class Context:
         _check(self.lib.phde_num_parts(self.h, ctypes.byref(k)), "phde_num_parts")
         return k.value
 
+    def num_boundary(self) -> int:
+        """Boundary vertices of the boundary-distance traversal (-1: not in use)."""
+        k = ctypes.c_int64()
+        _check(self.lib.phde_num_boundary(self.h, ctypes.byref(k)), "phde_num_boundary")
+        return k.value
+
     def sssp(self, source: int, delta: int = 0) -> np.ndarray:
         """Exact distances (DeltaStep, gapbs/sssp.cc:53); 2^30-1 when unreached."""
         d = np.empty(self.n, dtype=np.int32)

@@ -153,6 +153,14 @@ int phde_vertex_order(const phde_ctx *ctx, uint32_t *order, int *renumbered);
  * part (same levels).  *parts = 0 when the context does not use it
  * (low-diameter or weighted graphs, PHDE_PBFS=0). */
 int phde_num_parts(const phde_ctx *ctx, int64_t *parts);
+/* Boundary vertices of those parts when the context's traversals run by
+ * the boundary-distance method (no reference counterpart): each part's
+ * boundary-to-boundary distances inside the part are computed at creation,
+ * and a traversal propagates levels between parts over those distances
+ * before filling in each part's interior (same levels).  *boundary = -1
+ * when the context does not use it (as above, PHDE_OBFS=0, or an in-part
+ * distance of 255 or more). */
+int phde_num_boundary(const phde_ctx *ctx, int64_t *boundary);
 
 /* Breadth-first levels from one source: DOBFS (gapbs/bfs.cc:144-185).
  * dist[v] = hop count, -1 when unreached.  alpha/beta <= 0 use 15/18. */

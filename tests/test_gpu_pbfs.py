@@ -22,8 +22,8 @@ pytestmark = pytest.mark.gpu
 
 
 @contextmanager
-def context(g, part_bytes=None, pbfs=True):
-    env = {"PHDE_RELABEL": "1", "PHDE_PBFS": "1" if pbfs else "0"}
+def context(g, part_bytes=None, pbfs=True, obfs=True):
+    env = {"PHDE_RELABEL": "1", "PHDE_PBFS": "1" if pbfs else "0", "PHDE_OBFS": "1" if obfs else "0"}
     if part_bytes:
         env["PHDE_PBFS_BYTES"] = str(part_bytes)
     old = {k: os.environ.get(k) for k in env}
@@ -55,26 +55,41 @@ GRAPHS = {
 BUDGETS = [None, 4096, 600]
 
 
+MODES = ["obfs", "pbfs"]  # boundary-distance method, or rounds in LDS at every run
+
+
+def check_mode(ctx, mode):
+    parts = ctx.num_parts()
+    assert parts >= 1
+    if mode == "obfs":
+        assert ctx.num_boundary() >= 0
+    else:
+        assert ctx.num_boundary() == -1
+    return parts
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("budget", BUDGETS)
 @pytest.mark.parametrize("name", list(GRAPHS))
-def test_levels_exact(gpu, name, budget):
+def test_levels_exact(gpu, name, budget, mode):
     g = GRAPHS[name]()
     rng = np.random.default_rng(7)
     srcs = [0, g.n - 1, g.n // 2] + [int(x) for x in rng.choice(g.n, 4, replace=False)]
-    with context(g, budget) as ctx:
-        parts = ctx.num_parts()
-        assert parts >= 1, name
+    with context(g, budget, obfs=mode == "obfs") as ctx:
+        parts = check_mode(ctx, mode)
         for s in srcs:
             want = O.seq_bfs(g, s)
             assert np.array_equal(ctx.bfs(s), want), (name, budget, parts, s)
-    print(f"{name} budget {budget}: {parts} parts, {len(srcs)} sources exact")
+        nbd = ctx.num_boundary()
+    print(f"{name} budget {budget} {mode}: {parts} parts, {nbd} boundary vertices, {len(srcs)} sources exact")
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("budget", BUDGETS)
-def test_kcenters_same_as_without(gpu, budget):
+def test_kcenters_same_as_without(gpu, budget, mode):
     g = GRAPHS["holed"]()
-    with context(g, budget) as ctx:
-        assert ctx.num_parts() >= 1
+    with context(g, budget, obfs=mode == "obfs") as ctx:
+        check_mode(ctx, mode)
         a = ctx.kcenters(30, first=11)
     with context(g, pbfs=False) as ctx:
         assert ctx.num_parts() == 0
@@ -82,13 +97,14 @@ def test_kcenters_same_as_without(gpu, budget):
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("budget", [None, 4096])
 @pytest.mark.parametrize("name,s", [("holed", 30), ("rgg", 50), ("mesh", 20)])
-def test_layout_parity(gpu, name, s, budget):
+def test_layout_parity(gpu, name, s, budget, mode):
     g = GRAPHS[name]()
     ref = O.parhde(g, s, 2, lazy=True)
-    with context(g, budget) as ctx:
-        assert ctx.num_parts() >= 1
+    with context(g, budget, obfs=mode == "obfs") as ctx:
+        check_mode(ctx, mode)
         X, st = ctx.layout(s=s, num_coord=2)
         Xe, ste = ctx.layout(s=s, num_coord=2, eager=1)
     assert st.pivots == ref["pivots"]
@@ -96,9 +112,11 @@ def test_layout_parity(gpu, name, s, budget):
     assert np.array_equal(X, Xe) and ste.n_traversals == 3 * s
 
 
-def test_unreached_component(gpu):
+@pytest.mark.parametrize("mode", MODES)
+def test_unreached_component(gpu, mode):
     g = two_paths(400)
-    with context(g, 600) as ctx:
+    with context(g, 600, obfs=mode == "obfs") as ctx:
+        check_mode(ctx, mode)
         d = ctx.bfs(3)
         assert d[400:].tolist() == [-1] * 400 and d[:400].tolist() == [abs(i - 3) for i in range(400)]
         with pytest.raises(L.PhdeError) as e:
