@@ -605,6 +605,10 @@ __host__ __device__ constexpr int64_t orun_bytes(int64_t nb, int64_t nn) {
 }
 // Row stride of Dp: whole 16-byte groups.
 __host__ __device__ constexpr int64_t drow(int64_t nb) { return align_up(nb, 16); }
+// Bytes of a part's edge blob (nx external edges, nn neighbouring parts).
+__host__ __device__ constexpr int64_t oedge_bytes(int64_t nx, int64_t nn) {
+  return 2 * align_up(4 * nx, 16) + align_up(4 * nn, 16);
+}
 
 struct ObfsArgs {
   DevGraph g;
@@ -615,8 +619,12 @@ struct ObfsArgs {
   const uint32_t *vstart, *pverts;  // part-major vertex ranges; part-major index -> internal id
   const uint32_t *bstart;           // boundary-major ranges of the parts' boundary vertices
   const uint32_t *xoff, *noff;      // external edges and neighbouring parts of each part
-  const uint16_t *xsrc, *xnbr;      // external edge: source (boundary index in its part), neighbour slot
-  const uint32_t *xdst, *nbr;       // external edge target (boundary-major); neighbouring parts
+  // each part's edge blob (16-byte aligned sections): external edges'
+  // sources (boundary index in the part) and neighbour slots (uint16), their
+  // targets (boundary-major, uint32), the neighbouring parts (uint32)
+  const uint4 *oblob;
+  const uint32_t *oblob_off;  // per part, in 16-byte units
+  uint32_t olds;              // dynamic LDS bytes of the kernel
   const uint4 *cblob;
   const uint32_t *cblob_off;  // per part, in 16-byte units
   const unsigned char *dmat;
@@ -634,6 +642,72 @@ struct ObfsArgs {
   unsigned long long *stats;
 };
 constexpr int kCtlFin = 32;  // ctl word: next part of step 3
+
+// lv[b] = min(lv[b], sl[i] + D[sj[i]][b]) over the ns seeds (D: nb rows of
+// nbp bytes, kDInf = no path; in LDS or global memory); chg[b] = 1 where lv
+// drops.  A thread takes one 16-column group of the rows of every R-th seed.
+template <int Block>
+__device__ __forceinline__ void minplus(const unsigned char *D, int nb, int nbp, int ns, const uint16_t *sj,
+                                        const int32_t *sl, int32_t *lv, unsigned char *chg) {
+  const int tid = threadIdx.x;
+  const int G = nbp / 16;
+  if (G <= Block) {
+    const int R = Block / G;
+    const int g = tid % G, r = tid / G;
+    if (r >= R) return;
+    int32_t mn[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) mn[c] = kUnvisited;
+    for (int i0 = r; i0 < ns; i0 += 4 * R) {
+      uint4 w[4];
+      int32_t L[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = i0 + q * R;
+        const int ii = i < ns ? i : i0;
+        w[q] = *reinterpret_cast<const uint4 *>(D + (size_t)sj[ii] * nbp + 16 * g);
+        L[q] = sl[ii];
+        if (i >= ns) w[q] = make_uint4(~0u, ~0u, ~0u, ~0u);  // past the last seed: no path
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t wq[4] = {w[q].x, w[q].y, w[q].z, w[q].w};
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+          const int32_t b = (int32_t)((wq[c / 4] >> (8 * (c % 4))) & 0xffu);
+          const int32_t cand = b == kDInf ? kUnvisited : L[q] + b;
+          mn[c] = cand < mn[c] ? cand : mn[c];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+      const int col = 16 * g + c;
+      if (col < nb && mn[c] < lv[col]) {
+        const int32_t old = atomicMin(&lv[col], mn[c]);
+        if (old > mn[c]) chg[col] = 1;
+      }
+    }
+  } else {
+    for (int64_t it = tid; it < (int64_t)ns * G; it += Block) {
+      const int i = (int)(it / G), g = (int)(it % G);
+      const uint4 w = *reinterpret_cast<const uint4 *>(D + (size_t)sj[i] * nbp + 16 * g);
+      const uint32_t wq[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int c = 0; c < 16; c++) {
+        const int col = 16 * g + c;
+        const int32_t b = (int32_t)((wq[c / 4] >> (8 * (c % 4))) & 0xffu);
+        if (col < nb && b != kDInf) {
+          const int32_t cand = sl[i] + b;
+          if (cand < lv[col]) {
+            const int32_t old = atomicMin(&lv[col], cand);
+            if (old > cand) chg[col] = 1;
+          }
+        }
+      }
+    }
+  }
+}
 
 __global__ void obfs_init_kernel(ObfsArgs a, int64_t nbtot) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -697,12 +771,24 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, bool final_mode, unsigned
   const int nv = (int)(a.vstart[p + 1] - v0);
   const uint32_t b0 = a.bstart[p];
   const int nb = (int)(a.bstart[p + 1] - b0);
+  const bool stats = a.stats != nullptr && final_mode;
+  unsigned long long tp0 = 0, tp1 = 0;
+  if (stats && tid == 0) tp0 = __builtin_amdgcn_s_memrealtime();
   load_cblob<Block>(a.cblob, a.cblob_off, p, smem);
   if (tid == 0) {
     s_cnt[0] = s_cnt[1] = s_cnt[2] = 0;
     *s_min = kUnvisited;
   }
   __syncthreads();
+  // with stats: [20] image load, [21] seeds, [22] rounds, [23] write-out
+  auto phase = [&](int slot) {
+    if (stats && tid == 0) {
+      tp1 = __builtin_amdgcn_s_memrealtime();
+      atomicAdd(&a.stats[slot], tp1 - tp0);
+      tp0 = tp1;
+    }
+  };
+  phase(20);
   const uint16_t *lrow = reinterpret_cast<const uint16_t *>(smem);
   const int ne = lrow[nv];
   const uint16_t *ladj = lrow + nv + 1;
@@ -749,51 +835,64 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, bool final_mode, unsigned
   __syncthreads();
   const int ns = s_cnt[2];
   int32_t Lc = *s_min;
-  __syncthreads();  // every thread has read s_min before the rounds reset it
-  int cur = 0, rounds = 0;
+  __syncthreads();  // every thread has read s_min before it is reused
+  int rounds = 0;
+  // lists F0 / F1 alternate; their counts rotate over s_cnt[0..2] so that
+  // the count the next round appends to is cleared in this one
+  int li = 0, ci = 0;
   if (ns > 0) {
-    for (;;) {
-      uint16_t *Fc = cur ? F1 : F0;
-      uint16_t *Fn = cur ? F0 : F1;
-      // seeds at this level join the frontier (unless a shorter path
-      // reached them, which exact seeds never allow)
+    phase(21);
+    if (tid == 0) s_cnt[0] = s_cnt[1] = 0;
+    __syncthreads();
+    // the seeds at level Lc into list li (count ci); a seed that a shorter
+    // path reached (impossible for exact seeds) is not listed again
+    auto join = [&](int32_t level, uint16_t *F, int *cnt) {
       for (int ii = 0; ii < ns; ii += Block) {
         const int i = ii + tid;
         bool add = false;
         int u = 0;
-        if (i < ns && sl[i] == Lc) {
+        if (i < ns && sl[i] == level) {
           u = sv[i];
-          add = lvl[u] == Lc;
+          add = lvl[u] == level;
         }
         const unsigned long long m = __ballot(add);
         if (m) {
           const int leader = __ffsll((long long)m) - 1;
           int base = 0;
-          if (lane == leader) base = atomicAdd(&s_cnt[cur], __popcll(m));
+          if (lane == leader) base = atomicAdd(cnt, __popcll(m));
           base = __builtin_amdgcn_readlane(base, leader);
-          if (add) Fc[base + __popcll(m & lanemask_lt())] = (uint16_t)u;
+          if (add) F[base + __popcll(m & lanemask_lt())] = (uint16_t)u;
         }
       }
-      if (tid == 0) {
-        s_cnt[cur ^ 1] = 0;
-        *s_min = kUnvisited;
-      }
-      __syncthreads();
-      const int nf = s_cnt[cur];
+    };
+    join(Lc, F0, &s_cnt[0]);
+    __syncthreads();
+    for (;;) {
+      uint16_t *Fc = li ? F1 : F0;
+      uint16_t *Fn = li ? F0 : F1;
+      const int nf = s_cnt[ci];
+      const int cn_i = (ci + 1) % 3;
       if (nf == 0) {
         // nothing at this level: jump to the next seed level, if any
+        if (tid == 0) *s_min = kUnvisited;
+        __syncthreads();
         for (int i = tid; i < ns; i += Block)
           if (sl[i] > Lc) atomicMin(s_min, sl[i]);
         __syncthreads();
         const int32_t nL = *s_min;
-        __syncthreads();
         if (nL == kUnvisited) break;
         Lc = nL;
+        join(Lc, Fc, &s_cnt[ci]);
+        __syncthreads();
         continue;
       }
       rounds++;
       const int32_t nl = Lc + 1;
-      int *cn = &s_cnt[cur ^ 1];
+      int *cn = &s_cnt[cn_i];
+      if (tid == 0) s_cnt[(ci + 2) % 3] = 0;
+      // the seeds of the next level join the next list first; the claims
+      // below never list an exact seed (it already holds nl)
+      join(nl, Fn, cn);
       for (int f0 = 0; f0 < nf; f0 += Block / Lanes) {
         const int f = f0 + tid / Lanes;
         int xb = 0, e = 0;
@@ -836,10 +935,12 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, bool final_mode, unsigned
         }
       }
       __syncthreads();
-      cur ^= 1;
+      li ^= 1;
+      ci = cn_i;
       Lc = nl;
     }
   }
+  phase(22);
   if (!final_mode) {
     for (int j = tid; j < nb; j += Block) {
       const int32_t L = lvl[blist[j]];
@@ -870,6 +971,10 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, bool final_mode, unsigned
                                     (0xffffffffull - (unsigned long long)original_id(a.g, i));
       acc.key = kk > acc.key ? kk : acc.key;
     }
+  }
+  if (stats) {
+    __syncthreads();
+    phase(23);
   }
   return rounds;
 }
@@ -919,23 +1024,62 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
     const uint32_t b0 = a.bstart[p];
     const int nb = (int)(a.bstart[p + 1] - b0);
     const int nbp = (int)drow(nb);
-    const uint32_t x0 = a.xoff[p];
-    const int nx = (int)(a.xoff[p + 1] - x0);
-    const uint32_t n0 = a.noff[p];
-    const int nn = (int)(a.noff[p + 1] - n0);
+    const int nx = (int)(a.xoff[p + 1] - a.xoff[p]);
+    const int nn = (int)(a.noff[p + 1] - a.noff[p]);
+    // LDS: the run's state, then the part's image (its edge blob); Dp is
+    // read from global memory, only the seeds' rows (measured: copying the
+    // whole Dp into LDS when a part is taken, to serve its re-runs, made
+    // the first run -- the one on the wave's path -- slower, 1.15 -> 1.19 ms
+    // per traversal of RGG n=4.2M)
+    const int ob = (int)align_up(orun_bytes(nb, nn), 16);
+    const int eb = (int)oedge_bytes(nx, nn);
     int32_t *lv = reinterpret_cast<int32_t *>(smem);
     int32_t *sl = lv + nb;
     uint16_t *sj = reinterpret_cast<uint16_t *>(sl + nb);
     unsigned char *chg = reinterpret_cast<unsigned char *>(smem + 8 * nb + align_up(2 * nb, 4));
     unsigned char *nflag = chg + align_up(nb, 4);
-    const unsigned char *D = a.dmat + a.doff[p];
+    const uint16_t *xs = reinterpret_cast<const uint16_t *>(smem + ob);
+    const uint16_t *xn = xs + nx;
+    const uint32_t *xd = reinterpret_cast<const uint32_t *>(smem + ob + align_up(4 * nx, 16));
+    const uint32_t *nbl = reinterpret_cast<const uint32_t *>(smem + ob + 2 * align_up(4 * nx, 16));
     for (;;) {
       if (stats && tid == 0) {
         t0 = __builtin_amdgcn_s_memrealtime();
         n_act++;
       }
-      if (cached != p || cached_ver != s_ver) {
-        for (int j = tid; j < nb; j += Block) lv[j] = ld_dev(&a.blevel[b0 + j]);
+      const bool fresh = cached != p || cached_ver != s_ver;
+      // every load of the run in flight together: the levels lowered from
+      // outside, and for a fresh image the part's levels and edge blob
+      constexpr int kR = 4;
+      const bool few = nb <= kR * Block;
+      int32_t il[kR], ol[kR];
+      if (few) {
+#pragma unroll
+        for (int q = 0; q < kR; q++) {
+          const int j = tid + q * Block;
+          il[q] = j < nb ? ld_dev(&a.bilevel[b0 + j]) : kUnvisited;
+          ol[q] = fresh && j < nb ? ld_dev(&a.blevel[b0 + j]) : 0;
+        }
+      }
+      if (fresh) {
+        const int we = eb / 16;
+        const uint4 *ge = a.oblob + a.oblob_off[p];
+        uint4 *se = reinterpret_cast<uint4 *>(smem + ob);
+        for (int i = tid; i < we; i += 8 * Block) {
+          uint4 t[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) t[j] = ge[i + j * Block < we ? i + j * Block : i];
+#pragma unroll
+          for (int j = 0; j < 8; j++)
+            if (i + j * Block < we) se[i + j * Block] = t[j];
+        }
+        if (few) {
+#pragma unroll
+          for (int q = 0; q < kR; q++)
+            if (tid + q * Block < nb) lv[tid + q * Block] = ol[q];
+        } else {
+          for (int j = tid; j < nb; j += Block) lv[j] = ld_dev(&a.blevel[b0 + j]);
+        }
         if (stats && tid == 0) n_img++;
       }
       for (int j = tid; j < nb; j += Block) chg[j] = 0;
@@ -943,17 +1087,12 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
       if (tid == 0) s_ns = 0;
       __syncthreads();
       // seeds: boundary vertices lowered from outside
-      for (int jj = 0; jj < nb; jj += Block) {
-        const int j = jj + tid;
+      auto seed = [&](int j, int32_t v, bool in) {
         bool add = false;
-        int32_t il = 0;
-        if (j < nb) {
-          il = ld_dev(&a.bilevel[b0 + j]);
-          if (il < lv[j]) {
-            lv[j] = il;
-            chg[j] = 1;
-            add = true;
-          }
+        if (in && v < lv[j]) {
+          lv[j] = v;
+          chg[j] = 1;
+          add = true;
         }
         const unsigned long long m = __ballot(add);
         if (m) {
@@ -964,8 +1103,20 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
           if (add) {
             const int at = base + __popcll(m & lanemask_lt());
             sj[at] = (uint16_t)j;
-            sl[at] = il;
+            sl[at] = v;
           }
+        }
+      };
+      if (few) {
+#pragma unroll
+        for (int q = 0; q < kR; q++) {
+          const int j = tid + q * Block;
+          seed(j < nb ? j : 0, il[q], j < nb);
+        }
+      } else {
+        for (int jj = 0; jj < nb; jj += Block) {
+          const int j = jj + tid;
+          seed(j < nb ? j : 0, j < nb ? ld_dev(&a.bilevel[b0 + j]) : kUnvisited, j < nb);
         }
       }
       __syncthreads();
@@ -976,66 +1127,8 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
         t0 = t1;
         n_seed += ns;
       }
-      // min-plus: lv[b] = min(lv[b], sl[i] + Dp[sj[i]][b]) over the seeds;
-      // a thread takes one 16-column group of the rows of every R-th seed
-      const int G = nbp / 16;
-      if (ns > 0 && G <= Block) {
-        const int R = Block / G;
-        const int g = tid % G, r = tid / G;
-        if (r < R) {
-          int32_t mn[16];
-#pragma unroll
-          for (int c = 0; c < 16; c++) mn[c] = kUnvisited;
-          for (int i0 = r; i0 < ns; i0 += 4 * R) {
-            uint4 w[4];
-            int32_t L[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              const int i = i0 + q * R;
-              const int ii = i < ns ? i : i0;
-              w[q] = *reinterpret_cast<const uint4 *>(D + (size_t)sj[ii] * nbp + 16 * g);
-              L[q] = sl[ii];
-              if (i >= ns) w[q] = make_uint4(~0u, ~0u, ~0u, ~0u);  // past the last seed: no path
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              const uint32_t wq[4] = {w[q].x, w[q].y, w[q].z, w[q].w};
-#pragma unroll
-              for (int c = 0; c < 16; c++) {
-                const int32_t b = (int32_t)((wq[c / 4] >> (8 * (c % 4))) & 0xffu);
-                const int32_t cand = b == kDInf ? kUnvisited : L[q] + b;
-                mn[c] = cand < mn[c] ? cand : mn[c];
-              }
-            }
-          }
-#pragma unroll
-          for (int c = 0; c < 16; c++) {
-            const int col = 16 * g + c;
-            if (col < nb && mn[c] < lv[col]) {
-              const int32_t old = atomicMin(&lv[col], mn[c]);
-              if (old > mn[c]) chg[col] = 1;
-            }
-          }
-        }
-      } else if (ns > 0) {
-        for (int64_t it = tid; it < (int64_t)ns * G; it += Block) {
-          const int i = (int)(it / G), g = (int)(it % G);
-          const uint4 w = *reinterpret_cast<const uint4 *>(D + (size_t)sj[i] * nbp + 16 * g);
-          const uint32_t wq[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-          for (int c = 0; c < 16; c++) {
-            const int col = 16 * g + c;
-            const int32_t b = (int32_t)((wq[c / 4] >> (8 * (c % 4))) & 0xffu);
-            if (col < nb && b != kDInf) {
-              const int32_t cand = sl[i] + b;
-              if (cand < lv[col]) {
-                const int32_t old = atomicMin(&lv[col], cand);
-                if (old > cand) chg[col] = 1;
-              }
-            }
-          }
-        }
-      }
+      // min-plus: lv[b] = min(lv[b], sl[i] + Dp[sj[i]][b]) over the seeds
+      if (ns > 0) minplus<Block>(a.dmat + a.doff[p], nb, nbp, ns, sj, sl, lv, chg);
       __syncthreads();
       if (stats && tid == 0) {
         t1 = __builtin_amdgcn_s_memrealtime();
@@ -1058,21 +1151,23 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
           on[q] = false;
           nl[q] = 0;
           if (x < nx) {
-            const int j = a.xsrc[x0 + x];
+            const int j = xs[x];
             on[q] = chg[j] != 0;
             nl[q] = lv[j] + 1;
           }
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-          old[q] = on[q] ? atomicMin(&a.bilevel[a.xdst[x0 + xx + q * Block + tid]], nl[q]) : 0;
+        for (int q = 0; q < 4; q++) old[q] = on[q] ? atomicMin(&a.bilevel[xd[xx + q * Block + tid]], nl[q]) : 0;
 #pragma unroll
         for (int q = 0; q < 4; q++)
-          if (on[q] && old[q] > nl[q]) nflag[a.xnbr[x0 + xx + q * Block + tid]] = 1;
+          if (on[q] && old[q] > nl[q]) nflag[xn[xx + q * Block + tid]] = 1;
       }
       __syncthreads();
+      // queue the flagged parts (measured: a run that kept one newly
+      // queued neighbour for its own workgroup instead of the ring ran the
+      // wave's front out of order and re-ran parts more, 1.15 -> 1.42 ms)
       for (int i = tid; i < nn; i += Block)
-        if (nflag[i]) mark_lowered(a, a.nbr[n0 + i]);
+        if (nflag[i]) mark_lowered(a, nbl[i]);
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (stats && tid == 0) {
@@ -1258,6 +1353,21 @@ static const PbfsShape kShapes[] = {
     {256, 16, 1, pbfs_kernel<256, 16, 1>},
 };
 
+// Shapes of the boundary-distance kernels: threads, lanes per frontier
+// vertex and claims per lane of the interior passes.
+struct ObfsShape {
+  int block;
+  void (*fn)(ObfsArgs);
+  void (*src_fn)(ObfsArgs);
+};
+static const ObfsShape kObfsShapes[] = {
+    {1024, obfs_kernel<1024, 16, 1>, obfs_source_kernel<1024, 16, 1>},
+    {1024, obfs_kernel<1024, 8, 2>, obfs_source_kernel<1024, 8, 2>},
+    {512, obfs_kernel<512, 8, 2>, obfs_source_kernel<512, 8, 2>},
+    {512, obfs_kernel<512, 4, 3>, obfs_source_kernel<512, 4, 3>},
+    {256, obfs_kernel<256, 4, 3>, obfs_source_kernel<256, 4, 3>},
+};
+
 // The parts of a deep graph (device arrays) and the traversal's state.
 struct PartGraph {
   int64_t k = 0;
@@ -1274,25 +1384,25 @@ struct PartGraph {
   // the boundary-distance traversal (ovl false: not built, or PHDE_OBFS=0)
   bool ovl = false;
   int64_t nbtot = 0;
-  uint32_t *pverts = nullptr, *bstart = nullptr, *xoff = nullptr, *noff = nullptr, *xdst = nullptr, *nbr = nullptr,
-           *cblob_off = nullptr;
-  uint16_t *xsrc = nullptr, *xnbr = nullptr;
-  uint4 *cblob = nullptr;
+  uint32_t *pverts = nullptr, *bstart = nullptr, *xoff = nullptr, *noff = nullptr, *cblob_off = nullptr,
+           *oblob_off = nullptr;
+  uint4 *cblob = nullptr, *oblob = nullptr;
   unsigned char *dmat = nullptr;
   unsigned long long *doff = nullptr;
   int32_t *blevel = nullptr, *bilevel = nullptr;
+  ObfsShape oshape{};
+  size_t olds = 0;  // dynamic LDS of the boundary-distance kernels
 };
 
 // Frees the boundary-distance traversal's arrays (the partitioned traversal
 // stays usable).
 static void obfs_release(PartGraph *pg) {
-  void *ptr[] = {pg->pverts, pg->bstart, pg->xoff,  pg->noff, pg->xdst,   pg->nbr,    pg->cblob_off,
-                 pg->xsrc,   pg->xnbr,   pg->cblob, pg->dmat, pg->doff,   pg->blevel, pg->bilevel};
+  void *ptr[] = {pg->pverts,   pg->bstart, pg->xoff, pg->noff, pg->cblob_off, pg->oblob_off,
+                 pg->cblob,    pg->oblob,  pg->dmat, pg->doff, pg->blevel,    pg->bilevel};
   for (void *x : ptr)
     if (x) (void)hipFree(x);
-  pg->pverts = pg->bstart = pg->xoff = pg->noff = pg->xdst = pg->nbr = pg->cblob_off = nullptr;
-  pg->xsrc = pg->xnbr = nullptr;
-  pg->cblob = nullptr;
+  pg->pverts = pg->bstart = pg->xoff = pg->noff = pg->cblob_off = pg->oblob_off = nullptr;
+  pg->cblob = pg->oblob = nullptr;
   pg->dmat = nullptr;
   pg->doff = nullptr;
   pg->blevel = pg->bilevel = nullptr;
@@ -1321,12 +1431,12 @@ static int upload(T **dst, const std::vector<T> &src) {
 // Host-side arrays of the boundary-distance traversal, built part by part
 // with the partition.
 struct ObfsHost {
-  std::vector<uint32_t> bmidx, bstart, xoff{0}, xdst, noff{0}, nbr, coff{0};
-  std::vector<uint16_t> xsrc, xnbr;
-  std::vector<uint4> cblob;
+  std::vector<uint32_t> bmidx, bstart, xoff{0}, noff{0}, coff{0}, eoff{0};
+  std::vector<uint4> cblob, oblob;
   std::vector<unsigned long long> doff{0};
-  // largest LDS of step 3 (and 1), of a run of step 2, and of dmat_kernel
-  // with 32-, 16- and 8-bit masks
+  int64_t nx_tot = 0, nn_tot = 0;
+  // largest LDS of step 3 (and 1), of a run of step 2 (with its edge
+  // blob), and of dmat_kernel with 32-, 16- and 8-bit masks
   int64_t max_final = 0, max_orun = 0, max_dm[3] = {0, 0, 0}, max_nb = 0;
   bool ok = true;
   void add_part(int64_t p, uint32_t v0, uint32_t nv, const std::vector<uint32_t> &verts,
@@ -1336,17 +1446,31 @@ struct ObfsHost {
                 const std::vector<uint32_t> &pnbr) {
     if (!ok) return;
     const uint32_t b0 = bstart[p];
-    const int64_t ne = ladj.size(), nb = blist.size(), nn = pnbr.size();
-    // external edges come only from boundary vertices; pxdst holds the
-    // target's part-major index
-    for (size_t x = 0; x < pxsrc.size(); x++) {
-      xsrc.push_back((uint16_t)(bmidx[verts[v0 + pxsrc[x]]] - b0));
-      xdst.push_back(bmidx[verts[pxdst[x]]]);
-      xnbr.push_back(pxnbr[x]);
+    const int64_t ne = ladj.size(), nb = blist.size(), nn = pnbr.size(), nx = pxsrc.size();
+    // the edge blob: external edges come only from boundary vertices (their
+    // boundary index), pxdst holds the target's part-major index
+    {
+      const int64_t eb = oedge_bytes(nx, nn);
+      std::vector<unsigned char> raw(eb, 0);
+      uint16_t *xs = reinterpret_cast<uint16_t *>(raw.data());
+      uint32_t *xd = reinterpret_cast<uint32_t *>(raw.data() + align_up(4 * nx, 16));
+      uint32_t *nl = reinterpret_cast<uint32_t *>(raw.data() + 2 * align_up(4 * nx, 16));
+      for (int64_t x = 0; x < nx; x++) {
+        xs[x] = (uint16_t)(bmidx[verts[v0 + pxsrc[x]]] - b0);
+        xs[nx + x] = pxnbr[x];
+        xd[x] = bmidx[verts[pxdst[x]]];
+      }
+      for (int64_t i = 0; i < nn; i++) nl[i] = pnbr[i];
+      const size_t at = oblob.size();
+      oblob.resize(at + eb / 16);
+      if (eb) std::memcpy(&oblob[at], raw.data(), eb);
+      eoff.push_back((uint32_t)oblob.size());
+      nx_tot += nx;
+      nn_tot += nn;
+      xoff.push_back((uint32_t)nx_tot);
+      noff.push_back((uint32_t)nn_tot);
+      max_orun = std::max(max_orun, align_up(orun_bytes(nb, nn), 16) + eb);
     }
-    xoff.push_back((uint32_t)xsrc.size());
-    nbr.insert(nbr.end(), pnbr.begin(), pnbr.end());
-    noff.push_back((uint32_t)nbr.size());
     const int64_t cb = cblob_bytes(nv, ne, nb);
     std::vector<uint16_t> b16(cb / 2, 0);
     std::copy(lrow.begin(), lrow.end(), b16.begin());
@@ -1358,11 +1482,11 @@ struct ObfsHost {
     coff.push_back((uint32_t)cblob.size());
     doff.push_back(doff.back() + (unsigned long long)(nb * drow(nb)));
     max_final = std::max(max_final, final_bytes(nv, ne, nb));
-    max_orun = std::max(max_orun, orun_bytes(nb, nn));
     max_nb = std::max(max_nb, nb);
     const int64_t w[3] = {4, 2, 1};
     for (int i = 0; i < 3; i++) max_dm[i] = std::max(max_dm[i], cb + align_up(2 * nv, 16) + 3 * w[i] * nv);
-    if (xsrc.size() >= UINT32_MAX || nbr.size() >= UINT32_MAX || cblob.size() >= UINT32_MAX) ok = false;
+    if (nx_tot >= UINT32_MAX || nn_tot >= UINT32_MAX || cblob.size() >= UINT32_MAX || oblob.size() >= UINT32_MAX)
+      ok = false;
   }
 };
 
@@ -1376,6 +1500,7 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   // a level plus an in-part distance (< 255) must stay below kUnvisited
   if (c->g.n >= (int64_t)kUnvisited - 1024) return PHDE_OK;
   if (!oh.ok || std::max(oh.max_final, oh.max_orun) > kPartBytes) return PHDE_OK;
+  if (oh.oblob.empty()) oh.oblob.push_back(make_uint4(0, 0, 0, 0));
   int mask = -1;
   for (int i = 0; i < 3 && mask < 0; i++)
     if (oh.max_dm[i] <= kPartBytes) mask = i;
@@ -1384,12 +1509,10 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   PHDE_TRY_RC(upload(&pg->bstart, oh.bstart));
   PHDE_TRY_RC(upload(&pg->xoff, oh.xoff));
   PHDE_TRY_RC(upload(&pg->noff, oh.noff));
-  PHDE_TRY_RC(upload(&pg->xdst, oh.xdst));
-  PHDE_TRY_RC(upload(&pg->nbr, oh.nbr));
   PHDE_TRY_RC(upload(&pg->cblob_off, oh.coff));
-  PHDE_TRY_RC(upload(&pg->xsrc, oh.xsrc));
-  PHDE_TRY_RC(upload(&pg->xnbr, oh.xnbr));
+  PHDE_TRY_RC(upload(&pg->oblob_off, oh.eoff));
   PHDE_TRY_RC(upload(&pg->cblob, oh.cblob));
+  PHDE_TRY_RC(upload(&pg->oblob, oh.oblob));
   PHDE_TRY_RC(upload(&pg->doff, oh.doff));
   pg->nbtot = oh.bstart.back();
   const size_t dbytes = std::max<size_t>(oh.doff.back(), 16);
@@ -1421,8 +1544,13 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   PHDE_HIP(hipMemcpyAsync(&h_err, err.p, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  for (const void *f : {(const void *)obfs_kernel<1024, 16, 1>, (const void *)obfs_source_kernel<1024, 16, 1>})
-    PHDE_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kPartBytes));
+  pg->oshape = kObfsShapes[1];
+  if (const char *e = getenv("PHDE_OBFS_SHAPE"))
+    pg->oshape = kObfsShapes[std::max(0, std::min((int)(sizeof(kObfsShapes) / sizeof(kObfsShapes[0])) - 1, atoi(e)))];
+  // LDS: step 3's largest part, and a run's state with its edge blob
+  pg->olds = (size_t)align_up(std::max(oh.max_final, oh.max_orun), 256);
+  for (const void *f : {(const void *)pg->oshape.fn, (const void *)pg->oshape.src_fn})
+    PHDE_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pg->olds));
   if (pg->stats)
     std::fprintf(stderr, "[obfs] boundary %lld (largest part %lld), Dp %.1f MB in %.1f ms (%d-bit masks)%s\n",
                  (long long)pg->nbtot, (long long)oh.max_nb, oh.doff.back() / 1e6, ms, 32 >> mask,
@@ -1637,10 +1765,9 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, int64_t src, int32_t *dist_out, 
   a.bstart = pg->bstart;
   a.xoff = pg->xoff;
   a.noff = pg->noff;
-  a.xsrc = pg->xsrc;
-  a.xnbr = pg->xnbr;
-  a.xdst = pg->xdst;
-  a.nbr = pg->nbr;
+  a.oblob = pg->oblob;
+  a.oblob_off = pg->oblob_off;
+  a.olds = (uint32_t)pg->olds;
   a.cblob = pg->cblob;
   a.cblob_off = pg->cblob_off;
   a.dmat = pg->dmat;
@@ -1658,10 +1785,12 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, int64_t src, int32_t *dist_out, 
     const unsigned long long init[32] = {0, 0, 0, 0, 0, 0, 0, ~0ull, 0};
     PHDE_HIP(hipMemcpyAsync(pg->stats, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
   }
-  const auto fn = obfs_kernel<1024, 16, 1>;
+  const auto fn = pg->oshape.fn;
+  const int block = pg->oshape.block;
   int resident = 0;
-  PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, 1024, kPartBytes));
-  const int grid = std::max(1, c->num_cus) * std::max(1, resident);
+  PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, block, pg->olds));
+  // at most 4 workgroups per CU (the queue's ring holds k + 4 per CU)
+  const int grid = std::max(1, c->num_cus) * std::max(1, std::min(4, resident));
   if (resident < 1 || (int64_t)pg->ring_n < pg->k + grid) {
     obfs_release(pg);
     return PHDE_OK;
@@ -1670,10 +1799,10 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, int64_t src, int32_t *dist_out, 
       1, std::min<int64_t>((std::max<int64_t>(pg->nbtot, pg->ring_n) + 255) / 256, (int64_t)c->num_cus * 8));
   obfs_init_kernel<<<ib, 256, 0, c->stream>>>(a, pg->nbtot);
   PHDE_HIP(hipGetLastError());
-  obfs_source_kernel<1024, 16, 1><<<1, 1024, kPartBytes, c->stream>>>(a);
+  pg->oshape.src_fn<<<1, block, pg->olds, c->stream>>>(a);
   PHDE_HIP(hipGetLastError());
   void *args[] = {&a};
-  const hipError_t e = launch_resident((const void *)fn, grid, 1024, args, kPartBytes, c->stream);
+  const hipError_t e = launch_resident((const void *)fn, grid, block, args, pg->olds, c->stream);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     obfs_release(pg);
@@ -1704,6 +1833,9 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, int64_t src, int32_t *dist_out, 
                  s[3] * us / na, s[9] * us, s[10], s[10] / (double)pg->k);
     std::fprintf(stderr, "[obfs]   queue latency (queued to taken): %llu parts, mean %.2f us, %llu above 10 us, max %.1f us\n",
                  s[17], s[16] * 0.01 / (double)std::max(1ull, s[17]), s[18], s[19] * 0.01);
+    std::fprintf(stderr, "[obfs]   step 3 per part: image %.2f seeds %.2f rounds %.2f write-out %.2f us; %.3f us per round\n",
+                 s[20] * us / pg->k, s[21] * us / pg->k, s[22] * us / pg->k, s[23] * us / pg->k,
+                 s[22] * us / (double)std::max(1ull, s[10]));
   }
   *ran = true;
   return PHDE_OK;
