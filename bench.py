@@ -32,14 +32,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bfs_kernel_name(deep: bool, parts: int) -> str:
-    """The traversal kernel the library launches: one cooperative launch per
-    traversal -- the partitioned kernel for a graph it found deep and cut
-    into parts, else the multi-hop kernel for a deep graph unless
-    PHDE_BFS_HOPS=0, else the level-by-level one -- or one launch per level
-    with PHDE_BFS_PERSISTENT=0."""
+def bfs_kernel_name(deep: bool, parts: int, boundary: int) -> str:
+    """The traversal kernel the library launches: for a graph it found deep
+    and cut into parts, the boundary-distance kernel (obfs_kernel, after a
+    small init kernel and the source part's own search: the traversal's HIP
+    events cover all three) or, without its per-part distance tables, the
+    partitioned kernel; else the multi-hop kernel for a deep graph unless
+    PHDE_BFS_HOPS=0, else the level-by-level one -- one launch per traversal
+    -- or one launch per level with PHDE_BFS_PERSISTENT=0."""
     if os.environ.get("PHDE_BFS_PERSISTENT", "1") == "0":
         return "bfs_level_kernel"
+    if deep and parts > 0 and boundary >= 0:
+        return "obfs_kernel"
     if deep and parts > 0:
         return "pbfs_kernel"
     if deep and os.environ.get("PHDE_BFS_HOPS", "") not in ("0", "1"):
@@ -154,7 +158,7 @@ def main():
     t0 = time.time()
     ctx = L.Context(g)
     h2d_s = time.time() - t0
-    kernel = bfs_kernel_name(ctx.vertex_order()[1], ctx.num_parts())
+    kernel = bfs_kernel_name(ctx.vertex_order()[1], ctx.num_parts(), ctx.num_boundary())
     log(f"[bench] graph resident in HBM ({h2d_s * 1e3:.1f} ms incl. validation and H2D)")
     prec = L.PREC_FP32 if args.precision == "fp32" else L.PREC_FP64
     opts = dict(s=args.s, num_coord=2, precision=prec, eager=1 if args.eager else 0)
