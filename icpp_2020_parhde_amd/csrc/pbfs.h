@@ -19,5 +19,10 @@ int64_t pbfs_num_boundary(const phde_ctx *c);
 // not use the partitioned BFS, or a wait in it timed out (it then stays
 // off for this context and the caller takes another traversal).
 int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, bool *ran);
+// Levels from k <= 64 sources (internal ids) searched together by the
+// boundary-distance method: D gets n levels per source (internal ids,
+// kUnvisited when unreached).  *ran is false when the context does not use
+// that method (the caller takes another traversal).
+int pbfs_multi_run(phde_ctx *c, const int64_t *src, int k, int32_t *D, bool *ran);
 
 }  // namespace phde

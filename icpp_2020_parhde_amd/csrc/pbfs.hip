@@ -610,9 +610,11 @@ __host__ __device__ constexpr int64_t oedge_bytes(int64_t nx, int64_t nn) {
   return 2 * align_up(4 * nx, 16) + align_up(4 * nn, 16);
 }
 
+constexpr int kMaxSources = 64;  // sources of one multi-source search
+
 struct ObfsArgs {
   DevGraph g;
-  int32_t *level;     // output: levels (internal ids)
+  int32_t *level;     // output: levels (internal ids), n per source
   int32_t *min_dist;  // nullable: fused k-centers update
   KcResult *kc;
   int64_t k;
@@ -633,7 +635,14 @@ struct ObfsArgs {
   int32_t *bilevel;  // boundary-major: levels lowered from outside (atomicMin)
   uint32_t *st, *ctl;
   unsigned long long *ring;
-  uint32_t ring_n, src_part, src_loc;
+  uint32_t ring_n;
+  // sources searched together (1 for a k-centers traversal): work items
+  // are (source, part) pairs, item = source * k + part; per-source state
+  // (blevel, bilevel: nbtot entries each; st: k entries; level column: n)
+  // at the source's stride
+  int nsrc;
+  int64_t nbtot;
+  uint32_t src_part[kMaxSources], src_loc[kMaxSources];
   // with PHDE_PBFS_STATS: [0..3] wait, seed, min-plus, push (ticks summed
   // over workgroups); [4] runs, [5] seeds, [6] level loads, [7, 8] first
   // and last tick, [9] interior time, [10] interior rounds, [11] boundary
@@ -709,17 +718,18 @@ __device__ __forceinline__ void minplus(const unsigned char *D, int nb, int nbp,
   }
 }
 
-__global__ void obfs_init_kernel(ObfsArgs a, int64_t nbtot) {
+__global__ void obfs_init_kernel(ObfsArgs a) {
   const int64_t gtid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t gsize = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t j = gtid; j < nbtot; j += gsize) {
+  for (int64_t j = gtid; j < a.nsrc * a.nbtot; j += gsize) {
     a.blevel[j] = kUnvisited;
     a.bilevel[j] = kUnvisited;
   }
-  for (int64_t p = gtid; p < a.k; p += gsize) a.st[p] = p == a.src_part ? 1u : 0u;
+  // every source's own part queued: items s * k + src_part[s] in slots s
+  for (int64_t t = gtid; t < a.nsrc * a.k; t += gsize) a.st[t] = t % a.k == a.src_part[t / a.k] ? 1u : 0u;
   for (int64_t r = gtid; r < a.ring_n; r += gsize)
-    a.ring[r * kSlotStride] = r == 0 ? (1ull << 32) | (a.src_part + 1u) : 0ull;
-  if (gtid < kCtlWords) a.ctl[gtid] = gtid == kCtlTP || gtid == kCtlTP + 1 ? 1u : 0u;  // tail 1, pending 1
+    a.ring[r * kSlotStride] = r < a.nsrc ? ((unsigned long long)(r + 1) << 32) | (r * a.k + a.src_part[r] + 1u) : 0ull;
+  if (gtid < kCtlWords) a.ctl[gtid] = gtid == kCtlTP || gtid == kCtlTP + 1 ? (uint32_t)a.nsrc : 0u;  // tail, pending
   if (gtid == 0) {
     a.kc->key = 0;
     a.kc->sumsq[0] = a.kc->sumsq[1] = 0;
@@ -763,7 +773,7 @@ struct KcAcc {
 // level with an empty frontier jumps to the next seed level, so every vertex
 // is listed once.  Returns the rounds taken.
 template <int Block, int Lanes, int Claims>
-__device__ int part_bfs(const ObfsArgs &a, uint32_t p, bool final_mode, unsigned char *smem, int *s_cnt,
+__device__ int part_bfs(const ObfsArgs &a, uint32_t p, int src, bool final_mode, unsigned char *smem, int *s_cnt,
                         int *s_min, int32_t *s_dummy, KcAcc &acc) {
   const int tid = threadIdx.x;
   const int lane = tid % kWave;
@@ -808,10 +818,10 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, bool final_mode, unsigned
     if (j < nb) {
       if (final_mode) {
         u = blist[j];
-        L = ld_dev(&a.blevel[b0 + j]);
+        L = ld_dev(&a.blevel[src * a.nbtot + b0 + j]);
       }
-    } else if (j == nb && p == a.src_part) {
-      u = (int)a.src_loc;
+    } else if (j == nb && p == a.src_part[src]) {
+      u = (int)a.src_loc[src];
       L = 0;
     }
     const bool add = L != kUnvisited;
@@ -944,7 +954,7 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, bool final_mode, unsigned
   if (!final_mode) {
     for (int j = tid; j < nb; j += Block) {
       const int32_t L = lvl[blist[j]];
-      if (L != kUnvisited) st_dev(&a.bilevel[b0 + j], L);
+      if (L != kUnvisited) st_dev(&a.bilevel[src * a.nbtot + b0 + j], L);
     }
     return rounds;
   }
@@ -954,7 +964,7 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, bool final_mode, unsigned
   for (int v = tid; v < nv; v += Block) {
     const int32_t d = lvl[v];
     const uint32_t i = a.pverts[v0 + v];
-    a.level[i] = d;
+    a.level[src * a.g.n + i] = d;
     if (d == kUnvisited) {
       acc.unreached++;
     } else {
@@ -986,7 +996,7 @@ __global__ __launch_bounds__(Block) void obfs_source_kernel(ObfsArgs a) {
   __shared__ int s_cnt[3], s_min;
   __shared__ int32_t s_dummy[Block];
   KcAcc acc;
-  part_bfs<Block, Lanes, Claims>(a, a.src_part, false, smem, s_cnt, &s_min, s_dummy, acc);
+  part_bfs<Block, Lanes, Claims>(a, a.src_part[blockIdx.x], blockIdx.x, false, smem, s_cnt, &s_min, s_dummy, acc);
 }
 
 // Steps 2 and 3: every workgroup resident; the queue of parts, then the
@@ -1000,27 +1010,33 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
   const int tid = threadIdx.x;
   const int lane = tid % kWave;
   const bool stats = a.stats != nullptr;
-  uint32_t cached = kNoPart, cached_ver = 0;
+  // the item whose levels are in LDS (and its version), and the part whose
+  // edge blob is
+  uint32_t cached = kNoPart, cached_ver = 0, cached_part = kNoPart;
   unsigned long long t_wait = 0, t_seed = 0, t_mp = 0, t_push = 0, n_act = 0, n_seed = 0, n_img = 0;
   unsigned long long t_first = ~0ull, t_last = 0, t0 = 0, t1 = 0;
   for (;;) {
     if (stats && tid == 0) t0 = __builtin_amdgcn_s_memrealtime();
     if (tid == 0) {
-      const uint32_t p = pop_part(a);
-      if (p != kNoPart) {
-        s_ver = atomicAdd(&a.st[p], 1u) >> 2;  // queued -> running; the part's version
+      const uint32_t it = pop_part(a);
+      if (it != kNoPart) {
+        s_ver = atomicAdd(&a.st[it], 1u) >> 2;  // queued -> running; the item's version
         __builtin_amdgcn_s_waitcnt(0);
       }
-      s_part = p;
+      s_part = it;
     }
     __syncthreads();
-    const uint32_t p = s_part;
+    const uint32_t item = s_part;
     if (stats && tid == 0) {
       t1 = __builtin_amdgcn_s_memrealtime();
       t_wait += t1 - t0;
-      if (p != kNoPart && t1 < t_first) t_first = t1;
+      if (item != kNoPart && t1 < t_first) t_first = t1;
     }
-    if (p == kNoPart) break;
+    if (item == kNoPart) break;
+    const uint32_t src = item / (uint32_t)a.k, p = item % (uint32_t)a.k;
+    int32_t *blev = a.blevel + src * a.nbtot;
+    int32_t *bilev = a.bilevel + src * a.nbtot;
+    const uint32_t ibase = src * (uint32_t)a.k;  // this source's first item
     const uint32_t b0 = a.bstart[p];
     const int nb = (int)(a.bstart[p + 1] - b0);
     const int nbp = (int)drow(nb);
@@ -1047,7 +1063,8 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
         t0 = __builtin_amdgcn_s_memrealtime();
         n_act++;
       }
-      const bool fresh = cached != p || cached_ver != s_ver;
+      const bool fresh = cached != item || cached_ver != s_ver;
+      const bool fresh_blob = cached_part != p;
       // every load of the run in flight together: the levels lowered from
       // outside, and for a fresh image the part's levels and edge blob
       constexpr int kR = 4;
@@ -1057,11 +1074,11 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
 #pragma unroll
         for (int q = 0; q < kR; q++) {
           const int j = tid + q * Block;
-          il[q] = j < nb ? ld_dev(&a.bilevel[b0 + j]) : kUnvisited;
-          ol[q] = fresh && j < nb ? ld_dev(&a.blevel[b0 + j]) : 0;
+          il[q] = j < nb ? ld_dev(&bilev[b0 + j]) : kUnvisited;
+          ol[q] = fresh && j < nb ? ld_dev(&blev[b0 + j]) : 0;
         }
       }
-      if (fresh) {
+      if (fresh_blob) {
         const int we = eb / 16;
         const uint4 *ge = a.oblob + a.oblob_off[p];
         uint4 *se = reinterpret_cast<uint4 *>(smem + ob);
@@ -1073,12 +1090,14 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
           for (int j = 0; j < 8; j++)
             if (i + j * Block < we) se[i + j * Block] = t[j];
         }
+      }
+      if (fresh) {
         if (few) {
 #pragma unroll
           for (int q = 0; q < kR; q++)
             if (tid + q * Block < nb) lv[tid + q * Block] = ol[q];
         } else {
-          for (int j = tid; j < nb; j += Block) lv[j] = ld_dev(&a.blevel[b0 + j]);
+          for (int j = tid; j < nb; j += Block) lv[j] = ld_dev(&blev[b0 + j]);
         }
         if (stats && tid == 0) n_img++;
       }
@@ -1116,7 +1135,7 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
       } else {
         for (int jj = 0; jj < nb; jj += Block) {
           const int j = jj + tid;
-          seed(j < nb ? j : 0, j < nb ? ld_dev(&a.bilevel[b0 + j]) : kUnvisited, j < nb);
+          seed(j < nb ? j : 0, j < nb ? ld_dev(&bilev[b0 + j]) : kUnvisited, j < nb);
         }
       }
       __syncthreads();
@@ -1139,7 +1158,7 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
       // external edges (four in flight per thread) and flag their parts
       for (int j = tid; j < nb; j += Block)
         if (chg[j]) {
-          st_dev(&a.blevel[b0 + j], lv[j]);
+          st_dev(&blev[b0 + j], lv[j]);
           if (stats) atomicAdd(&a.stats[11], 1ull);
         }
       for (int xx = 0; xx < nx; xx += 4 * Block) {
@@ -1157,7 +1176,7 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
           }
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) old[q] = on[q] ? atomicMin(&a.bilevel[xd[xx + q * Block + tid]], nl[q]) : 0;
+        for (int q = 0; q < 4; q++) old[q] = on[q] ? atomicMin(&bilev[xd[xx + q * Block + tid]], nl[q]) : 0;
 #pragma unroll
         for (int q = 0; q < 4; q++)
           if (on[q] && old[q] > nl[q]) nflag[xn[xx + q * Block + tid]] = 1;
@@ -1167,7 +1186,7 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
       // queued neighbour for its own workgroup instead of the ring ran the
       // wave's front out of order and re-ran parts more, 1.15 -> 1.42 ms)
       for (int i = tid; i < nn; i += Block)
-        if (nflag[i]) mark_lowered(a, nbl[i]);
+        if (nflag[i]) mark_lowered(a, ibase + nbl[i]);
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
       if (stats && tid == 0) {
@@ -1178,21 +1197,22 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
       if (tid == 0) {
         const uint32_t v = s_ver;
         s_ver = v + 1u;
-        const uint32_t old = atomicCAS(&a.st[p], v << 2 | 2u, (v + 1u) << 2);
+        const uint32_t old = atomicCAS(&a.st[item], v << 2 | 2u, (v + 1u) << 2);
         if (old == (v << 2 | 2u)) {
           s_rerun = 0;
           const unsigned long long tp = atomicSub(tp_word(a.ctl), 1ull);
           if ((uint32_t)tp == 1u) mark_done(a, (uint32_t)(tp >> 32));  // the last part: step 2 is over
         } else {
           // lowered again while it ran: run it once more
-          atomicExch(&a.st[p], (v + 1u) << 2 | 2u);
+          atomicExch(&a.st[item], (v + 1u) << 2 | 2u);
           __builtin_amdgcn_s_waitcnt(0);
           s_rerun = 1;
         }
       }
       __syncthreads();
-      cached = p;
+      cached = item;
       cached_ver = s_ver;
+      cached_part = p;
       if (!s_rerun) break;
     }
   }
@@ -1216,11 +1236,14 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
   unsigned long long t_fin = 0, n_fr = 0;
   if (stats && tid == 0) t0 = __builtin_amdgcn_s_memrealtime();
   for (;;) {
+    // (part, source) pairs part by part, so that the sources of one part
+    // run together and its image is read from HBM about once
     if (tid == 0) s_part = atomicAdd(&a.ctl[kCtlFin], 1u);
     __syncthreads();
-    const uint32_t p = s_part;
-    if ((int64_t)p >= a.k) break;
-    n_fr += part_bfs<Block, Lanes, Claims>(a, p, true, smem, s_cnt, &s_min, s_dummy, acc);
+    const uint32_t t = s_part;
+    if ((int64_t)t >= a.k * a.nsrc) break;
+    n_fr += part_bfs<Block, Lanes, Claims>(a, t / (uint32_t)a.nsrc, (int)(t % (uint32_t)a.nsrc), true, smem, s_cnt,
+                                           &s_min, s_dummy, acc);
     __syncthreads();
   }
   if (stats && tid == 0) {
@@ -1392,6 +1415,7 @@ struct PartGraph {
   int32_t *blevel = nullptr, *bilevel = nullptr;
   ObfsShape oshape{};
   size_t olds = 0;  // dynamic LDS of the boundary-distance kernels
+  int ocap = 1;     // sources the item state, ring and boundary levels have room for
 };
 
 // Frees the boundary-distance traversal's arrays (the partitioned traversal
@@ -1753,11 +1777,40 @@ int64_t pbfs_num_boundary(const phde_ctx *c) {
 }
 
 // One traversal by the boundary-distance method (steps 1-3 above).
-static int obfs_run(phde_ctx *c, PartGraph *pg, int64_t src, int32_t *dist_out, int32_t *min_dist, bool *ran) {
+// Room for nsrc sources searched together: item state (nsrc * k), the
+// ring, the per-source boundary levels.
+static int obfs_reserve(phde_ctx *c, PartGraph *pg, int nsrc) {
+  if (pg->ocap >= nsrc) return PHDE_OK;
+  const uint32_t ring_n = (uint32_t)(nsrc * pg->k + 4 * std::max(1, c->num_cus) + 64);
+  PHDE_HIP(hipStreamSynchronize(c->stream));
+  for (void *x : {(void *)pg->st, (void *)pg->ring, (void *)pg->blevel, (void *)pg->bilevel, (void *)pg->stats})
+    if (x) (void)hipFree(x);
+  pg->st = nullptr;
+  pg->ring = nullptr;
+  pg->blevel = pg->bilevel = nullptr;
+  const bool stats = pg->stats != nullptr;
+  pg->stats = nullptr;
+  PHDE_HIP(hipMalloc(&pg->st, sizeof(uint32_t) * nsrc * pg->k));
+  PHDE_HIP(hipMalloc(&pg->ring, sizeof(unsigned long long) * kSlotStride * ring_n));
+  PHDE_HIP(hipMalloc(&pg->blevel, sizeof(int32_t) * std::max<int64_t>(nsrc * pg->nbtot, 1)));
+  PHDE_HIP(hipMalloc(&pg->bilevel, sizeof(int32_t) * std::max<int64_t>(nsrc * pg->nbtot, 1)));
+  if (stats) PHDE_HIP(hipMalloc(&pg->stats, sizeof(unsigned long long) * (32 + ring_n)));
+  pg->ring_n = ring_n;
+  pg->ocap = nsrc;
+  return PHDE_OK;
+}
+
+// One boundary-distance search (steps 1-3 above) from nsrc <= kMaxSources
+// sources (internal ids) together; levels to out (n per source, internal
+// ids).  min_dist (one source only): the fused k-centers step.
+static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, int32_t *out, int32_t *min_dist,
+                    bool *ran) {
+  PHDE_TRY_RC(obfs_reserve(c, pg, nsrc));
+  const int64_t src = srcs[0];
   ObfsArgs a;
   a.g = c->g;
-  a.level = dist_out ? dist_out : c->d_dist;
-  a.min_dist = min_dist;
+  a.level = out;
+  a.min_dist = nsrc == 1 ? min_dist : nullptr;
   a.kc = c->d_kc;
   a.k = pg->k;
   a.vstart = pg->vstart;
@@ -1778,8 +1831,12 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, int64_t src, int32_t *dist_out, 
   a.ctl = pg->ctl;
   a.ring = pg->ring;
   a.ring_n = pg->ring_n;
-  a.src_part = pg->h_part_of[src];
-  a.src_loc = pg->h_loc[src];
+  a.nsrc = nsrc;
+  a.nbtot = pg->nbtot;
+  for (int i = 0; i < nsrc; i++) {
+    a.src_part[i] = pg->h_part_of[srcs[i]];
+    a.src_loc[i] = pg->h_loc[srcs[i]];
+  }
   a.stats = pg->stats;
   if (pg->stats) {
     const unsigned long long init[32] = {0, 0, 0, 0, 0, 0, 0, ~0ull, 0};
@@ -1789,17 +1846,17 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, int64_t src, int32_t *dist_out, 
   const int block = pg->oshape.block;
   int resident = 0;
   PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, block, pg->olds));
-  // at most 4 workgroups per CU (the queue's ring holds k + 4 per CU)
+  // at most 4 workgroups per CU (the queue's ring holds the items + 4 per CU)
   const int grid = std::max(1, c->num_cus) * std::max(1, std::min(4, resident));
-  if (resident < 1 || (int64_t)pg->ring_n < pg->k + grid) {
+  if (resident < 1 || (int64_t)pg->ring_n < nsrc * pg->k + grid) {
     obfs_release(pg);
     return PHDE_OK;
   }
   const int ib = (int)std::max<int64_t>(
-      1, std::min<int64_t>((std::max<int64_t>(pg->nbtot, pg->ring_n) + 255) / 256, (int64_t)c->num_cus * 8));
-  obfs_init_kernel<<<ib, 256, 0, c->stream>>>(a, pg->nbtot);
+      1, std::min<int64_t>((std::max<int64_t>(nsrc * pg->nbtot, pg->ring_n) + 255) / 256, (int64_t)c->num_cus * 8));
+  obfs_init_kernel<<<ib, 256, 0, c->stream>>>(a);
   PHDE_HIP(hipGetLastError());
-  pg->oshape.src_fn<<<1, block, pg->olds, c->stream>>>(a);
+  pg->oshape.src_fn<<<nsrc, block, pg->olds, c->stream>>>(a);
   PHDE_HIP(hipGetLastError());
   void *args[] = {&a};
   const hipError_t e = launch_resident((const void *)fn, grid, block, args, pg->olds, c->stream);
@@ -1841,12 +1898,19 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, int64_t src, int32_t *dist_out, 
   return PHDE_OK;
 }
 
+int pbfs_multi_run(phde_ctx *c, const int64_t *src, int k, int32_t *D, bool *ran) {
+  *ran = false;
+  PartGraph *pg = c->pg;
+  if (!pg || pg->disabled || !c->bfs_persistent || !pg->ovl || k < 1 || k > kMaxSources) return PHDE_OK;
+  return obfs_run(c, pg, src, k, D, nullptr, ran);
+}
+
 int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, bool *ran) {
   *ran = false;
   PartGraph *pg = c->pg;
   if (!pg || pg->disabled || !c->bfs_persistent) return PHDE_OK;
   if (pg->ovl) {
-    PHDE_TRY_RC(obfs_run(c, pg, src, dist_out, min_dist, ran));
+    PHDE_TRY_RC(obfs_run(c, pg, &src, 1, dist_out ? dist_out : c->d_dist, min_dist, ran));
     if (*ran) return PHDE_OK;
   }
   PbfsArgs a;

@@ -28,6 +28,7 @@
 #include "hops.h"
 #include "phde_common.h"
 #include "levels.h"
+#include "pbfs.h"
 #include "traverse.h"
 
 
@@ -906,6 +907,20 @@ int msbfs_run(phde_ctx *c, const int64_t *sources, int k, int32_t *D) {
     if (!c->ts_cnt) PHDE_HIP(hipMalloc(&c->ts_cnt, 256));
   }
   PHDE_HIP(hipMemcpyAsync(c->ms_src, sources, sizeof(int64_t) * k, hipMemcpyHostToDevice, c->stream));
+  if (c->pg) {
+    // a deep graph cut into parts with per-part distance tables: the k
+    // searches run together by the boundary-distance method (pbfs.hip)
+    bool ran = false;
+    hipEvent_t pev;
+    c->timer.begin(c->stream, PHDE_KERNEL_MSBFS, &pev);
+    rc = pbfs_multi_run(c, sources, k, D, &ran);
+    if (!rc && ran) {
+      mshop_finish_kernel<<<std::max(1, c->num_cus) * 4, kTBlock, 0, c->stream>>>(D, n * k);
+      PHDE_HIP(hipGetLastError());
+    }
+    c->timer.end(c->stream, PHDE_KERNEL_MSBFS, pev);
+    if (rc || ran) return rc;
+  }
   // hops per round and first batch of the concurrent searches (PHDE_MS_HOPS=0:
   // the bit-parallel sweep for every graph)
   static const int ms_hops = [] {

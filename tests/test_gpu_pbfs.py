@@ -122,3 +122,23 @@ def test_unreached_component(gpu, mode):
         with pytest.raises(L.PhdeError) as e:
             ctx.layout(s=4, num_coord=2)
         assert e.value.code == -7
+
+
+@pytest.mark.parametrize("budget", BUDGETS)
+@pytest.mark.parametrize("name", list(GRAPHS))
+@pytest.mark.parametrize("k", [1, 7, 64])
+def test_multi_source_columns_exact(gpu, name, budget, k):
+    # the k searches of one multi-source call run together by the
+    # boundary-distance method: every column equals its own textbook BFS,
+    # with repeated sources and sources in one part
+    g = GRAPHS[name]()
+    rng = np.random.default_rng(100 + k)
+    src = [int(x) for x in rng.choice(g.n, size=k, replace=k > g.n)]
+    if k >= 7:
+        src[1] = src[0]          # a repeated source
+        src[2] = (src[0] + 1) % g.n  # a neighbour in the id order, most often in the same part
+    with context(g, budget) as ctx:
+        check_mode(ctx, "obfs")
+        D = ctx.msbfs(src)
+    for j, s in enumerate(src):
+        assert np.array_equal(D[:, j], O.seq_bfs(g, s)), (name, budget, k, j)
