@@ -598,6 +598,13 @@ __host__ __device__ constexpr int64_t cblob_bytes(int64_t nv, int64_t ne, int64_
 __host__ __device__ constexpr int64_t final_bytes(int64_t nv, int64_t ne, int64_t nb) {
   return cblob_bytes(nv, ne, nb) + 4 * nv + align_up(4 * (nv + 1), 4) + align_up(2 * (nb + 2), 4) + 4 * (nb + 2);
 }
+// LDS of a multi-source interior pass (part_bfs_multi): the image, three
+// 32-bit source masks per vertex, the seeds' relative levels (bytes, 32 per
+// boundary vertex) and the sources' lowest levels in the part.
+constexpr int kMultiW = 32;
+__host__ __device__ constexpr int64_t mfinal_bytes(int64_t nv, int64_t ne, int64_t nb) {
+  return cblob_bytes(nv, ne, nb) + 12 * nv + align_up(kMultiW * nb, 16) + 4 * kMultiW + 16;
+}
 // LDS of a run in step 2: boundary levels and seed levels (int32 nb each),
 // seed indices (uint16 nb), changed flags (nb) and neighbouring-part flags (nn).
 __host__ __device__ constexpr int64_t orun_bytes(int64_t nb, int64_t nn) {
@@ -643,6 +650,9 @@ struct ObfsArgs {
   int nsrc;
   int64_t nbtot;
   uint32_t src_part[kMaxSources], src_loc[kMaxSources];
+  // multi-source interiors: each vertex's arrival round per source of the
+  // current group of 32 (bytes, part-major rows of 32; 255: not reached)
+  unsigned char *mscr;
   // with PHDE_PBFS_STATS: [0..3] wait, seed, min-plus, push (ticks summed
   // over workgroups); [4] runs, [5] seeds, [6] level loads, [7, 8] first
   // and last tick, [9] interior time, [10] interior rounds, [11] boundary
@@ -989,6 +999,139 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, int src, bool final_mode,
   return rounds;
 }
 
+// Step 3 of a multi-source search for sources s0 .. s0 + w - 1 (w <= 32) of
+// part p, all at once: a bit-parallel BFS inside the part (bit i of a
+// vertex's mask: reached from source s0 + i).  Levels are relative to each
+// source's lowest level in the part (lmin[i]: its lowest boundary level, or
+// 0 in the source's own part); round r holds the vertices at level
+// lmin[i] + r for source i, and a boundary vertex joins source i's frontier
+// at its own relative level (exact, so it is never reached earlier).  Each
+// vertex's arrival round per source goes to a byte row of mscr written and
+// read back by the thread that owns the vertex; the write-out fills the w
+// level columns.  The part's image must already be in LDS.  Returns false
+// (nothing written) when a relative level or a round reaches 255: the
+// caller then takes the sources one at a time.
+template <int Block>
+__device__ bool part_bfs_multi(const ObfsArgs &a, uint32_t p, int s0, int w, unsigned char *smem, int *s_mf) {
+  const int tid = threadIdx.x;
+  const uint32_t v0 = a.vstart[p];
+  const int nv = (int)(a.vstart[p + 1] - v0);
+  const uint32_t b0 = a.bstart[p];
+  const int nb = (int)(a.bstart[p + 1] - b0);
+  const uint16_t *lrow = reinterpret_cast<const uint16_t *>(smem);
+  const int ne = lrow[nv];
+  const uint16_t *ladj = lrow + nv + 1;
+  const uint16_t *blist = ladj + ne;
+  uint32_t *vis = reinterpret_cast<uint32_t *>(smem + cblob_bytes(nv, ne, nb));
+  uint32_t *fr = vis + nv;
+  uint32_t *nx = fr + nv;
+  unsigned char *rel = reinterpret_cast<unsigned char *>(nx + nv);
+  int32_t *lmin = reinterpret_cast<int32_t *>(rel + align_up(kMultiW * nb, 16));
+  unsigned char *r8 = a.mscr + (size_t)v0 * kMultiW;
+  for (int v = tid; v < nv; v += Block) {
+    vis[v] = fr[v] = nx[v] = 0;
+    uint4 *row = reinterpret_cast<uint4 *>(r8 + (size_t)v * kMultiW);
+    row[0] = row[1] = make_uint4(~0u, ~0u, ~0u, ~0u);
+  }
+  if (tid < kMultiW) lmin[tid] = tid < w && a.src_part[s0 + tid] == p ? 0 : kUnvisited;
+  if (tid == 0) s_mf[0] = s_mf[1] = s_mf[2] = s_mf[3] = 0;
+  __syncthreads();
+  // each source's lowest boundary level in the part
+  for (int idx = tid; idx < nb * w; idx += Block) {
+    const int i = idx / nb, b = idx % nb;
+    const int32_t L = ld_dev(&a.blevel[(s0 + i) * a.nbtot + b0 + b]);
+    if (L != kUnvisited) atomicMin(&lmin[i], L);
+  }
+  __syncthreads();
+  // the seeds' relative levels (255: not a seed of that source)
+  for (int idx = tid; idx < nb * kMultiW; idx += Block) {
+    const int i = idx / nb, b = idx % nb;
+    int r = 255;
+    if (i < w) {
+      const int32_t L = ld_dev(&a.blevel[(s0 + i) * a.nbtot + b0 + b]);
+      if (L != kUnvisited) {
+        const int32_t d = L - lmin[i];
+        if (d >= 255)
+          s_mf[0] = 1;
+        else {
+          r = d;
+          atomicMax(&s_mf[1], d);
+        }
+      }
+    }
+    rel[b * kMultiW + i] = (unsigned char)r;
+  }
+  __syncthreads();
+  if (s_mf[0]) return false;
+  const int rmax = s_mf[1];
+  const uint32_t *rel32 = reinterpret_cast<const uint32_t *>(rel);
+  for (int r = 0;; r++) {
+    // the seeds of round r join (and the sources in their own part at 0)
+    for (int idx = tid; idx < nb * (kMultiW / 4); idx += Block) {
+      const int b = idx / (kMultiW / 4), q = idx % (kMultiW / 4);
+      const uint32_t word = rel32[idx];
+      uint32_t bits = 0;
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        if (((word >> (8 * c)) & 0xffu) == (uint32_t)r) bits |= 1u << (4 * q + c);
+      if (bits) {
+        const int v = blist[b];
+        atomicOr(&fr[v], bits);
+        atomicOr(&vis[v], bits);
+      }
+    }
+    if (r == 0 && tid < w && a.src_part[s0 + tid] == p) {
+      const int v = (int)a.src_loc[s0 + tid];
+      atomicOr(&fr[v], 1u << tid);
+      atomicOr(&vis[v], 1u << tid);
+    }
+    __syncthreads();
+    // the round's frontier: arrival rounds recorded, masks offered to the
+    // neighbours
+    for (int v = tid; v < nv; v += Block) {
+      const uint32_t f = fr[v];
+      if (!f) continue;
+      uint32_t bits = f;
+      while (bits) {
+        const int i = __ffs(bits) - 1;
+        bits &= bits - 1u;
+        r8[(size_t)v * kMultiW + i] = (unsigned char)r;
+      }
+      const int e = lrow[v + 1];
+      for (int x = lrow[v]; x < e; x++) atomicOr(&nx[ladj[x]], f);
+    }
+    __syncthreads();
+    for (int v = tid; v < nv; v += Block) {
+      const uint32_t nw = nx[v] & ~vis[v];
+      fr[v] = nw;
+      vis[v] |= nw;
+      nx[v] = 0;
+      if (nw) s_mf[2 + (r & 1)] = 1;
+    }
+    if (tid == 0) s_mf[2 + ((r + 1) & 1)] = 0;
+    __syncthreads();
+    if (!s_mf[2 + (r & 1)] && r >= rmax) break;
+    // a frontier that would reach round 255 has no byte for its rounds
+    // (nothing is written yet: the caller takes the sources one at a time)
+    if (r + 1 >= 255) return false;
+  }
+  // write-out: each thread reads back its own vertices' rows
+  for (int v = tid; v < nv; v += Block) {
+    const uint32_t i_int = a.pverts[v0 + v];
+    const uint4 *row = reinterpret_cast<const uint4 *>(r8 + (size_t)v * kMultiW);
+    const uint4 q0 = row[0], q1 = row[1];
+    const uint32_t wd[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+    for (int i = 0; i < kMultiW; i++)
+      if (i < w) {
+        const uint32_t d8 = (wd[i / 4] >> (8 * (i % 4))) & 0xffu;
+        a.level[(s0 + i) * a.g.n + i_int] = d8 == 255u ? kUnvisited : lmin[i] + (int32_t)d8;
+      }
+  }
+  __syncthreads();
+  return true;
+}
+
 // Step 1: one workgroup, the source's part.
 template <int Block, int Lanes, int Claims>
 __global__ __launch_bounds__(Block) void obfs_source_kernel(ObfsArgs a) {
@@ -1231,20 +1374,20 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
   // the queue is empty and every run has completed: the boundary levels
   // are exact (unless an error stopped the search; the host then repeats
   // the traversal another way)
-  if (ld_dev(&a.ctl[kCtlErr])) return;
+  // a multi-source search fills its interiors in obfs_multi_final_kernel
+  if (a.nsrc > 1 || ld_dev(&a.ctl[kCtlErr])) return;
   KcAcc acc;
   unsigned long long t_fin = 0, n_fr = 0;
   if (stats && tid == 0) t0 = __builtin_amdgcn_s_memrealtime();
-  for (;;) {
-    // (part, source) pairs part by part, so that the sources of one part
-    // run together and its image is read from HBM about once
-    if (tid == 0) s_part = atomicAdd(&a.ctl[kCtlFin], 1u);
-    __syncthreads();
-    const uint32_t t = s_part;
-    if ((int64_t)t >= a.k * a.nsrc) break;
-    n_fr += part_bfs<Block, Lanes, Claims>(a, t / (uint32_t)a.nsrc, (int)(t % (uint32_t)a.nsrc), true, smem, s_cnt,
-                                           &s_min, s_dummy, acc);
-    __syncthreads();
+  {
+    for (;;) {
+      if (tid == 0) s_part = atomicAdd(&a.ctl[kCtlFin], 1u);
+      __syncthreads();
+      const uint32_t p = s_part;
+      if ((int64_t)p >= a.k) break;
+      n_fr += part_bfs<Block, Lanes, Claims>(a, p, 0, true, smem, s_cnt, &s_min, s_dummy, acc);
+      __syncthreads();
+    }
   }
   if (stats && tid == 0) {
     t_fin = __builtin_amdgcn_s_memrealtime() - t0;
@@ -1265,6 +1408,53 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
     if (acc.unreached) atomicAdd(&a.kc->unreached, acc.unreached);
     if (maxd > 0) atomicMax(reinterpret_cast<unsigned long long *>(&a.kc->depth), (unsigned long long)maxd);
   }
+}
+
+// Step 3 of a multi-source search (after obfs_kernel's queue): part by part,
+// the sources in groups of 32 by part_bfs_multi where the part fits, else
+// one at a time.  Every block takes parts until none is left.
+template <int Block, int Lanes, int Claims>
+__global__ __launch_bounds__(Block) void obfs_multi_final_kernel(ObfsArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  __shared__ uint32_t s_part;
+  __shared__ int s_cnt[3], s_min, s_mf[4];
+  __shared__ int32_t s_dummy[Block];
+  const int tid = threadIdx.x;
+  KcAcc acc;
+  unsigned long long n_fr = 0;
+    // part by part: groups of 32 sources at once where the part fits
+    for (;;) {
+      if (tid == 0) s_part = atomicAdd(&a.ctl[kCtlFin], 1u);
+      __syncthreads();
+      const uint32_t p = s_part;
+      if ((int64_t)p >= a.k) break;
+      const int nv = (int)(a.vstart[p + 1] - a.vstart[p]);
+      const int nb = (int)(a.bstart[p + 1] - a.bstart[p]);
+      const int nblob = (int)(a.cblob_off[p + 1] - a.cblob_off[p]);  // the image, in 16-byte units
+      const bool fits = (int64_t)16 * nblob + 12 * nv + align_up(kMultiW * nb, 16) + 4 * kMultiW + 16 <= (int64_t)a.olds;
+      bool image = false;
+      for (int s0 = 0; s0 < a.nsrc; s0 += kMultiW) {
+        const int w = a.nsrc - s0 < kMultiW ? a.nsrc - s0 : kMultiW;
+        bool done = false;
+        if (fits) {
+          if (!image) {
+            load_cblob<Block>(a.cblob, a.cblob_off, p, smem);
+            __syncthreads();
+            image = true;
+          }
+          done = part_bfs_multi<Block>(a, p, s0, w, smem, s_mf);
+        }
+        if (!done) {
+          for (int i = 0; i < w; i++) {
+            n_fr += part_bfs<Block, Lanes, Claims>(a, p, s0 + i, true, smem, s_cnt, &s_min, s_dummy, acc);
+            __syncthreads();
+          }
+          image = false;
+        }
+      }
+      __syncthreads();
+    }
+  (void)n_fr;
 }
 
 // Dp of every part (context creation): bit-parallel BFS inside the part
@@ -1382,14 +1572,14 @@ struct ObfsShape {
   int block;
   void (*fn)(ObfsArgs);
   void (*src_fn)(ObfsArgs);
+  void (*mfin_fn)(ObfsArgs);
 };
+#define OBFS_SHAPE(B, L, C) {B, obfs_kernel<B, L, C>, obfs_source_kernel<B, L, C>, obfs_multi_final_kernel<B, L, C>}
 static const ObfsShape kObfsShapes[] = {
-    {1024, obfs_kernel<1024, 16, 1>, obfs_source_kernel<1024, 16, 1>},
-    {1024, obfs_kernel<1024, 8, 2>, obfs_source_kernel<1024, 8, 2>},
-    {512, obfs_kernel<512, 8, 2>, obfs_source_kernel<512, 8, 2>},
-    {512, obfs_kernel<512, 4, 3>, obfs_source_kernel<512, 4, 3>},
-    {256, obfs_kernel<256, 4, 3>, obfs_source_kernel<256, 4, 3>},
+    OBFS_SHAPE(1024, 16, 1), OBFS_SHAPE(1024, 8, 2), OBFS_SHAPE(512, 8, 2), OBFS_SHAPE(512, 4, 3),
+    OBFS_SHAPE(256, 4, 3),
 };
+#undef OBFS_SHAPE
 
 // The parts of a deep graph (device arrays) and the traversal's state.
 struct PartGraph {
@@ -1416,13 +1606,14 @@ struct PartGraph {
   ObfsShape oshape{};
   size_t olds = 0;  // dynamic LDS of the boundary-distance kernels
   int ocap = 1;     // sources the item state, ring and boundary levels have room for
+  unsigned char *mscr = nullptr;  // multi-source interiors: arrival rounds (n x 32 bytes)
 };
 
 // Frees the boundary-distance traversal's arrays (the partitioned traversal
 // stays usable).
 static void obfs_release(PartGraph *pg) {
-  void *ptr[] = {pg->pverts,   pg->bstart, pg->xoff, pg->noff, pg->cblob_off, pg->oblob_off,
-                 pg->cblob,    pg->oblob,  pg->dmat, pg->doff, pg->blevel,    pg->bilevel};
+  void *ptr[] = {pg->pverts, pg->bstart, pg->xoff, pg->noff,   pg->cblob_off, pg->oblob_off, pg->cblob,
+                 pg->oblob,  pg->dmat,   pg->doff, pg->blevel, pg->bilevel,   pg->mscr};
   for (void *x : ptr)
     if (x) (void)hipFree(x);
   pg->pverts = pg->bstart = pg->xoff = pg->noff = pg->cblob_off = pg->oblob_off = nullptr;
@@ -1430,6 +1621,7 @@ static void obfs_release(PartGraph *pg) {
   pg->dmat = nullptr;
   pg->doff = nullptr;
   pg->blevel = pg->bilevel = nullptr;
+  pg->mscr = nullptr;
   pg->ovl = false;
 }
 
@@ -1461,7 +1653,7 @@ struct ObfsHost {
   int64_t nx_tot = 0, nn_tot = 0;
   // largest LDS of step 3 (and 1), of a run of step 2 (with its edge
   // blob), and of dmat_kernel with 32-, 16- and 8-bit masks
-  int64_t max_final = 0, max_orun = 0, max_dm[3] = {0, 0, 0}, max_nb = 0;
+  int64_t max_final = 0, max_orun = 0, max_mf = 0, max_dm[3] = {0, 0, 0}, max_nb = 0;
   bool ok = true;
   void add_part(int64_t p, uint32_t v0, uint32_t nv, const std::vector<uint32_t> &verts,
                 const std::vector<uint16_t> &lrow, const std::vector<uint16_t> &ladj,
@@ -1506,6 +1698,7 @@ struct ObfsHost {
     coff.push_back((uint32_t)cblob.size());
     doff.push_back(doff.back() + (unsigned long long)(nb * drow(nb)));
     max_final = std::max(max_final, final_bytes(nv, ne, nb));
+    max_mf = std::max(max_mf, mfinal_bytes(nv, ne, nb));
     max_nb = std::max(max_nb, nb);
     const int64_t w[3] = {4, 2, 1};
     for (int i = 0; i < 3; i++) max_dm[i] = std::max(max_dm[i], cb + align_up(2 * nv, 16) + 3 * w[i] * nv);
@@ -1571,9 +1764,11 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   pg->oshape = kObfsShapes[1];
   if (const char *e = getenv("PHDE_OBFS_SHAPE"))
     pg->oshape = kObfsShapes[std::max(0, std::min((int)(sizeof(kObfsShapes) / sizeof(kObfsShapes[0])) - 1, atoi(e)))];
-  // LDS: step 3's largest part, and a run's state with its edge blob
-  pg->olds = (size_t)align_up(std::max(oh.max_final, oh.max_orun), 256);
-  for (const void *f : {(const void *)pg->oshape.fn, (const void *)pg->oshape.src_fn})
+  // LDS: step 3's largest part, a run's state with its edge blob and, up to
+  // 152 KB (the kernel's static LDS takes 4 KB more), the multi-source
+  // interior pass (parts beyond it take their sources one at a time)
+  pg->olds = (size_t)align_up(std::max({oh.max_final, oh.max_orun, std::min<int64_t>(oh.max_mf, 152 * 1024)}), 256);
+  for (const void *f : {(const void *)pg->oshape.fn, (const void *)pg->oshape.src_fn, (const void *)pg->oshape.mfin_fn})
     PHDE_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pg->olds));
   if (pg->stats)
     std::fprintf(stderr, "[obfs] boundary %lld (largest part %lld), Dp %.1f MB in %.1f ms (%d-bit masks)%s\n",
@@ -1795,6 +1990,7 @@ static int obfs_reserve(phde_ctx *c, PartGraph *pg, int nsrc) {
   PHDE_HIP(hipMalloc(&pg->blevel, sizeof(int32_t) * std::max<int64_t>(nsrc * pg->nbtot, 1)));
   PHDE_HIP(hipMalloc(&pg->bilevel, sizeof(int32_t) * std::max<int64_t>(nsrc * pg->nbtot, 1)));
   if (stats) PHDE_HIP(hipMalloc(&pg->stats, sizeof(unsigned long long) * (32 + ring_n)));
+  if (nsrc > 1 && !pg->mscr) PHDE_HIP(hipMalloc(&pg->mscr, (size_t)kMultiW * std::max<int64_t>(c->g.n, 1)));
   pg->ring_n = ring_n;
   pg->ocap = nsrc;
   return PHDE_OK;
@@ -1833,6 +2029,7 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
   a.ring_n = pg->ring_n;
   a.nsrc = nsrc;
   a.nbtot = pg->nbtot;
+  a.mscr = pg->mscr;
   for (int i = 0; i < nsrc; i++) {
     a.src_part[i] = pg->h_part_of[srcs[i]];
     a.src_loc[i] = pg->h_loc[srcs[i]];
@@ -1864,6 +2061,10 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
     (void)hipGetLastError();
     obfs_release(pg);
     return PHDE_OK;
+  }
+  if (nsrc > 1) {
+    pg->oshape.mfin_fn<<<grid, block, pg->olds, c->stream>>>(a);
+    PHDE_HIP(hipGetLastError());
   }
   unsigned int flag = 0;
   PHDE_HIP(hipMemcpyAsync(c->h_kc, c->d_kc, sizeof(KcResult), hipMemcpyDeviceToHost, c->stream));
