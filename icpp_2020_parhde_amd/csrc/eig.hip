@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kEigThreads) void power_eig_kernel(double *K, const
     }
     // tmpOld = eVecs (ParallelComponents.cpp:83-90)
     for (int e = tid; e < dd; e += kEigThreads) T[e] = E[e];
-    if (tid < 64) flags[tid] = 0;
+    if (tid < 128) flags[tid] = 0;
     __syncthreads();
     // eVecs[k + i*p] += K[k + j*p] * tmpOld[j + i*p], j ascending (:91-100)
     for (int e = tid; e < dd; e += kEigThreads) {
@@ -128,21 +128,29 @@ __global__ __launch_bounds__(kEigThreads) void power_eig_kernel(double *K, const
       E[e] = acc;
     }
     __syncthreads();
-    // deflation (:102-112)
+    // deflation (:102-112).  Column j is published in two steps: its
+    // entries (flags[j]) as soon as they are final, then its squared norm
+    // (flags[64 + j]), so that a later column's dot product with it runs
+    // while its norm is still being summed; the arithmetic is unchanged.
     for (int i = wid; i < dim; i += kEigWaves) {
       double x = on ? E[lane + i * dim] : 0.0;
       for (int j = 0; j < i; j++) {
         while (__hip_atomic_load(&flags[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
           __builtin_amdgcn_s_sleep(1);
         const double ej = on ? E[lane + j * dim] : 0.0;
-        const double fac = seq_sum(ej * x, dim, wsum) / djj[j];
+        const double dot = seq_sum(ej * x, dim, wsum);
+        while (__hip_atomic_load(&flags[64 + j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+          __builtin_amdgcn_s_sleep(1);
+        const double fac = dot / djj[j];
         x = x - fac * ej;
       }
       if (on) E[lane + i * dim] = x;
+      wave_lds_sync();
+      if (lane == 0) __hip_atomic_store(&flags[i], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       const double d = seq_sum(x * x, dim, wsum);
       if (lane == 0) {
         djj[i] = d;
-        __hip_atomic_store(&flags[i], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_store(&flags[64 + i], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
     }
     __syncthreads();
@@ -410,7 +418,7 @@ int power_eig_dev(phde_ctx *c, double *K_dev, const double *init_dev, double *ev
                   int max_iter, double *work_dev, int *status_dev) {
   int maxsh = 0;
   PHDE_HIP(hipDeviceGetAttribute(&maxsh, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-  const size_t small = (193 + kEigWaves * kSumRow) * sizeof(double) + 64 * sizeof(int);
+  const size_t small = (193 + kEigWaves * kSumRow) * sizeof(double) + 128 * sizeof(int);
   size_t shmem = 3 * sizeof(double) * dim * dim + small;
   double *Tglob = nullptr;
   if (shmem > (size_t)maxsh) {
