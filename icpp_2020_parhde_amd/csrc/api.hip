@@ -335,7 +335,7 @@ static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, 
     PHDE_TRY(init.alloc((size_t)s * s));
     PHDE_HIP(hipMemcpyAsync(init.p, iv.data(), sizeof(double) * s * s, hipMemcpyHostToDevice,
                             c->stream));
-    PHDE_TRY(power_eig_dev(c, K_dev, init.p, evals.p, s, 100000, work.p, status.p));
+    PHDE_TRY(power_eig_dev(c, K_dev, init.p, evals.p, s, 100000, status.p));
     gather_cols_kernel<<<1, 256, 0, c->stream>>>(K_dev, s, nc, largest, V.p);
   } else {
     PHDE_TRY(evecs.alloc((size_t)s * s));
@@ -1187,12 +1187,11 @@ int phde_kcenters(phde_ctx *c, int s, int64_t first, int64_t *sources, int32_t *
 
 int phde_power_eig(phde_ctx *c, double *K, double *evals, int dim, int32_t *iters) {
   if (!c || !K || !evals || dim < 1 || dim > 64) return PHDE_ERR_ARG;
-  DBuf<double> Kd, init, ev, work;
+  DBuf<double> Kd, init, ev;
   DBuf<int> status;
   PHDE_TRY(Kd.alloc((size_t)dim * dim));
   PHDE_TRY(init.alloc((size_t)dim * dim));
   PHDE_TRY(ev.alloc(dim));
-  PHDE_TRY(work.alloc((size_t)dim * dim));
   PHDE_TRY(status.alloc(4));
   std::vector<int32_t> r(dim * dim);
   glibc_rand_sequence(1, dim * dim, r.data());
@@ -1201,7 +1200,7 @@ int phde_power_eig(phde_ctx *c, double *K, double *evals, int dim, int32_t *iter
   PHDE_HIP(hipMemcpyAsync(Kd.p, K, sizeof(double) * dim * dim, hipMemcpyHostToDevice, c->stream));
   PHDE_HIP(hipMemcpyAsync(init.p, iv.data(), sizeof(double) * dim * dim, hipMemcpyHostToDevice,
                           c->stream));
-  PHDE_TRY(power_eig_dev(c, Kd.p, init.p, ev.p, dim, 100000, work.p, status.p));
+  PHDE_TRY(power_eig_dev(c, Kd.p, init.p, ev.p, dim, 100000, status.p));
   int hs[2] = {0, 0};
   PHDE_HIP(hipMemcpyAsync(hs, status.p, sizeof(int) * 2, hipMemcpyDeviceToHost, c->stream));
   PHDE_HIP(hipMemcpyAsync(K, Kd.p, sizeof(double) * dim * dim, hipMemcpyDeviceToHost, c->stream));

@@ -4,11 +4,11 @@
 
 namespace phde {
 
-// Reference power method on K_dev (dim*dim, overwritten).  init_dev holds
-// the dim*dim start values; work_dev dim*dim doubles; status_dev[0] = 0 ok,
-// 1 NaN/Inf, 2 iteration cap; status_dev[1] = iterations.
+// Reference power method on K_dev (dim*dim, dim <= 64, overwritten).
+// init_dev holds the dim*dim start values; status_dev[0] = 0 ok, 1
+// NaN/Inf, 2 iteration cap; status_dev[1] = iterations.
 int power_eig_dev(phde_ctx *c, double *K_dev, const double *init_dev, double *evals_dev, int dim,
-                  int max_iter, double *work_dev, int *status_dev);
+                  int max_iter, int *status_dev);
 // Exact (generalised when B_dev != nullptr) symmetric eigensolve; work_dev
 // 3*dim*dim doubles; status_dev[0] = 1 when B is not positive definite.
 int sym_eig_dev(phde_ctx *c, const double *A_dev, const double *B_dev, int dim, double *evals_dev,
