@@ -187,6 +187,7 @@ static void ctx_free(phde_ctx *c) {
   if (c->d_dummy) (void)hipFree(c->d_dummy);
   if (c->h_bar) (void)hipHostFree(c->h_bar);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
+  for (hipEvent_t e : c->stage_events) (void)hipEventDestroy(e);
   if (c->h_small) (void)hipHostFree(c->h_small);
   traverse_release(c);
   learned_release(c);
@@ -264,8 +265,10 @@ struct LayoutOut {
 // A large result (coordinates, distance columns) to the caller's memory.
 // The runtime's own staging of a copy into pageable memory measured ≈9.6
 // GB/s on MI355X (268 MB of RGG-16.8M coordinates: 28 ms); instead the
-// result goes to a pinned buffer of the context in one copy, and host
-// threads copy it on to the caller.
+// result goes to a pinned buffer of the context in 8 MB chunks, each
+// followed by an event, and host threads copy every chunk on to the caller
+// as soon as its event completes, so that the host copies overlap the
+// transfers.
 static int result_to_host(phde_ctx *c, void *host, const void *dev, size_t bytes) {
   auto direct = [&]() -> int {
     PHDE_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -284,32 +287,50 @@ static int result_to_host(phde_ctx *c, void *host, const void *dev, size_t bytes
     }
     c->h_stage_bytes = bytes;
   }
-  PHDE_HIP(hipMemcpyAsync(c->h_stage, dev, bytes, hipMemcpyDeviceToHost, c->stream));
-  PHDE_HIP(hipStreamSynchronize(c->stream));
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t nt = std::min<size_t>(std::min(8u, hw), bytes >> 22);
+  constexpr size_t kChunk = (size_t)8 << 20;
+  const size_t nch = (bytes + kChunk - 1) / kChunk;
+  while (c->stage_events.size() < nch) {
+    hipEvent_t e;
+    PHDE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->stage_events.push_back(e);
+  }
   const char *src = static_cast<const char *>(c->h_stage);
   char *dst = static_cast<char *>(host);
-  if (nt <= 1) {
-    std::memcpy(dst, src, bytes);
-    return PHDE_OK;
+  for (size_t i = 0; i < nch; i++) {
+    const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
+    PHDE_HIP(hipMemcpyAsync(static_cast<char *>(c->h_stage) + off, static_cast<const char *>(dev) + off, len,
+                            hipMemcpyDeviceToHost, c->stream));
+    PHDE_HIP(hipEventRecord(c->stage_events[i], c->stream));
   }
-  const size_t part = ((bytes + nt - 1) / nt + 4095) & ~(size_t)4095;
-  std::vector<std::thread> th;
-  size_t done = 0;  // bytes handed to threads, in order
-  try {
-    for (size_t t = 0; t < nt; t++) {
-      const size_t b = t * part;
-      if (b >= bytes) break;
-      const size_t len = std::min(part, bytes - b);
-      th.emplace_back([=] { std::memcpy(dst + b, src + b, len); });
-      done = b + len;
+  // thread t copies chunks t, t + nt, ... after their transfers
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min(8u, hw), nch);
+  std::vector<int> rc(nt, PHDE_OK);
+  auto work = [&](size_t t) {
+    for (size_t i = t; i < nch; i += nt) {
+      if (hipEventSynchronize(c->stage_events[i]) != hipSuccess) {
+        rc[t] = PHDE_ERR_DEVICE;
+        return;
+      }
+      const size_t off = i * kChunk;
+      std::memcpy(dst + off, src + off, std::min(kChunk, bytes - off));
     }
+  };
+  std::vector<std::thread> th;
+  size_t started = 1;  // thread 0's share is the calling thread's
+  try {
+    for (; started < nt; started++) th.emplace_back(work, started);
   } catch (...) {
-    // no thread to spare: the calling thread copies the rest
+    // no thread to spare: the calling thread takes the remaining shares
   }
-  if (done < bytes) std::memcpy(dst + done, src + done, bytes - done);
+  work(0);
+  for (size_t t = started; t < nt; t++) work(t);
   for (auto &t : th) t.join();
+  for (int r : rc)
+    if (r != PHDE_OK) {
+      set_error("copying a result to the host failed");
+      return r;
+    }
   return PHDE_OK;
 }
 

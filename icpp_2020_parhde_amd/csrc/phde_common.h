@@ -126,6 +126,7 @@ struct phde_ctx {
   unsigned int *d_bar = nullptr, *h_bar = nullptr;  // grid-barrier counter + error flag
   void *h_stage = nullptr;  // pinned staging of large results copied to the caller
   size_t h_stage_bytes = 0;
+  std::vector<hipEvent_t> stage_events;  // one per 8 MB chunk of a staged copy
   int32_t *d_dummy = nullptr;                       // per-thread scratch of the BFS claims
   // mapped host flags polled by the level driver (levels.h) and their
   // device-side aliases; d_done is a device-memory copy for the kernels
