@@ -74,7 +74,7 @@ class PhdeStats(ctypes.Structure):
         ("fallback", ctypes.c_int32),
         ("n_candidates", ctypes.c_int32),
         ("n_examined", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("n_gram", ctypes.c_int32),
         ("normdist", ctypes.c_double * 192),
         ("eigenvalues", ctypes.c_double * 64),
         ("dorth_err", ctypes.c_double),

@@ -246,7 +246,7 @@ struct LayoutOut {
   phde_stats *st = nullptr;
   double t_dist = 0, t_orth = 0, t_mmult = 0, t_eig = 0, t_coords = 0, t_select = 0;
   double energy = 0;  // Hall energy of the layout, when requested
-  int ntrav = 0, nacc = 0, ndisc = 0, iters = 0, nexam = 0;
+  int ntrav = 0, nacc = 0, ndisc = 0, iters = 0, nexam = 0, ngram = 0;
   long long levels = 0;
   void pivot(int64_t p) {
     if (st && ntrav < 192) st->pivots[ntrav] = p;
@@ -475,10 +475,44 @@ static bool cgs_blocked(bool weighted, bool fp32) {
   return !weighted && env;
 }
 
+// FP64 ParHDE takes its first candidates (up to s, at most kGramMax) through
+// the Gram-space CGS (Orth::gram_cgs) when no column is accepted yet;
+// PHDE_CGS_GRAM=0 leaves them to the blocked CGS.
+static bool cgs_gram(bool weighted, bool fp32) {
+  static const bool env = [] {
+    const char *e = getenv("PHDE_CGS_GRAM");
+    return !(e && atoi(e) == 0);
+  }();
+  return !weighted && !fp32 && env && cgs_blocked(weighted, fp32);
+}
+
 static int orth_candidates(Orth &orth, const phde_options *o, bool weighted, bool fp32,
                            const int32_t *const *cols, const double *sumsq, int count, int &accepted,
                            LayoutOut &out, int *consumed = nullptr) {
   int i = 0;
+  // the verdicts of candidates [i, i + looked): counts, norms, verbose lines
+  auto record = [&](const std::vector<bool> &taken, const std::vector<double> &norms, int looked, int before) {
+    int acc = before;
+    for (int b = 0; b < looked; b++) {
+      out.examined(norms[b]);
+      if (taken[b]) {
+        acc++;
+      } else {
+        if (o->verbose) std::printf("discarding vec %d, normdist %g\n", acc + 1, norms[b]);
+        out.ndisc++;
+      }
+    }
+  };
+  if (cgs_gram(weighted, fp32) && accepted == 0 && count > 0) {
+    const int nc = std::min({count, o->s, kGramMax});
+    std::vector<bool> taken;
+    std::vector<double> norms;
+    int looked = 0;
+    PHDE_TRY(orth.gram_cgs(cols, sumsq, nc, o->s, accepted, taken, norms, looked));
+    record(taken, norms, looked, 0);
+    out.ngram += looked;
+    i = looked;
+  }
   if (cgs_blocked(weighted, fp32)) {
     while (i < count && accepted < o->s) {
       const int nc = std::min(kCgsBlock, count - i);
@@ -490,17 +524,11 @@ static int orth_candidates(Orth &orth, const phde_options *o, bool weighted, boo
       PHDE_TRY(orth.cgs_block(cols + i, scales, nc, o->s, accepted, taken, norms));
       // candidates looked at: up to the one that completed s (or all)
       int looked = 0, acc = before;
-      for (int b = 0; b < nc; b++) {
-        if (acc >= o->s) break;
+      for (int b = 0; b < nc && acc < o->s; b++) {
         looked++;
-        out.examined(norms[b]);
-        if (taken[b]) {
-          acc++;
-        } else {
-          if (o->verbose) std::printf("discarding vec %d, normdist %g\n", acc + 1, norms[b]);
-          out.ndisc++;
-        }
+        if (taken[b]) acc++;
       }
+      record(taken, norms, looked, before);
       i += looked;
     }
   } else {
@@ -1170,6 +1198,7 @@ int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats
     stats->hall_energy_kcenters = e_kc;
     stats->fallback = fallback;
     stats->n_examined = out.nexam;
+    stats->n_gram = out.ngram;
     for (int k = 0; k < 16; k++) {
       stats->kernel_ms[k] = c->timer.ms[k];
       stats->kernel_launches[k] = c->timer.launches[k];

@@ -11,6 +11,9 @@ namespace phde {
 constexpr int kOnesSlot = 64;
 // Candidates per block of the blocked classical Gram-Schmidt.
 constexpr int kCgsBlock = 8;
+// Candidates per Gram-space classical Gram-Schmidt (with the ones vector,
+// at most 64 coefficients: one per lane of a wave).
+constexpr int kGramMax = 63;
 
 // Device state of one D-orthogonalisation: S (n x s, column-major, FP64 or
 // FP32 storage), the candidate a (FP64), cached denominators
@@ -25,6 +28,9 @@ struct Orth {
   double *a = nullptr, *part = nullptr, *den = nullptr, *coef = nullptr, *scal = nullptr;
   double *h_scal = nullptr, *ones = nullptr;
   double *blkA = nullptr, *blkAcc = nullptr, *blkPart = nullptr, *blkCoef = nullptr;  // blocked CGS
+  double *gPart = nullptr, *gG = nullptr, *gM = nullptr, *gOut = nullptr, *h_gOut = nullptr;  // Gram CGS
+  int *gMax = nullptr;
+  int gBlocks = 0;
   const double *deg_ = nullptr;
   double u0 = 0;
 
@@ -45,6 +51,15 @@ struct Orth {
   // norms[b]: its norm after the projection.
   int cgs_block(const int32_t *const *cols, const double *scales, int nc, int s_target, int &accepted,
                 std::vector<bool> &taken, std::vector<double> &norms);
+  // Classical GS of nc <= kGramMax distance candidates (cols[b] scaled by
+  // 1 / sqrt(sumsq[b]), sumsq[b] the exact sum of its squares) with no
+  // column accepted yet, in Gram space: the Gram matrices of [1, cols] are
+  // summed exactly in one pass, the reference's projections and discard
+  // rule run on coefficient vectors, S is formed in one pass.  consumed:
+  // candidates looked at (0 when the Gram sums could not be exact: the
+  // caller then takes every candidate through cgs_block).
+  int gram_cgs(const int32_t *const *cols, const double *sumsq, int nc, int s_target, int &accepted,
+               std::vector<bool> &taken, std::vector<double> &norms, int &consumed);
   // host out[k] = sum_i deg_i u0 S_ik for k < s, out[s] = sum_i deg_i u0^2
   // (the D-inner products of the ones vector); synchronises
   int ones_dots(double *out);

@@ -456,6 +456,329 @@ __global__ __launch_bounds__(kOrthBlock) void cgs_sub_range_kernel(double *a, co
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// ---------------------------------------------------------------- Gram CGS
+// FP64 storage, unweighted graphs.  The candidates are distance columns, so
+// C = [1, d_1 .. d_N] holds small non-negative integers and the Gram
+// matrices G_I = C^T C and G_D = C^T D C (D: the integer vertex degrees)
+// are sums of integer products.  While max(sum deg, n) * max(d)^2 < 2^53
+// every partial sum is an integer below 2^53, which FP64 holds exactly:
+// one pass over C gives both matrices exactly, whatever the order of the
+// sums.  The reference's classical GS (ParHDE.cpp:190-285) then runs on
+// coefficient vectors in one wave, lane e holding coefficient e: candidate
+// j is a = C (scale_j e_j); the ones vector is C (u0 e_0); an accepted
+// column is S_k = C m_k, with P_k = G_D m_k and den_k = m_k^T P_k kept; the
+// projection coefficient of S_k is num_k / den_k with num_k = scale_j
+// P_k[j]; the residual r has ||C r||_2^2 = r^T G_I r.  The products G r
+// and G_D m sum large terms of both signs down to small results (the
+// residual of a nearly dependent candidate), so they run in double-double
+// arithmetic (error-free products and sums, one rounding at the end); the
+// outer dot products then lose no more than the vector CGS does.  S = C M
+// is formed in one pass at the end.
+
+constexpr int kGcThreads = 256;  // four waves per block of the Gram pass
+constexpr int kGcTiles = 10;     // 16 x 16 tiles of the upper triangle of a 64 x 64 matrix
+constexpr int kGcPart = 2 * kGcTiles * 256;  // partial sums per wave: two matrices' tiles, 4 per lane
+
+// Column pointers, scales and exact sums of squares of the candidates,
+// passed by value.
+struct GramCols {
+  const int32_t *col[kGramMax];
+  double scale[kGramMax];
+  double sumsq[kGramMax];
+};
+
+// Upper-triangle tile number t < 10 -> (p, q), p <= q < 4.
+__device__ __forceinline__ void gram_tile(int t, int &p, int &q) {
+  p = 0;
+  while (t >= 4 - p) {
+    t -= 4 - p;
+    p++;
+  }
+  q = p + t;
+}
+
+// Partial sums of G_I and G_D on the FP64 matrix cores.  C (n x 64, column
+// 0 the ones, columns 1..N the candidates, zero past N) is read straight
+// from the distance columns: a wave takes 4 rows per step, lane l supplying
+// C[row + l/16][16 p + l%16] for p < 4, and issues one
+// v_mfma_f64_16x16x4f64 per tile (p <= q) of each matrix (G_D's left
+// operand scaled by the row's degree).  Operands and products are integers
+// and so is every partial sum below 2^53, which FP64 holds exactly; the
+// result is the same in any order.  Each wave walks a contiguous range of
+// steps (its column segments stay in cache across steps), kGcSteps at a
+// time, the next group's loads issued before the current group's products;
+// it writes its sums to part[wave][tile][lane][4].  part_max[block] = the
+// largest distance the block read.
+constexpr int kGcSteps = 4;
+typedef double gc_v4d __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kGcThreads) void gram_tiles_kernel(GramCols cols, int N, const double *deg, int64_t n,
+                                                               double *part, int *part_max) {
+  __shared__ int smax[kGcThreads / kWave];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int kr = lane / 16, cl = lane % 16;
+  const int32_t *cp[4];
+  int keep[4];  // lane's column p: 0 zero, 1 ones, 2 a candidate
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    const int c = 16 * p + cl;
+    keep[p] = c == 0 ? 1 : c <= N ? 2 : 0;
+    cp[p] = cols.col[min(max(c - 1, 0), N - 1)];
+  }
+  gc_v4d acc[2 * kGcTiles];
+#pragma unroll
+  for (int t = 0; t < 2 * kGcTiles; t++) acc[t] = gc_v4d{0.0, 0.0, 0.0, 0.0};
+  int dmax = 0;
+  const int64_t nsteps = (n + 3) / 4;
+  const int64_t gw = (int64_t)blockIdx.x * (kGcThreads / kWave) + wave;
+  const int64_t nw = (int64_t)gridDim.x * (kGcThreads / kWave);
+  const int64_t s0 = nsteps * gw / nw, s1 = nsteps * (gw + 1) / nw;
+  int vn[kGcSteps][4];
+  double dn[kGcSteps];
+  // unconditional loads (rows clamped in range), masked when used
+  auto load = [&](int64_t g0) {
+#pragma unroll
+    for (int u = 0; u < kGcSteps; u++) {
+      const int64_t rr = min((g0 + u) * 4 + kr, n - 1);
+      dn[u] = deg[rr];
+#pragma unroll
+      for (int p = 0; p < 4; p++) vn[u][p] = cp[p][rr];
+    }
+  };
+  if (s0 < s1) load(s0);
+  for (int64_t g0 = s0; g0 < s1; g0 += kGcSteps) {
+    int vc[kGcSteps][4];
+    double dc[kGcSteps];
+#pragma unroll
+    for (int u = 0; u < kGcSteps; u++) {
+      dc[u] = dn[u];
+#pragma unroll
+      for (int p = 0; p < 4; p++) vc[u][p] = vn[u][p];
+    }
+    if (g0 + kGcSteps < s1) load(g0 + kGcSteps);
+#pragma unroll
+    for (int u = 0; u < kGcSteps; u++) {
+      const bool rok = g0 + u < s1 && (g0 + u) * 4 + kr < n;
+      double a[4], ad[4];
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        const int v = !rok ? 0 : keep[p] == 1 ? 1 : keep[p] == 2 ? vc[u][p] : 0;
+        dmax = max(dmax, v);
+        a[p] = (double)v;
+        ad[p] = a[p] * dc[u];  // integer product below 2^53: exact
+      }
+#pragma unroll
+      for (int t = 0; t < kGcTiles; t++) {
+        const int p = t < 4 ? 0 : t < 7 ? 1 : t < 9 ? 2 : 3;
+        const int q = t < 4 ? t : t < 7 ? t - 3 : t < 9 ? t - 5 : 3;
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[p], a[q], acc[t], 0, 0, 0);
+        acc[kGcTiles + t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ad[p], a[q], acc[kGcTiles + t], 0, 0, 0);
+      }
+    }
+  }
+  double *o = part + gw * kGcPart;
+#pragma unroll
+  for (int t = 0; t < 2 * kGcTiles; t++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) o[(t * kWave + lane) * 4 + v] = acc[t][v];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) dmax = max(dmax, __shfl_xor(dmax, off));
+  if (lane == 0) smax[wave] = dmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int m = 0;
+    for (int w = 0; w < kGcThreads / kWave; w++) m = max(m, smax[w]);
+    part_max[blockIdx.x] = m;
+  }
+}
+
+// G[mat][i][j] (W x W, both triangles; mat 0 = G_I, 1 = G_D) = the sum of
+// the waves' partial tiles, one wave per entry; gmax = the largest
+// distance.  The partials are integers and so is every partial sum: any
+// order gives the same result.  Lane l's output v of a 16 x 16 tile is its
+// row l/16 + 4 v, column l%16 (the lane layout of v_mfma_f64_16x16x4f64).
+__global__ void gram_sum_kernel(const double *part, const int *part_max, int nblk, int W, double *G, int *gmax) {
+  const int nparts = nblk * (kGcThreads / kWave);
+  const int lane = threadIdx.x % kWave;
+  const int e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (e == 0) {
+    int m = 0;
+    for (int b = lane; b < nblk; b += kWave) m = max(m, part_max[b]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_xor(m, o));
+    if (lane == 0) *gmax = m;
+  }
+  if (e >= kGcPart) return;
+  const int t = e / 256, l = (e / 4) % kWave, v = e % 4;
+  const int mat = t / kGcTiles;
+  int p, q;
+  gram_tile(t % kGcTiles, p, q);
+  const int i = 16 * p + l / 16 + 4 * v, j = 16 * q + l % 16;
+  if (i >= W || j >= W) return;
+  double s = 0;
+  for (int b = lane; b < nparts; b += kWave) s += part[(int64_t)b * kGcPart + e];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane != 0) return;
+  G[(int64_t)mat * W * W + i * W + j] = s;
+  G[(int64_t)mat * W * W + j * W + i] = s;
+}
+
+// hi + lo += a * b without rounding error in the product or the leading sum
+// (Dekker / Knuth two-product and two-sum on FMA hardware; no contraction,
+// which would fuse a * b into the sums and break the error-free steps).
+__device__ __forceinline__ void dd_fma(double &hi, double &lo, double a, double b) {
+#pragma clang fp contract(off)
+  const double p = a * b;
+  const double pe = fma(a, b, -p);
+  const double s = hi + p;
+  const double bb = s - hi;
+  const double se = (hi - (s - bb)) + (p - bb);
+  hi = s;
+  lo += se + pe;
+}
+
+// sum_f G[f][lane] x[f] (G rows of stride 64, x in LDS) in double-double:
+// four interleaved chains, combined error-free, rounded once.
+__device__ __forceinline__ double dd_dot(const double *G, const double *x, int W, int lane) {
+#pragma clang fp contract(off)
+  double h[4] = {0, 0, 0, 0}, l[4] = {0, 0, 0, 0};
+  int f = 0;
+  for (; f + 4 <= W; f += 4)
+#pragma unroll
+    for (int u = 0; u < 4; u++) dd_fma(h[u], l[u], G[(f + u) * 64 + lane], x[f + u]);
+  for (; f < W; f++) dd_fma(h[0], l[0], G[f * 64 + lane], x[f]);
+  double hi = h[0], lo = l[0];
+#pragma unroll
+  for (int u = 1; u < 4; u++) {
+    const double s = hi + h[u];
+    const double bb = s - hi;
+    lo += ((hi - (s - bb)) + (h[u] - bb)) + l[u];
+    hi = s;
+  }
+  return hi + lo;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;  // the same value in every lane (each step adds a commuted pair)
+}
+
+// The coefficient-space CGS, one wave.  LDS: GI, GD with row stride 64
+// (row f holds G[f][*]), M [k][64], PT [e][64] (P_k[e] at e*64 + k), and a
+// broadcast row.  Outputs: M_out [k][64], den[k] (the accepted columns'
+// D-norms squared), out[0..N) the candidates' norms, out[N..2N) 1 when
+// taken, out[2N] candidates looked at, out[2N+1] columns accepted.
+__global__ __launch_bounds__(kWave) void gram_cgs_kernel(const double *G, const int *gmax, int N, int W,
+                                                         GramCols cols, double u0, int s_target,
+                                                         double *M_out, double *den, double *out) {
+  extern __shared__ double csm[];
+  double *GI = csm, *GD = GI + W * 64, *Ml = GD + W * 64, *PT = Ml + kGramMax * 64, *row = PT + 64 * 64;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < W * 64; e += kWave) {
+    const int f = e / 64, c = e % 64;
+    GI[e] = c < W ? G[f * W + c] : 0.0;
+    GD[e] = c < W ? G[W * W + f * W + c] : 0.0;
+  }
+  for (int e = lane; e < 64 * 64; e += kWave) PT[e] = 0.0;
+  __syncthreads();
+  // exact sums: every partial sum an integer below 2^53, and the diagonal
+  // of G_I equal to the exact sums of squares the traversals counted
+  const double dm = (double)*gmax;
+  bool exact = max(GD[0], GI[0]) * dm * dm < 9007199254740992.0;
+  for (int j = 1; j <= N; j++) exact = exact && GI[j * 64 + j] == cols.sumsq[j - 1];
+  if (!exact) {
+    if (lane == 0) {
+      out[2 * N] = 0;
+      out[2 * N + 1] = 0;
+    }
+    return;
+  }
+  const double den1 = u0 * u0 * GD[0];  // sum_i u0^2 deg_i
+  double my_den = 1.0;                  // lane k: den_k
+  int acc = 0, looked = 0;
+  for (int j = 1; j <= N && acc < s_target; j++) {
+    const double sc = cols.scale[j - 1];
+    // projection coefficients: lane k < acc of column k; the ones vector's
+    const double ck = lane < acc ? (PT[j * 64 + lane] * sc) * (1 / my_den) : 0.0;
+    const double c1 = (GD[j] * u0 * sc) * (1 / den1);
+    row[lane] = ck;
+    __syncthreads();
+    // sum_k ck M_k, four interleaved partial sums
+    double t4[4] = {0, 0, 0, 0};
+    int k = 0;
+    for (; k + 4 <= acc; k += 4)
+#pragma unroll
+      for (int u = 0; u < 4; u++) t4[u] = fma(row[k + u], Ml[(k + u) * 64 + lane], t4[u]);
+    for (; k < acc; k++) t4[0] = fma(row[k], Ml[k * 64 + lane], t4[0]);
+    double tmp = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+    if (lane == 0) tmp = fma(c1, u0, tmp);
+    const double r = (lane == j ? sc : 0.0) - tmp;
+    __syncthreads();
+    row[lane] = r;
+    __syncthreads();
+    const double q = wave_sum(r * dd_dot(GI, row, W, lane));
+    const double nd = sqrt(max(q, 0.0));
+    looked = j;
+    if (lane == 0) {
+      out[j - 1] = nd;
+      out[N + j - 1] = nd >= 0.001 ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (nd >= 0.001) {
+      const double m = r * (1 / nd);
+      Ml[acc * 64 + lane] = m;
+      M_out[acc * 64 + lane] = m;
+      row[lane] = m;
+      __syncthreads();
+      const double p = dd_dot(GD, row, W, lane);
+      PT[lane * 64 + acc] = p;
+      const double d = wave_sum(m * p);
+      if (lane == acc) my_den = d;
+      if (lane == 0) den[acc] = d;
+      acc++;
+      __syncthreads();
+    }
+  }
+  if (lane == 0) {
+    out[2 * N] = looked;
+    out[2 * N + 1] = acc;
+  }
+}
+
+// S[:, k] = C M_k for the accepted columns k < ncols, one row per thread,
+// the coefficients summed in order.  M_k is exactly zero past coefficient
+// last[k] (its own candidate), so the sum stops at the 8-wide group that
+// holds it (adding fma(0, d, s) = s changes nothing).  Column pointers past
+// N repeat column N - 1, whose coefficients there are zero.
+struct GramForm {
+  int last[kGramMax];
+};
+
+template <int WT>
+__global__ __launch_bounds__(kOrthBlock) void gram_form_kernel(GramCols cols, GramForm fm, int64_t n,
+                                                              const double *__restrict__ M, int ncols, double *S) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  double d[WT];
+  d[0] = 1.0;
+#pragma unroll
+  for (int e = 1; e < WT; e++) d[e] = (double)cols.col[e - 1][v];
+  for (int k = 0; k < ncols; k++) {
+    const double *m = M + k * 64;
+    const int last = fm.last[k];
+    double s = 0;
+#pragma unroll
+    for (int g = 0; g < WT / 8; g++) {
+      if (g * 8 <= last) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) s = fma(m[g * 8 + i], d[g * 8 + i], s);
+      }
+    }
+    S[(int64_t)k * n + v] = s;
+  }
+}
+
 // ---------------------------------------------------------------- host side
 
 static int grid_for(int64_t n, int cap) {
@@ -508,6 +831,14 @@ void Orth::release() {
   if (blkAcc) (void)hipFree(blkAcc);
   if (blkPart) (void)hipFree(blkPart);
   if (blkCoef) (void)hipFree(blkCoef);
+  if (gPart) (void)hipFree(gPart);
+  if (gMax) (void)hipFree(gMax);
+  if (gG) (void)hipFree(gG);
+  if (gM) (void)hipFree(gM);
+  if (gOut) (void)hipFree(gOut);
+  if (h_gOut) (void)hipHostFree(h_gOut);
+  gPart = gG = gM = gOut = h_gOut = nullptr;
+  gMax = nullptr;
   S = nullptr;
   a = part = den = coef = scal = h_scal = ones = nullptr;
   blkA = blkAcc = blkPart = blkCoef = nullptr;
@@ -712,6 +1043,74 @@ int Orth::cgs_block(const int32_t *const *cols, const double *scales, int nc, in
   // bit as the candidate-by-candidate kernels
   return fp32 ? cgs2_block_t<float>(*this, cols, scales, nc, s_target, accepted, taken, norms)
               : cgs_block_t<double>(*this, cols, scales, nc, s_target, accepted, taken, norms);
+}
+
+template <int WT>
+static void gram_form(Orth &o, const GramCols &gc, const GramForm &fm, int ncols) {
+  const int64_t blocks = (o.n + kOrthBlock - 1) / kOrthBlock;
+  gram_form_kernel<WT><<<(unsigned)blocks, kOrthBlock, 0, o.ctx->stream>>>(gc, fm, o.n, o.gM, ncols, (double *)o.S);
+}
+
+int Orth::gram_cgs(const int32_t *const *cols, const double *sumsq, int nc, int s_target, int &accepted,
+                   std::vector<bool> &taken, std::vector<double> &norms, int &consumed) {
+  if (nc < 1 || nc > kGramMax || accepted != 0 || fp32) {
+    set_error("internal: Gram CGS arguments");
+    return PHDE_ERR_ARG;
+  }
+  hipStream_t st = ctx->stream;
+  const int W = (nc + 1 + 3) / 4 * 4;
+  if (!gPart) {
+    const int64_t nsteps = (n + 3) / 4;
+    gBlocks = (int)std::max<int64_t>(1, std::min<int64_t>((nsteps + 3) / 4, 2 * std::max(ctx->num_cus, 1)));
+    PHDE_HIP(hipMalloc(&gPart, sizeof(double) * (size_t)gBlocks * (kGcThreads / kWave) * kGcPart));
+    PHDE_HIP(hipMalloc(&gMax, sizeof(int) * (gBlocks + 1)));
+    PHDE_HIP(hipMalloc(&gG, sizeof(double) * 2 * 64 * 64));
+    PHDE_HIP(hipMalloc(&gM, sizeof(double) * kGramMax * 64));
+    PHDE_HIP(hipMalloc(&gOut, sizeof(double) * (2 * kGramMax + 2)));
+    PHDE_HIP(hipHostMalloc(&h_gOut, sizeof(double) * (2 * kGramMax + 2), hipHostMallocDefault));
+    PHDE_HIP(hipFuncSetAttribute((const void *)gram_cgs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(double) * (2 * 64 * 64 + kGramMax * 64 + 64 * 64 + 64))));
+  }
+  GramCols gc{};
+  for (int b = 0; b < kGramMax; b++) {
+    gc.col[b] = cols[std::min(b, nc - 1)];
+    gc.scale[b] = b < nc ? 1 / std::sqrt(sumsq[b]) : 0.0;
+    gc.sumsq[b] = b < nc ? sumsq[b] : 0.0;
+  }
+  gram_tiles_kernel<<<gBlocks, kGcThreads, 0, st>>>(gc, nc, deg_, n, gPart, gMax);
+  gram_sum_kernel<<<kGcPart / 4, 4 * kWave, 0, st>>>(gPart, gMax, gBlocks, W, gG, gMax + gBlocks);
+  const size_t cgs_lds = sizeof(double) * (2 * W * 64 + kGramMax * 64 + 64 * 64 + 64);
+  gram_cgs_kernel<<<1, kWave, cgs_lds, st>>>(gG, gMax + gBlocks, nc, W, gc, u0, s_target, gM, den, gOut);
+  PHDE_HIP(hipGetLastError());
+  PHDE_HIP(hipMemcpyAsync(h_gOut, gOut, sizeof(double) * (2 * nc + 2), hipMemcpyDeviceToHost, st));
+  PHDE_HIP(hipStreamSynchronize(st));
+  consumed = (int)h_gOut[2 * nc];
+  const int acc = (int)h_gOut[2 * nc + 1];
+  taken.assign(consumed, false);
+  norms.assign(consumed, 0.0);
+  for (int b = 0; b < consumed; b++) {
+    norms[b] = h_gOut[b];
+    taken[b] = h_gOut[nc + b] != 0.0;
+  }
+  if (acc > 0) {
+    GramForm fm{};
+    for (int b = 0, k = 0; b < consumed; b++)
+      if (taken[b]) fm.last[k++] = b + 1;
+    const int WT = (nc + 1 + 7) / 8 * 8;
+    switch (WT) {
+      case 8: gram_form<8>(*this, gc, fm, acc); break;
+      case 16: gram_form<16>(*this, gc, fm, acc); break;
+      case 24: gram_form<24>(*this, gc, fm, acc); break;
+      case 32: gram_form<32>(*this, gc, fm, acc); break;
+      case 40: gram_form<40>(*this, gc, fm, acc); break;
+      case 48: gram_form<48>(*this, gc, fm, acc); break;
+      case 56: gram_form<56>(*this, gc, fm, acc); break;
+      default: gram_form<64>(*this, gc, fm, acc); break;
+    }
+    PHDE_HIP(hipGetLastError());
+  }
+  accepted = acc;
+  return PHDE_OK;
 }
 
 template <typename T>

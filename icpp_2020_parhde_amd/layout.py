@@ -76,6 +76,7 @@ class LayoutStats:
     normdist: list = field(default_factory=list)     # ||a|| of each examined candidate after projection
     eigenvalues: list = field(default_factory=list)  # the s eigenvalue estimates, solver order
     dorth_err: float = 0.0                           # with diagnostics=1: D-orthogonality error of [1 S]
+    n_gram: int = 0                                  # candidates taken through the Gram-space CGS
 
     @classmethod
     def from_c(cls, s: PhdeStats, dim: int = 0) -> "LayoutStats":
@@ -88,7 +89,8 @@ class LayoutStats:
                    {k: int(s.kernel_launches[i]) for i, k in enumerate(KERNEL_CLASSES)},
                    float(s.ms_select), float(s.hall_energy), float(s.hall_energy_kcenters),
                    int(s.fallback), int(s.n_candidates), [float(s.normdist[i]) for i in range(ne)],
-                   [float(s.eigenvalues[i]) for i in range(min(dim, 64))], float(s.dorth_err))
+                   [float(s.eigenvalues[i]) for i in range(min(dim, 64))], float(s.dorth_err),
+                   int(s.n_gram))
 
 
 def default_options(**kw) -> PhdeOptions:

@@ -98,7 +98,7 @@ typedef struct phde_stats {
   int32_t n_candidates;        /* pivots proposed by the selection */
   /* numerical diagnostics (ParHDE / WParHDE) */
   int32_t n_examined;          /* candidates taken through the orthogonalisation (<= 192 recorded) */
-  int32_t reserved;
+  int32_t n_gram;              /* of those, candidates taken through the Gram-space CGS (FP64 ParHDE) */
   double normdist[192];        /* per examined candidate, in order: ||a||_2 after the projection, the
                                   value of the discard test ||a|| < 0.001 (ParHDE.cpp:267-283) */
   double eigenvalues[64];      /* the s eigenvalue estimates of the s*s eigensolve, in the solver's

@@ -1,8 +1,16 @@
-"""Blocked classical Gram-Schmidt (several candidates share the passes over
-the columns accepted before them) against the one-candidate-at-a-time
-path: the same accepted columns and bit-identical coordinates, on graphs
-with and without discarded candidates.  PHDE_CGS_BLOCK is read once per
-process, so the one-at-a-time reference runs in a child process."""
+"""The batched classical Gram-Schmidt paths against the one-candidate-at-a-
+time path, on graphs with and without discarded candidates:
+
+- the blocked CGS (several candidates share the passes over the columns
+  accepted before them, PHDE_CGS_GRAM=0): the same accepted columns and
+  bit-identical coordinates;
+- the Gram-space CGS (the default for FP64 ParHDE: exact integer Gram
+  matrices of the distance columns, the projections on coefficient
+  vectors): the same pivots and discards, coordinates within the FP64
+  tolerance and every candidate's norm within 1e-9 relative.
+
+The switches are read once per process, so every variant runs in a child
+process."""
 import json
 import os
 import subprocess
@@ -22,6 +30,7 @@ CASES = {
     "rgg": ("rgg", 30000, 12, 2, False),
     "mesh_eager": ("mesh", 90, 20, 3, True),
     "path_discards": ("path", 400, 12, 2, False),
+    "rgg_s50": ("rgg", 60000, 50, 2, False),
 }
 
 
@@ -48,21 +57,41 @@ sys.path.insert(0, {tests!r})
 from test_gpu_cgs_block import _layout
 X, st = _layout({kind!r}, {size}, {s}, {c}, {eager})
 np.save({out!r}, X)
-print(json.dumps({{"pivots": list(st.pivots), "ndisc": int(st.n_discarded)}}))
+print(json.dumps({{"pivots": list(st.pivots), "ndisc": int(st.n_discarded),
+                   "normdist": list(st.normdist), "ngram": int(st.n_gram)}}))
 """
 
 
-@pytest.mark.parametrize("case", list(CASES))
-def test_blocked_cgs_equals_one_at_a_time(gpu, case, tmp_path):
+def _run(case, tmp_path, tag, **env_over):
     kind, size, s, c, eager = CASES[case]
-    X, st = _layout(kind, size, s, c, eager)
-    out = str(tmp_path / "x.npy")
-    env = dict(os.environ, PHDE_CGS_BLOCK="0")
+    out = str(tmp_path / f"{tag}.npy")
+    env = dict(os.environ, **env_over)
     code = CHILD.format(root=str(ROOT), tests=str(ROOT / "tests"), kind=kind, size=size, s=s, c=c,
                         eager=eager, out=out)
     r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
-    ref = json.loads(r.stdout.strip().splitlines()[-1])
-    assert list(st.pivots) == ref["pivots"]
-    assert int(st.n_discarded) == ref["ndisc"]
-    assert np.array_equal(X, np.load(out))
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    res["X"] = np.load(out)
+    return res
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_batched_cgs_against_one_at_a_time(gpu, case, tmp_path):
+    one = _run(case, tmp_path, "one", PHDE_CGS_BLOCK="0")
+    blk = _run(case, tmp_path, "blk", PHDE_CGS_GRAM="0")
+    gram = _run(case, tmp_path, "gram")
+    assert one["ngram"] == 0 and blk["ngram"] == 0
+    # the Gram path took the first min(s, candidates) candidates
+    assert gram["ngram"] == min(CASES[case][2], len(gram["normdist"]))
+    assert blk["pivots"] == one["pivots"] and blk["ndisc"] == one["ndisc"]
+    assert np.array_equal(blk["X"], one["X"])
+    assert gram["pivots"] == one["pivots"] and gram["ndisc"] == one["ndisc"]
+    a, b = np.array(gram["normdist"]), np.array(one["normdist"])
+    assert len(a) == len(b)
+    kept = b >= 0.001
+    assert np.all(a[~kept] < 0.001)
+    assert np.max(np.abs(a[kept] / b[kept] - 1)) < 1e-9
+    X, Y = gram["X"], one["X"]
+    for k in range(Y.shape[1]):
+        x = X[:, k] if np.dot(X[:, k], Y[:, k]) >= 0 else -X[:, k]
+        assert np.max(np.abs(x - Y[:, k])) / np.max(np.abs(Y[:, k])) < 1e-8, k
