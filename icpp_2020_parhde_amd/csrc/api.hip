@@ -182,6 +182,7 @@ static void ctx_free(phde_ctx *c) {
   for (void *p : {(void *)c->sp_hub_row, (void *)c->sp_hub_c0, (void *)c->sp_chunk_row, (void *)c->sp_chunk_beg})
     if (p) (void)hipFree(p);
   if (c->d_orig) (void)hipFree(c->d_orig);
+  if (c->d_esrc) (void)hipFree(c->d_esrc);
   if (c->d_perm) (void)hipFree(c->d_perm);
   if (c->d_bar) (void)hipFree(c->d_bar);
   if (c->d_dummy) (void)hipFree(c->d_dummy);
@@ -338,7 +339,8 @@ static int result_to_host(phde_ctx *c, void *host, const void *dev, size_t bytes
 // caller_rows (then the coordinates need no reordering; no Hall energy).
 static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, bool fp32, int s,
                          double *K_dev, const double *B_dev, bool largest, double *coords,
-                         LayoutOut &out, bool want_energy = false, bool caller_rows = false) {
+                         LayoutOut &out, bool want_energy = false, bool caller_rows = false,
+                         Orth *gram = nullptr) {
   const int nc = o->num_coord;
   auto t0 = clk::now();
   DBuf<double> init, evals, work, V, evecs;
@@ -385,7 +387,11 @@ static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, 
   PHDE_TRY(X.alloc((size_t)c->g.n * nc));
   PHDE_TRY(part.alloc((size_t)nc * c->num_cus * 8 + 64));
   PHDE_TRY(inv.alloc(nc));
-  PHDE_TRY(coords_compute(c, S_dev, fp32, s, nc, V.p, X.p, part.p, inv.p));
+  // S in coefficient form (Gram-space CGS): X = C (M V)
+  if (gram && gram->gram_pending)
+    PHDE_TRY(gram->gram_coords(V.p, nc, X.p, part.p, inv.p));
+  else
+    PHDE_TRY(coords_compute(c, S_dev, fp32, s, nc, V.p, X.p, part.p, inv.p));
   DBuf<double> Xo;  // rows in the caller's numbering
   if (renumbered(c) && !caller_rows) {
     PHDE_TRY(Xo.alloc((size_t)c->g.n * nc));
@@ -408,6 +414,16 @@ static int finish_layout(phde_ctx *c, const phde_options *o, const void *S_dev, 
 // the accepted columns and the ones vector are from D-orthogonal (the
 // reference's columns are D-orthogonal and of unit L2 norm, so B is
 // diagonal, not the identity, and is normalised by its diagonal first).
+static double dorth_from_gram(const std::vector<double> &B, int d) {
+  double e = 0;
+  for (int i = 0; i < d; i++)
+    for (int k = 0; k < d; k++) {
+      const double v = B[(size_t)i * d + k] / std::sqrt(B[(size_t)i * d + i] * B[(size_t)k * d + k]);
+      e = std::max(e, std::fabs(v - (i == k ? 1.0 : 0.0)));
+    }
+  return e;
+}
+
 static int dorth_error(phde_ctx *c, Orth &orth, bool fp32, int s, double *err) {
   DBuf<double> G;
   PHDE_TRY(G.alloc((size_t)s * s));
@@ -421,13 +437,7 @@ static int dorth_error(phde_ctx *c, Orth &orth, bool fp32, int s, double *err) {
   for (int k = 0; k < s; k++) B[k + 1] = B[(size_t)(k + 1) * d] = ones[k];
   for (int i = 0; i < s; i++)
     for (int k = 0; k < s; k++) B[(size_t)(i + 1) * d + k + 1] = g[(size_t)i * s + k];
-  double e = 0;
-  for (int i = 0; i < d; i++)
-    for (int k = 0; k < d; k++) {
-      const double v = B[(size_t)i * d + k] / std::sqrt(B[(size_t)i * d + i] * B[(size_t)k * d + k]);
-      e = std::max(e, std::fabs(v - (i == k ? 1.0 : 0.0)));
-    }
-  *err = e;
+  *err = dorth_from_gram(B, d);
   return PHDE_OK;
 }
 
@@ -553,6 +563,31 @@ static int not_enough_pivots(const phde_options *o, int accepted, int limit) {
 static int hde_tail(phde_ctx *c, const phde_options *o, Orth &orth, bool fp32, double *coords,
                     LayoutOut &out, bool want_energy) {
   const int s = o->s;
+  // PHDE_GRAM_TAIL=0: form S and take the vector path below
+  static const bool gram_tail = [] {
+    const char *e = getenv("PHDE_GRAM_TAIL");
+    return !(e && atoi(e) == 0);
+  }();
+  if (orth.gram_pending && !gram_tail) PHDE_TRY(orth.form_S());
+  if (orth.gram_pending) {
+    // S in coefficient form: S^T L S (and S^T D S) from the exact C^T L C
+    auto t2 = clk::now();
+    DBuf<double> K, B;
+    PHDE_TRY(K.alloc((size_t)s * s));
+    const bool gen = o->eig_mode == PHDE_EIG_GENERALIZED;
+    if (gen) PHDE_TRY(B.alloc((size_t)s * s));
+    std::vector<double> Bf;
+    if (o->diagnostics && out.st) Bf.resize((size_t)(s + 1) * (s + 1));
+    bool ok = false;
+    PHDE_TRY(orth.gram_lap(K.p, B.p, Bf.empty() ? nullptr : Bf.data(), ok));
+    if (ok) {
+      PHDE_HIP(hipStreamSynchronize(c->stream));
+      out.t_mmult += ms_since(t2);
+      if (!Bf.empty()) out.st->dorth_err = dorth_from_gram(Bf, s + 1);
+      return finish_layout(c, o, nullptr, fp32, s, K.p, B.p, false, coords, out, want_energy, false, &orth);
+    }
+    PHDE_TRY(orth.form_S());
+  }
   if (o->diagnostics && out.st) PHDE_TRY(dorth_error(c, orth, fp32, s, &out.st->dorth_err));
   auto t2 = clk::now();
   Lap lap;

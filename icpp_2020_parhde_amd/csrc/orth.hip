@@ -19,6 +19,7 @@
 
 #include "phde_common.h"
 #include "orth.h"
+#include "spmm.h"
 
 namespace phde {
 
@@ -479,13 +480,6 @@ constexpr int kGcThreads = 256;  // four waves per block of the Gram pass
 constexpr int kGcTiles = 10;     // 16 x 16 tiles of the upper triangle of a 64 x 64 matrix
 constexpr int kGcPart = 2 * kGcTiles * 256;  // partial sums per wave: two matrices' tiles, 4 per lane
 
-// Column pointers, scales and exact sums of squares of the candidates,
-// passed by value.
-struct GramCols {
-  const int32_t *col[kGramMax];
-  double scale[kGramMax];
-  double sumsq[kGramMax];
-};
 
 // Upper-triangle tile number t < 10 -> (p, q), p <= q < 4.
 __device__ __forceinline__ void gram_tile(int t, int &p, int &q) {
@@ -508,11 +502,13 @@ __device__ __forceinline__ void gram_tile(int t, int &p, int &q) {
 // steps (its column segments stay in cache across steps), kGcSteps at a
 // time, the next group's loads issued before the current group's products;
 // it writes its sums to part[wave][tile][lane][4].  part_max[block] = the
-// largest distance the block read.
+// largest distance the block read.  With pack: the row's 2-bit residues
+// d mod 4 of the 64 columns, word p holding columns 16 p .. 16 p + 15 (two
+// bits each, gathered across the 16 lanes of the row).
 constexpr int kGcSteps = 4;
 typedef double gc_v4d __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(kGcThreads) void gram_tiles_kernel(GramCols cols, int N, const double *deg, int64_t n,
-                                                               double *part, int *part_max) {
+                                                               double *part, int *part_max, uint4 *pack) {
   __shared__ int smax[kGcThreads / kWave];
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const int kr = lane / 16, cl = lane % 16;
@@ -559,12 +555,21 @@ __global__ __launch_bounds__(kGcThreads) void gram_tiles_kernel(GramCols cols, i
     for (int u = 0; u < kGcSteps; u++) {
       const bool rok = g0 + u < s1 && (g0 + u) * 4 + kr < n;
       double a[4], ad[4];
+      unsigned res[4];
 #pragma unroll
       for (int p = 0; p < 4; p++) {
         const int v = !rok ? 0 : keep[p] == 1 ? 1 : keep[p] == 2 ? vc[u][p] : 0;
         dmax = max(dmax, v);
         a[p] = (double)v;
         ad[p] = a[p] * dc[u];  // integer product below 2^53: exact
+        res[p] = (unsigned)(v & 3) << (2 * cl);
+      }
+      if (pack) {
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1)
+#pragma unroll
+          for (int p = 0; p < 4; p++) res[p] |= __shfl_xor(res[p], off);
+        if (cl == 0 && rok) pack[(g0 + u) * 4 + kr] = make_uint4(res[0], res[1], res[2], res[3]);
       }
 #pragma unroll
       for (int t = 0; t < kGcTiles; t++) {
@@ -746,14 +751,232 @@ __global__ __launch_bounds__(kWave) void gram_cgs_kernel(const double *G, const 
   }
 }
 
+// ------------------------------------------------------------ Gram tail
+// S^T L S without S.  L = D - A and the graph is symmetric, so
+// C^T L C = 1/2 sum over directed edges (u, v) of delta delta^T with
+// delta = C_u - C_v.  BFS levels differ by at most one across an edge, so
+// every entry of delta is -1, 0 or 1 and follows from the levels mod 4:
+// delta_c = ((d_c(u) - d_c(v)) mod 4) read as 0, 1 or -1 (3); a residue
+// difference of 2 cannot come from BFS levels and flags the columns as
+// unusable here.  The sum runs on the int8 matrix cores with exact int32
+// accumulation; K = M^T (C^T L C) M and the other s x s products of the
+// coefficient vectors are then formed in double-double arithmetic (the
+// coefficient vectors of nearly dependent columns are large and of mixed
+// sign, the products small).
+
+// Per 2-bit field: (x - y) mod 4 (fields do not borrow from each other).
+__device__ __forceinline__ unsigned field_sub(unsigned x, unsigned y) {
+  constexpr unsigned H = 0xAAAAAAAAu;
+  return ((x | H) - (y & ~H)) ^ ((x ^ ~y) & H);
+}
+// Fields equal to 2 (high bit set, low bit clear).
+__device__ __forceinline__ unsigned field_two(unsigned d) { return d & ~(d << 1) & 0xAAAAAAAAu; }
+// A field 0, 1, 3 as the int8 0, 1, -1 (0x00, 0x01, 0xff).
+__device__ __forceinline__ unsigned field_i8(unsigned f) {
+  const unsigned lo = f & 1u, hi = (f >> 1) & 1u;
+  return lo | ((lo & hi) * 0xfeu);
+}
+
+constexpr int kLgThreads = 256;
+constexpr int kLgPart = 3 * 1024;  // per wave: three 32 x 32 int32 tiles
+typedef int lg_v16i __attribute__((ext_vector_type(16)));
+
+// Per-wave partial sums of delta delta^T over a contiguous range of
+// directed edges, 64 at a time: lane l forms the residue differences of
+// edge l (gathering the two endpoints' packed residues) into LDS, then for
+// each 16-edge group lane l (column c = l % 32, edges 8 (l / 32) .. + 7)
+// packs delta_c and delta_{c+32} of its 8 edges as int8 and the wave
+// issues v_mfma_i32_32x32x16_i8 for the tiles (0,0), (0,1), (1,1) of the
+// 64 x 64 result (lane l supplies A[l%32][8(l/32) + b] and B[8(l/32) + b]
+// [l%32]; its output v is row 8(v/4) + 4(l/32) + v%4, column l%32).
+// part[wave][tile][lane][16]; *bad |= 1 when a residue difference is 2.
+__global__ __launch_bounds__(kLgThreads) void lap_gram_kernel(const uint32_t *esrc, const uint32_t *adj, int64_t m,
+                                                             const uint4 *pack, int *part, int *bad) {
+  __shared__ uint4 sd[kLgThreads / kWave][kWave];
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int h = lane / 32, c = lane % 32;
+  const int shift = 2 * (c % 16), wsel = c / 16;
+  lg_v16i a00 = {}, a01 = {}, a11 = {};
+  unsigned badl = 0;
+  const int64_t gw = (int64_t)blockIdx.x * (kLgThreads / kWave) + wave;
+  const int64_t nw = (int64_t)gridDim.x * (kLgThreads / kWave);
+  const int64_t e0 = m * gw / nw, e1 = m * (gw + 1) / nw;
+  const unsigned *sw = (const unsigned *)&sd[wave][0];
+  for (int64_t eb = e0; eb < e1; eb += kWave) {
+    const int64_t e = eb + lane;
+    uint4 d = make_uint4(0u, 0u, 0u, 0u);
+    if (e < e1) {
+      const uint4 pu = pack[esrc[e]], pv = pack[adj[e]];
+      d = make_uint4(field_sub(pu.x, pv.x), field_sub(pu.y, pv.y), field_sub(pu.z, pv.z), field_sub(pu.w, pv.w));
+      badl |= field_two(d.x) | field_two(d.y) | field_two(d.z) | field_two(d.w);
+    }
+    sd[wave][lane] = d;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      long x0 = 0, x1 = 0;
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const int k = 16 * g + 8 * h + b;
+        const unsigned w0 = sw[4 * k + wsel], w1 = sw[4 * k + 2 + wsel];
+        x0 |= (long)field_i8((w0 >> shift) & 3u) << (8 * b);
+        x1 |= (long)field_i8((w1 >> shift) & 3u) << (8 * b);
+      }
+      a00 = __builtin_amdgcn_mfma_i32_32x32x16_i8(x0, x0, a00, 0, 0, 0);
+      a01 = __builtin_amdgcn_mfma_i32_32x32x16_i8(x0, x1, a01, 0, 0, 0);
+      a11 = __builtin_amdgcn_mfma_i32_32x32x16_i8(x1, x1, a11, 0, 0, 0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  int *o = part + gw * kLgPart + lane * 16;
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    o[v] = a00[v];
+    o[1024 + v] = a01[v];
+    o[2048 + v] = a11[v];
+  }
+  if (badl) atomicOr(bad, 1);
+}
+
+// KC[i][j] (64 x 64, both triangles) = 1/2 the sum of the waves' tiles, one
+// wave per entry, in 64-bit integers (exact).
+__global__ void lap_sum_kernel(const int *part, int nparts, double *KC) {
+  const int lane = threadIdx.x % kWave;
+  const int e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (e >= kLgPart) return;
+  long long s = 0;
+  for (int b = lane; b < nparts; b += kWave) s += part[(int64_t)b * kLgPart + e];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane != 0) return;
+  const int t = e / 1024, l = (e / 16) % kWave, v = e % 16;
+  const int bi = t == 2 ? 1 : 0, bj = t == 0 ? 0 : 1;
+  const int i = 32 * bi + 8 * (v / 4) + 4 * (l / 32) + v % 4, j = 32 * bj + l % 32;
+  const double val = (double)(s / 2);
+  KC[i * 64 + j] = val;
+  KC[j * 64 + i] = val;
+}
+
+// R = M^T X M (s x s, both triangles, R[a][b] at a * s + b) for a symmetric
+// X (64 x 64 in LDS order [e][f]) and the coefficient vectors M ([k][64]),
+// both steps in double-double: T = X M (rows e, column b, hi and lo kept),
+// then R = M^T T.  With ones: also R1[b] = (X M)[0][b] (the D-products of
+// the ones column of C with the columns).  One block of kGtThreads.
+constexpr int kGtThreads = 256;
+__device__ void gram_mxm(const double *X, const double *M, int s, double *Th, double *Tl, double *R, double *R1) {
+#pragma clang fp contract(off)
+  for (int q = threadIdx.x; q < 64 * s; q += kGtThreads) {
+    const int e = q / s, b = q % s;
+    double hi = 0, lo = 0;
+    for (int f = 0; f < 64; f++) dd_fma(hi, lo, X[e * 64 + f], M[b * 64 + f]);
+    const double t = hi + lo;
+    Th[e * 64 + b] = t;
+    Tl[e * 64 + b] = lo - (t - hi);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < s * s; q += kGtThreads) {
+    const int a = q / s, b = q % s;
+    if (a > b) continue;
+    double hi = 0, lo = 0;
+    for (int e = 0; e < 64; e++) {
+      const double mv = M[a * 64 + e];
+      dd_fma(hi, lo, mv, Th[e * 64 + b]);
+      lo += mv * Tl[e * 64 + b];
+    }
+    const double r = hi + lo;
+    R[a * s + b] = r;
+    R[b * s + a] = r;
+  }
+  if (R1)
+    for (int b = threadIdx.x; b < s; b += kGtThreads) R1[b] = Th[b];
+  __syncthreads();
+}
+
+// K = M^T KC M; with B: B = M^T G_D M, and with Bf ((s+1) x (s+1)) also
+// the D-Gram of [u0 1, S] (B bordered by the ones column).  G_D is W x W.
+__global__ __launch_bounds__(kGtThreads) void gram_tail_kernel(const double *KC, const double *G, int W, const double *Mg,
+                                                              int s, double u0, double *K, double *B, double *Bf) {
+  extern __shared__ double tsm[];
+  double *X = tsm, *M = X + 64 * 64, *Th = M + kGramMax * 64, *Tl = Th + 64 * 64, *R1 = Tl + 64 * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += kGtThreads) X[e] = KC[e];
+  for (int e = threadIdx.x; e < s * 64; e += kGtThreads) M[e] = Mg[e];
+  __syncthreads();
+  gram_mxm(X, M, s, Th, Tl, K, nullptr);
+  if (!B) return;
+  for (int e = threadIdx.x; e < 64 * 64; e += kGtThreads) {
+    const int i = e / 64, j = e % 64;
+    X[e] = i < W && j < W ? G[W * W + i * W + j] : 0.0;
+  }
+  __syncthreads();
+  gram_mxm(X, M, s, Th, Tl, B, R1);
+  if (!Bf) return;
+  for (int q = threadIdx.x; q < s * s; q += kGtThreads) Bf[(q / s + 1) * (s + 1) + q % s + 1] = B[q];
+  if (threadIdx.x == 0) Bf[0] = u0 * u0 * X[0];
+  for (int b = threadIdx.x; b < s; b += kGtThreads) {
+    Bf[b + 1] = u0 * R1[b];
+    Bf[(b + 1) * (s + 1)] = u0 * R1[b];
+  }
+}
+
+// W[c][e] = sum_k M[k][e] V[k + c * s] (the coordinates' coefficients on
+// the columns of C), in double-double, rounded once; lane e, one wave.
+__global__ void gram_w_kernel(const double *M, const double *V, int s, int nc, double *W) {
+#pragma clang fp contract(off)
+  const int e = threadIdx.x;
+  for (int c = 0; c < nc; c++) {
+    double hi = 0, lo = 0;
+    for (int k = 0; k < s; k++) dd_fma(hi, lo, M[k * 64 + e], V[k + c * s]);
+    W[c * 64 + e] = hi + lo;
+  }
+}
+
+// X[v][c] = sum_e C[v][e] W[c][e] (column-major n x nc), one row per
+// thread, with per-block partial sums of X^2 (part[c * gridDim.x + b]).
+// Column pointers past N repeat column N - 1 (W is zero there).
+template <int WT>
+__global__ __launch_bounds__(kOrthBlock) void gram_coords_kernel(GramCols cols, int64_t n, const double *__restrict__ W,
+                                                                int nc, double *X, double *part) {
+  __shared__ double sh[kOrthBlock / kWave][8];
+  double ss[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    double d[WT];
+    d[0] = 1.0;
+#pragma unroll
+    for (int e = 1; e < WT; e++) d[e] = (double)cols.col[e - 1][v];
+    for (int c = 0; c < nc; c++) {
+      const double *w = W + c * 64;
+      double x = 0;
+#pragma unroll
+      for (int e = 0; e < WT; e++) x = fma(w[e], d[e], x);
+      X[v + (int64_t)c * n] = x;
+      ss[c] = fma(x, x, ss[c]);
+    }
+  }
+  const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  for (int c = 0; c < nc; c++) {
+    double v = ss[c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) sh[wv][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < nc) {
+    double v = 0;
+    for (int q = 0; q < kOrthBlock / kWave; q++) v += sh[q][threadIdx.x];
+    part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = v;
+  }
+}
+
 // S[:, k] = C M_k for the accepted columns k < ncols, one row per thread,
 // the coefficients summed in order.  M_k is exactly zero past coefficient
 // last[k] (its own candidate), so the sum stops at the 8-wide group that
 // holds it (adding fma(0, d, s) = s changes nothing).  Column pointers past
 // N repeat column N - 1, whose coefficients there are zero.
-struct GramForm {
-  int last[kGramMax];
-};
 
 template <int WT>
 __global__ __launch_bounds__(kOrthBlock) void gram_form_kernel(GramCols cols, GramForm fm, int64_t n,
@@ -837,8 +1060,15 @@ void Orth::release() {
   if (gM) (void)hipFree(gM);
   if (gOut) (void)hipFree(gOut);
   if (h_gOut) (void)hipHostFree(h_gOut);
-  gPart = gG = gM = gOut = h_gOut = nullptr;
-  gMax = nullptr;
+  if (gPack) (void)hipFree(gPack);
+  if (gLapPart) (void)hipFree(gLapPart);
+  if (gBad) (void)hipFree(gBad);
+  if (gKC) (void)hipFree(gKC);
+  if (gW) (void)hipFree(gW);
+  gPart = gG = gM = gOut = h_gOut = gKC = gW = nullptr;
+  gMax = gLapPart = gBad = nullptr;
+  gPack = nullptr;
+  gram_pending = false;
   S = nullptr;
   a = part = den = coef = scal = h_scal = ones = nullptr;
   blkA = blkAcc = blkPart = blkCoef = nullptr;
@@ -1051,6 +1281,14 @@ static void gram_form(Orth &o, const GramCols &gc, const GramForm &fm, int ncols
   gram_form_kernel<WT><<<(unsigned)blocks, kOrthBlock, 0, o.ctx->stream>>>(gc, fm, o.n, o.gM, ncols, (double *)o.S);
 }
 
+template <int WT>
+static void gram_coords_launch(Orth &o, const GramCols &gc, int nb, int nc, double *X, double *part) {
+  gram_coords_kernel<WT><<<nb, kOrthBlock, 0, o.ctx->stream>>>(gc, o.n, o.gW, nc, X, part);
+}
+
+// Width of the unrolled coefficient loops: N + 1 columns rounded up to 8.
+static int gram_width(int N) { return (N + 1 + 7) / 8 * 8; }
+
 int Orth::gram_cgs(const int32_t *const *cols, const double *sumsq, int nc, int s_target, int &accepted,
                    std::vector<bool> &taken, std::vector<double> &norms, int &consumed) {
   if (nc < 1 || nc > kGramMax || accepted != 0 || fp32) {
@@ -1068,6 +1306,7 @@ int Orth::gram_cgs(const int32_t *const *cols, const double *sumsq, int nc, int 
     PHDE_HIP(hipMalloc(&gM, sizeof(double) * kGramMax * 64));
     PHDE_HIP(hipMalloc(&gOut, sizeof(double) * (2 * kGramMax + 2)));
     PHDE_HIP(hipHostMalloc(&h_gOut, sizeof(double) * (2 * kGramMax + 2), hipHostMallocDefault));
+    PHDE_HIP(hipMalloc(&gPack, sizeof(uint4) * (size_t)n));
     PHDE_HIP(hipFuncSetAttribute((const void *)gram_cgs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)(sizeof(double) * (2 * 64 * 64 + kGramMax * 64 + 64 * 64 + 64))));
   }
@@ -1077,7 +1316,7 @@ int Orth::gram_cgs(const int32_t *const *cols, const double *sumsq, int nc, int 
     gc.scale[b] = b < nc ? 1 / std::sqrt(sumsq[b]) : 0.0;
     gc.sumsq[b] = b < nc ? sumsq[b] : 0.0;
   }
-  gram_tiles_kernel<<<gBlocks, kGcThreads, 0, st>>>(gc, nc, deg_, n, gPart, gMax);
+  gram_tiles_kernel<<<gBlocks, kGcThreads, 0, st>>>(gc, nc, deg_, n, gPart, gMax, (uint4 *)gPack);
   gram_sum_kernel<<<kGcPart / 4, 4 * kWave, 0, st>>>(gPart, gMax, gBlocks, W, gG, gMax + gBlocks);
   const size_t cgs_lds = sizeof(double) * (2 * W * 64 + kGramMax * 64 + 64 * 64 + 64);
   gram_cgs_kernel<<<1, kWave, cgs_lds, st>>>(gG, gMax + gBlocks, nc, W, gc, u0, s_target, gM, den, gOut);
@@ -1092,24 +1331,119 @@ int Orth::gram_cgs(const int32_t *const *cols, const double *sumsq, int nc, int 
     norms[b] = h_gOut[b];
     taken[b] = h_gOut[nc + b] != 0.0;
   }
-  if (acc > 0) {
-    GramForm fm{};
-    for (int b = 0, k = 0; b < consumed; b++)
-      if (taken[b]) fm.last[k++] = b + 1;
-    const int WT = (nc + 1 + 7) / 8 * 8;
-    switch (WT) {
-      case 8: gram_form<8>(*this, gc, fm, acc); break;
-      case 16: gram_form<16>(*this, gc, fm, acc); break;
-      case 24: gram_form<24>(*this, gc, fm, acc); break;
-      case 32: gram_form<32>(*this, gc, fm, acc); break;
-      case 40: gram_form<40>(*this, gc, fm, acc); break;
-      case 48: gram_form<48>(*this, gc, fm, acc); break;
-      case 56: gram_form<56>(*this, gc, fm, acc); break;
-      default: gram_form<64>(*this, gc, fm, acc); break;
-    }
-    PHDE_HIP(hipGetLastError());
-  }
+  gCols = gc;
+  gForm = GramForm{};
+  for (int b = 0, k = 0; b < consumed; b++)
+    if (taken[b]) gForm.last[k++] = b + 1;
+  gN = nc;
+  gAcc = acc;
   accepted = acc;
+  // every column accepted: S stays in coefficient form for the tail;
+  // otherwise the vector CGS continues on S
+  gram_pending = acc > 0;
+  if (acc < s_target) return form_S();
+  return PHDE_OK;
+}
+
+int Orth::form_S() {
+  if (!gram_pending) return PHDE_OK;
+  gram_pending = false;
+  switch (gram_width(gN)) {
+    case 8: gram_form<8>(*this, gCols, gForm, gAcc); break;
+    case 16: gram_form<16>(*this, gCols, gForm, gAcc); break;
+    case 24: gram_form<24>(*this, gCols, gForm, gAcc); break;
+    case 32: gram_form<32>(*this, gCols, gForm, gAcc); break;
+    case 40: gram_form<40>(*this, gCols, gForm, gAcc); break;
+    case 48: gram_form<48>(*this, gCols, gForm, gAcc); break;
+    case 56: gram_form<56>(*this, gCols, gForm, gAcc); break;
+    default: gram_form<64>(*this, gCols, gForm, gAcc); break;
+  }
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
+// Source vertex of every directed edge (once per context).
+__global__ void edge_source_kernel(const uint32_t *rowptr, int64_t n, uint32_t *esrc) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < n; u += stride)
+    for (uint32_t e = rowptr[u]; e < rowptr[u + 1]; e++) esrc[e] = (uint32_t)u;
+}
+
+int Orth::gram_lap(double *K_dev, double *B_dev, double *Bfull_host, bool &ok) {
+  ok = false;
+  if (!gram_pending) {
+    set_error("internal: Gram tail without Gram columns");
+    return PHDE_ERR_ARG;
+  }
+  hipStream_t st = ctx->stream;
+  const int64_t m = ctx->g.m;
+  if (!ctx->d_esrc) {
+    PHDE_HIP(hipMalloc(&ctx->d_esrc, sizeof(uint32_t) * (size_t)std::max<int64_t>(m, 1)));
+    edge_source_kernel<<<grid_for(n, 16 * std::max(ctx->num_cus, 1)), kOrthBlock, 0, st>>>(ctx->g.rowptr, n,
+                                                                                           ctx->d_esrc);
+  }
+  if (!gLapPart) {
+    gLapBlocks = (int)std::max<int64_t>(1, std::min<int64_t>((m + 1023) / 1024, 2 * std::max(ctx->num_cus, 1)));
+    PHDE_HIP(hipMalloc(&gLapPart, sizeof(int) * (size_t)gLapBlocks * (kLgThreads / kWave) * kLgPart));
+    PHDE_HIP(hipMalloc(&gBad, sizeof(int) * 4));
+    PHDE_HIP(hipMalloc(&gKC, sizeof(double) * 64 * 64));
+    PHDE_HIP(hipMalloc(&gW, sizeof(double) * 8 * 64));
+    PHDE_HIP(hipFuncSetAttribute((const void *)gram_tail_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(double) * (3 * 64 * 64 + kGramMax * 64 + 64))));
+  }
+  PHDE_TRY_RC(fill_i32(ctx, gBad, 1, 0));
+  lap_gram_kernel<<<gLapBlocks, kLgThreads, 0, st>>>(ctx->d_esrc, ctx->g.adj, m, (const uint4 *)gPack, gLapPart, gBad);
+  lap_sum_kernel<<<kLgPart / 4, 4 * kWave, 0, st>>>(gLapPart, gLapBlocks * (kLgThreads / kWave), gKC);
+  PHDE_HIP(hipGetLastError());
+  int bad = 0;
+  PHDE_HIP(hipMemcpyAsync(&bad, gBad, sizeof(int), hipMemcpyDeviceToHost, st));
+  PHDE_HIP(hipStreamSynchronize(st));
+  if (bad) return PHDE_OK;
+  const int s_ = gAcc;
+  const int W = (gN + 1 + 3) / 4 * 4;
+  DBuf<double> Bs, Bf;
+  double *Bp = B_dev;
+  if (Bfull_host) {
+    PHDE_TRY_RC(Bf.alloc((size_t)(s_ + 1) * (s_ + 1)));
+    if (!Bp) {
+      PHDE_TRY_RC(Bs.alloc((size_t)s_ * s_));
+      Bp = Bs.p;
+    }
+  }
+  const size_t lds = sizeof(double) * (3 * 64 * 64 + kGramMax * 64 + 64);
+  gram_tail_kernel<<<1, kGtThreads, lds, st>>>(gKC, gG, W, gM, s_, u0, K_dev, Bp, Bf.p);
+  PHDE_HIP(hipGetLastError());
+  if (Bfull_host) {
+    PHDE_HIP(hipMemcpyAsync(Bfull_host, Bf.p, sizeof(double) * (s_ + 1) * (s_ + 1), hipMemcpyDeviceToHost, st));
+    PHDE_HIP(hipStreamSynchronize(st));
+  }
+  ok = true;
+  return PHDE_OK;
+}
+
+int Orth::gram_coords(const double *V_dev, int nc, double *X_dev, double *part, double *inv) {
+  if (!gram_pending || nc < 1 || nc > 8) {
+    set_error("internal: Gram coordinates arguments");
+    return PHDE_ERR_ARG;
+  }
+  hipStream_t st = ctx->stream;
+  hipEvent_t ev;
+  ctx->timer.begin(st, PHDE_KERNEL_COORDS, &ev);
+  gram_w_kernel<<<1, kWave, 0, st>>>(gM, V_dev, gAcc, nc, gW);
+  const int nb = grid_for(n, 4 * std::max(ctx->num_cus, 1));
+  switch (gram_width(gN)) {
+    case 8: gram_coords_launch<8>(*this, gCols, nb, nc, X_dev, part); break;
+    case 16: gram_coords_launch<16>(*this, gCols, nb, nc, X_dev, part); break;
+    case 24: gram_coords_launch<24>(*this, gCols, nb, nc, X_dev, part); break;
+    case 32: gram_coords_launch<32>(*this, gCols, nb, nc, X_dev, part); break;
+    case 40: gram_coords_launch<40>(*this, gCols, nb, nc, X_dev, part); break;
+    case 48: gram_coords_launch<48>(*this, gCols, nb, nc, X_dev, part); break;
+    case 56: gram_coords_launch<56>(*this, gCols, nb, nc, X_dev, part); break;
+    default: gram_coords_launch<64>(*this, gCols, nb, nc, X_dev, part); break;
+  }
+  PHDE_HIP(hipGetLastError());
+  PHDE_TRY_RC(coords_normalise(ctx, X_dev, nc, part, nb, inv));
+  ctx->timer.end(st, PHDE_KERNEL_COORDS, ev);
   return PHDE_OK;
 }
 

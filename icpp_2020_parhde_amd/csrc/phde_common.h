@@ -157,6 +157,7 @@ struct phde_ctx {
   float *lp_score = nullptr, *lp_target = nullptr;
   phde_mpnn *net = nullptr;  // borrowed (phde_set_pivot_net)
   phde::PartGraph *pg = nullptr;  // partitioned BFS (pbfs.hip); null when not in use
+  uint32_t *d_esrc = nullptr;     // source vertex of each directed edge (built on first use)
   phde::Timer timer;
 };
 

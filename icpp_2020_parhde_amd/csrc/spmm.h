@@ -24,6 +24,7 @@ struct Lap {
 int spmm_hubs(phde_ctx *c);
 int coords_compute(phde_ctx *c, const void *Scol, bool fp32, int s, int nc, const double *V_dev,
                    double *X_dev, double *part, double *inv);
+int coords_normalise(phde_ctx *c, double *X_dev, int nc, const double *part, int nblk, double *inv);
 int hall_energy_dev(phde_ctx *c, const double *Y_dev, int nc, double *part_dev, double *h_out2);
 
 }  // namespace phde

@@ -525,6 +525,16 @@ int coords_compute(phde_ctx *c, const void *Scol, bool fp32, int s, int nc, cons
   return PHDE_OK;
 }
 
+// X[:, k] *= 1 / ||X[:, k]||_2 from per-block partial sums of squares
+// (part[k * nblk + b]), as coords_compute finishes.
+int coords_normalise(phde_ctx *c, double *X_dev, int nc, const double *part, int nblk, double *inv) {
+  const int64_t n = c->g.n;
+  coords_norm_kernel<<<1, 64, 0, c->stream>>>(part, nblk, nc, inv);
+  scale_cols_kernel<<<blocks_for(c, (n * nc + kBlk - 1) / kBlk, 4), kBlk, 0, c->stream>>>(X_dev, n, nc, inv);
+  PHDE_HIP(hipGetLastError());
+  return PHDE_OK;
+}
+
 // S^T D S for the generalised eigenproblem: the row-tiled Gram with the
 // rows of the left operand scaled by deg.
 int weighted_gram(phde_ctx *c, const void *S_dev, bool fp32, int s, double *out_dev) {

@@ -7,7 +7,10 @@ time path, on graphs with and without discarded candidates:
 - the Gram-space CGS (the default for FP64 ParHDE: exact integer Gram
   matrices of the distance columns, the projections on coefficient
   vectors): the same pivots and discards, coordinates within the FP64
-  tolerance and every candidate's norm within 1e-9 relative.
+  tolerance and every candidate's norm within 1e-9 relative;
+- after the Gram-space CGS, S^T L S from the exact C^T L C and the
+  coordinates as C (M V) (the default) against forming S and taking the
+  SpMM path (PHDE_GRAM_TAIL=0): coordinates within the FP64 tolerance.
 
 The switches are read once per process, so every variant runs in a child
 process."""
@@ -80,6 +83,7 @@ def test_batched_cgs_against_one_at_a_time(gpu, case, tmp_path):
     one = _run(case, tmp_path, "one", PHDE_CGS_BLOCK="0")
     blk = _run(case, tmp_path, "blk", PHDE_CGS_GRAM="0")
     gram = _run(case, tmp_path, "gram")
+    gram_s = _run(case, tmp_path, "gram_s", PHDE_GRAM_TAIL="0")
     assert one["ngram"] == 0 and blk["ngram"] == 0
     # the Gram path took the first min(s, candidates) candidates
     assert gram["ngram"] == min(CASES[case][2], len(gram["normdist"]))
@@ -91,7 +95,8 @@ def test_batched_cgs_against_one_at_a_time(gpu, case, tmp_path):
     kept = b >= 0.001
     assert np.all(a[~kept] < 0.001)
     assert np.max(np.abs(a[kept] / b[kept] - 1)) < 1e-9
-    X, Y = gram["X"], one["X"]
-    for k in range(Y.shape[1]):
-        x = X[:, k] if np.dot(X[:, k], Y[:, k]) >= 0 else -X[:, k]
-        assert np.max(np.abs(x - Y[:, k])) / np.max(np.abs(Y[:, k])) < 1e-8, k
+    assert gram_s["pivots"] == gram["pivots"] and gram_s["normdist"] == gram["normdist"]
+    for X, Y in ((gram["X"], one["X"]), (gram_s["X"], one["X"]), (gram["X"], gram_s["X"])):
+        for k in range(Y.shape[1]):
+            x = X[:, k] if np.dot(X[:, k], Y[:, k]) >= 0 else -X[:, k]
+            assert np.max(np.abs(x - Y[:, k])) / np.max(np.abs(Y[:, k])) < 1e-8, k
