@@ -478,15 +478,15 @@ void glibc_rand_sequence(unsigned seed, int count, int32_t *out) {
 
 // ---------------------------------------------------------------- no allocation
 // phde_power_eig_noalloc runs the power method above without allocating:
-// its scratch — the dim*dim start values and, when the power method's
-// matrices do not all fit in LDS, its global T (dim*dim) — is this
-// file-scope device array (64 KiB), which every device the library runs on
-// holds until the process ends.  Host code never passes `scratch` itself
+// its scratch — the dim*dim start values (the power method keeps its
+// matrices in registers and LDS) — is this file-scope device array
+// (32 KiB), which every device the library runs on holds until the process
+// ends.  Host code never passes `scratch` itself
 // (in host code the name is a host-side stand-in, not a device address):
 // each device's address comes from hipGetSymbolAddress, once per device.
 constexpr int kNoAllocMaxDim = 64;
 constexpr int kMaxDevices = 64;
-static __device__ __align__(256) double scratch[2 * kNoAllocMaxDim * kNoAllocMaxDim];
+static __device__ __align__(256) double scratch[kNoAllocMaxDim * kNoAllocMaxDim];
 
 static std::mutex g_scratch_mu;
 static double *g_scratch_dev[kMaxDevices] = {};

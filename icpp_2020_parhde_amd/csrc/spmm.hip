@@ -305,13 +305,16 @@ __global__ __launch_bounds__(kBlk) void coords_kernel(const T *S, const double *
   }
 }
 
-// inv[k] = 1 / sqrt(sum_b part[k][b]) (getCoordinates, DrawingBase.cpp:200).
+// inv[k] = 1 / sqrt(sum_b part[k][b]) (getCoordinates, DrawingBase.cpp:200);
+// wave k sums column k's partials (lane-strided, then a fixed shuffle tree).
 __global__ void coords_norm_kernel(const double *part, int nblk, int c, double *inv) {
-  const int k = threadIdx.x;
+  const int k = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   if (k >= c) return;
   double s = 0;
-  for (int b = 0; b < nblk; b++) s += part[(int64_t)k * nblk + b];
-  inv[k] = 1 / sqrt(s);
+  for (int b = lane; b < nblk; b += kWave) s += part[(int64_t)k * nblk + b];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (lane == 0) inv[k] = 1 / sqrt(s);
 }
 
 __global__ void scale_cols_kernel(double *X, int64_t n, int c, const double *inv) {
@@ -517,7 +520,7 @@ int coords_compute(phde_ctx *c, const void *Scol, bool fp32, int s, int nc, cons
   else
     coords_kernel<double><<<nb, kBlk, 0, c->stream>>>((const double *)Scol, V_dev, n, s, nc, X_dev,
                                                       part);
-  coords_norm_kernel<<<1, 64, 0, c->stream>>>(part, nb, nc, inv);
+  coords_norm_kernel<<<1, 8 * kWave, 0, c->stream>>>(part, nb, nc, inv);
   scale_cols_kernel<<<blocks_for(c, (n * nc + kBlk - 1) / kBlk, 4), kBlk, 0, c->stream>>>(X_dev, n,
                                                                                         nc, inv);
   c->timer.end(c->stream, PHDE_KERNEL_COORDS, ev);
@@ -529,7 +532,7 @@ int coords_compute(phde_ctx *c, const void *Scol, bool fp32, int s, int nc, cons
 // (part[k * nblk + b]), as coords_compute finishes.
 int coords_normalise(phde_ctx *c, double *X_dev, int nc, const double *part, int nblk, double *inv) {
   const int64_t n = c->g.n;
-  coords_norm_kernel<<<1, 64, 0, c->stream>>>(part, nblk, nc, inv);
+  coords_norm_kernel<<<1, 8 * kWave, 0, c->stream>>>(part, nblk, nc, inv);
   scale_cols_kernel<<<blocks_for(c, (n * nc + kBlk - 1) / kBlk, 4), kBlk, 0, c->stream>>>(X_dev, n, nc, inv);
   PHDE_HIP(hipGetLastError());
   return PHDE_OK;

@@ -192,7 +192,7 @@ int phde_power_eig(phde_ctx *ctx, double *K, double *evals, int dim, int32_t *it
  * nothing.  No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc or
- * hipFree: its scratch (start values and a work matrix, 64 KiB) is a
+ * hipFree: its scratch (the start values, 32 KiB) is a
  * static device array of the library, one per GPU, held until the process
  * ends, so calls that overlap on one GPU race on it.  K_dev (dim*dim,
  * column-major) is overwritten with the eigenvector matrix, evals_dev (dim)
