@@ -170,9 +170,86 @@ static int ctx_renumber(phde_ctx *c) {
   return spmm_hubs(c);
 }
 
+// ---------------------------------------------------------------- temporaries
+static thread_local phde_ctx *t_ctx = nullptr;
+CtxScope::CtxScope(phde_ctx *c) : prev(t_ctx) { t_ctx = c; }
+CtxScope::~CtxScope() { t_ctx = prev; }
+// Bytes of temporaries a context keeps for its later calls, per kind.
+constexpr size_t kTmpCacheCap = (size_t)32 << 30;
+
+static void cache_drop(std::multimap<size_t, void *> &m, size_t &bytes, bool host) {
+  for (auto &kv : m) (void)(host ? hipHostFree(kv.second) : hipFree(kv.second));
+  m.clear();
+  bytes = 0;
+}
+
+void tmp_release(phde_ctx *c) {
+  cache_drop(c->dev_cache, c->dev_cached, false);
+  cache_drop(c->host_cache, c->host_cached, true);
+}
+
+static int cache_take(std::multimap<size_t, void *> &m, size_t &cached, size_t bytes, void **p) {
+  auto it = m.find(bytes);
+  if (it == m.end()) return 0;
+  *p = it->second;
+  m.erase(it);
+  cached -= bytes;
+  return 1;
+}
+
+int tmp_malloc(void **p, size_t bytes) {
+  phde_ctx *c = t_ctx;
+  if (c && cache_take(c->dev_cache, c->dev_cached, bytes, p)) return PHDE_OK;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && c && !c->dev_cache.empty()) {
+    (void)hipGetLastError();
+    cache_drop(c->dev_cache, c->dev_cached, false);  // the cache held the memory: give it back, try again
+    e = hipMalloc(p, bytes);
+  }
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error("device allocation of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    return PHDE_ERR_NOMEM;
+  }
+  return PHDE_OK;
+}
+
+void tmp_free(void *p, size_t bytes) {
+  phde_ctx *c = t_ctx;
+  if (c && c->dev_cached + bytes <= kTmpCacheCap) {
+    c->dev_cache.emplace(bytes, p);
+    c->dev_cached += bytes;
+    return;
+  }
+  (void)hipFree(p);
+}
+
+int tmp_host_malloc(void **p, size_t bytes) {
+  phde_ctx *c = t_ctx;
+  if (c && cache_take(c->host_cache, c->host_cached, bytes, p)) return PHDE_OK;
+  const hipError_t e = hipHostMalloc(p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error("pinned allocation of " + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+    return PHDE_ERR_NOMEM;
+  }
+  return PHDE_OK;
+}
+
+void tmp_host_free(void *p, size_t bytes) {
+  phde_ctx *c = t_ctx;
+  if (c && c->host_cached + bytes <= kTmpCacheCap) {
+    c->host_cache.emplace(bytes, p);
+    c->host_cached += bytes;
+    return;
+  }
+  (void)hipHostFree(p);
+}
+
 static void ctx_free(phde_ctx *c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  tmp_release(c);
   void *dev[] = {c->d_rowptr, c->d_adj, c->d_w, c->d_deg32, c->d_degw, c->d_dist, c->d_q[0],
                  c->d_q[1], c->d_hq[0], c->d_hq[1], c->d_bm[0], c->d_bm[1], c->d_slots, c->d_kc,
                  c->d_part};
@@ -1099,6 +1176,7 @@ int phde_num_boundary(const phde_ctx *c, int64_t *boundary) {
 }
 
 int phde_bfs_dev(phde_ctx *c, int64_t source, int alpha, int beta, int32_t *dist_dev) {
+  CtxScope scope_(c);
   if (!c || !dist_dev || source < 0 || source >= c->g.n) return PHDE_ERR_ARG;
   PHDE_TRY(bfs_run(c, to_internal(c, source), alpha, beta, c->d_dist, nullptr, false));
   if (renumbered(c)) {
@@ -1125,6 +1203,7 @@ static int columns_to_host(phde_ctx *c, const int32_t *D, int k, int32_t *host) 
 }
 
 int phde_bfs(phde_ctx *c, int64_t source, int alpha, int beta, int32_t *dist) {
+  CtxScope scope_(c);
   if (!c || !dist || source < 0 || source >= c->g.n) return PHDE_ERR_ARG;
   PHDE_TRY(bfs_run(c, to_internal(c, source), alpha, beta, c->d_dist, nullptr, false));
   PHDE_TRY(dist_finalize(c, c->d_dist, c->d_dist, kUnvisited, -1));
@@ -1132,6 +1211,7 @@ int phde_bfs(phde_ctx *c, int64_t source, int alpha, int beta, int32_t *dist) {
 }
 
 int phde_sssp(phde_ctx *c, int64_t source, int32_t delta, int32_t *dist) {
+  CtxScope scope_(c);
   if (!c || !dist || source < 0 || source >= c->g.n) return PHDE_ERR_ARG;
   if (!c->g.w) return PHDE_ERR_WEIGHTS;
   PHDE_TRY(bfs_setup(c));
@@ -1140,6 +1220,7 @@ int phde_sssp(phde_ctx *c, int64_t source, int32_t delta, int32_t *dist) {
 }
 
 int phde_msbfs(phde_ctx *c, const int64_t *sources, int k, int32_t *dist) {
+  CtxScope scope_(c);
   if (!c || !sources || !dist || k < 1 || k > 64) return PHDE_ERR_ARG;
   std::vector<int64_t> isrc(k);
   for (int i = 0; i < k; i++) {
@@ -1153,6 +1234,7 @@ int phde_msbfs(phde_ctx *c, const int64_t *sources, int k, int32_t *dist) {
 }
 
 int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats *stats) {
+  CtxScope scope_(c);
   if (!c || !opt || !coords) return PHDE_ERR_ARG;
   const phde_options &o = *opt;
   if (o.s < 1 || o.s > 64 || o.num_coord < 1 || o.num_coord > 8 || o.num_coord > o.s ||
@@ -1243,6 +1325,7 @@ int phde_layout(phde_ctx *c, const phde_options *opt, double *coords, phde_stats
 }
 
 int phde_kcenters(phde_ctx *c, int s, int64_t first, int64_t *sources, int32_t *min_dist) {
+  CtxScope scope_(c);
   if (!c || s < 1 || first < 0 || first >= c->g.n) return PHDE_ERR_ARG;
   const int64_t n = c->g.n;
   PHDE_TRY(bfs_setup(c));
@@ -1271,6 +1354,7 @@ int phde_kcenters(phde_ctx *c, int s, int64_t first, int64_t *sources, int32_t *
 }
 
 int phde_power_eig(phde_ctx *c, double *K, double *evals, int dim, int32_t *iters) {
+  CtxScope scope_(c);
   if (!c || !K || !evals || dim < 1 || dim > 64) return PHDE_ERR_ARG;
   DBuf<double> Kd, init, ev;
   DBuf<int> status;
@@ -1297,6 +1381,7 @@ int phde_power_eig(phde_ctx *c, double *K, double *evals, int dim, int32_t *iter
 
 int phde_sym_eig(phde_ctx *c, const double *A, const double *B, int dim, double *evals,
                  double *evecs) {
+  CtxScope scope_(c);
   if (!c || !A || !evals || !evecs || dim < 1 || dim > 64) return PHDE_ERR_ARG;
   DBuf<double> Ad, Bd, ev, V, work;
   DBuf<int> status;
@@ -1320,6 +1405,7 @@ int phde_sym_eig(phde_ctx *c, const double *A, const double *B, int dim, double 
 }
 
 int phde_hall_energy(phde_ctx *c, const double *Y, int nc, double *energy) {
+  CtxScope scope_(c);
   if (!c || !Y || !energy || nc < 1) return PHDE_ERR_ARG;
   DBuf<double> Yd, part;
   PHDE_TRY(Yd.alloc((size_t)c->g.n * nc));

@@ -1,5 +1,6 @@
 // orth.h — internal interface of the D-orthogonalisation stage (orth.hip).
 #pragma once
+#include <utility>
 #include <vector>
 
 #include "phde_common.h"
@@ -58,6 +59,11 @@ struct Orth {
   int gLapBlocks = 0;
   const double *deg_ = nullptr;
   double u0 = 0;
+
+  // device / pinned buffers of this orthogonalisation (context temporaries)
+  std::vector<std::pair<void *, size_t>> dev_bufs, host_bufs;
+  int dalloc(void **p, size_t bytes);
+  int halloc(void **p, size_t bytes);
 
   int setup(phde_ctx *c, int64_t n, int s, bool fp32);
   void release();

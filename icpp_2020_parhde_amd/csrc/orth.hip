@@ -1031,47 +1031,40 @@ int Orth::setup(phde_ctx *c, int64_t n_, int s_, bool fp32_) {
   fp32 = fp32_;
   nblk = grid_for(n, 2 * std::max(c->num_cus, 1) * 4);
   const size_t esz = fp32 ? sizeof(float) : sizeof(double);
-  PHDE_HIP(hipMalloc(&S, esz * (size_t)n * s));
-  PHDE_HIP(hipMalloc(&a, sizeof(double) * n));
-  PHDE_HIP(hipMalloc(&part, sizeof(double) * (size_t)nblk * (kOnesSlot + 2) * 2));
-  PHDE_HIP(hipMalloc(&den, sizeof(double) * (kOnesSlot + 1)));
-  PHDE_HIP(hipMalloc(&coef, sizeof(double) * (kOnesSlot + 1)));
-  PHDE_HIP(hipMalloc(&scal, sizeof(double) * 8));
-  PHDE_HIP(hipHostMalloc(&h_scal, sizeof(double) * 8, hipHostMallocDefault));
+  PHDE_TRY_RC(dalloc((void **)&S, esz * (size_t)n * s));
+  PHDE_TRY_RC(dalloc((void **)&a, sizeof(double) * n));
+  PHDE_TRY_RC(dalloc((void **)&part, sizeof(double) * (size_t)nblk * (kOnesSlot + 2) * 2));
+  PHDE_TRY_RC(dalloc((void **)&den, sizeof(double) * (kOnesSlot + 1)));
+  PHDE_TRY_RC(dalloc((void **)&coef, sizeof(double) * (kOnesSlot + 1)));
+  PHDE_TRY_RC(dalloc((void **)&scal, sizeof(double) * 8));
+  PHDE_TRY_RC(halloc((void **)&h_scal, sizeof(double) * 8));
   return PHDE_OK;
 }
 
 void Orth::release() {
-  if (S) (void)hipFree(S);
-  if (a) (void)hipFree(a);
-  if (part) (void)hipFree(part);
-  if (den) (void)hipFree(den);
-  if (coef) (void)hipFree(coef);
-  if (scal) (void)hipFree(scal);
-  if (ones) (void)hipFree(ones);
-  if (h_scal) (void)hipHostFree(h_scal);
-  if (blkA) (void)hipFree(blkA);
-  if (blkAcc) (void)hipFree(blkAcc);
-  if (blkPart) (void)hipFree(blkPart);
-  if (blkCoef) (void)hipFree(blkCoef);
-  if (gPart) (void)hipFree(gPart);
-  if (gMax) (void)hipFree(gMax);
-  if (gG) (void)hipFree(gG);
-  if (gM) (void)hipFree(gM);
-  if (gOut) (void)hipFree(gOut);
-  if (h_gOut) (void)hipHostFree(h_gOut);
-  if (gPack) (void)hipFree(gPack);
-  if (gLapPart) (void)hipFree(gLapPart);
-  if (gBad) (void)hipFree(gBad);
-  if (gKC) (void)hipFree(gKC);
-  if (gW) (void)hipFree(gW);
+  for (auto &b : dev_bufs) tmp_free(b.first, b.second);
+  for (auto &b : host_bufs) tmp_host_free(b.first, b.second);
+  dev_bufs.clear();
+  host_bufs.clear();
+  S = nullptr;
+  a = part = den = coef = scal = h_scal = ones = nullptr;
+  blkA = blkAcc = blkPart = blkCoef = nullptr;
   gPart = gG = gM = gOut = h_gOut = gKC = gW = nullptr;
   gMax = gLapPart = gBad = nullptr;
   gPack = nullptr;
   gram_pending = false;
-  S = nullptr;
-  a = part = den = coef = scal = h_scal = ones = nullptr;
-  blkA = blkAcc = blkPart = blkCoef = nullptr;
+}
+
+int Orth::dalloc(void **p, size_t bytes) {
+  PHDE_TRY_RC(tmp_malloc(p, bytes));
+  dev_bufs.emplace_back(*p, bytes);
+  return PHDE_OK;
+}
+
+int Orth::halloc(void **p, size_t bytes) {
+  PHDE_TRY_RC(tmp_host_malloc(p, bytes));
+  host_bufs.emplace_back(*p, bytes);
+  return PHDE_OK;
 }
 
 // den of the ones vector: sum_i u0*u0*deg_i (ParHDE.cpp:226).
@@ -1133,7 +1126,7 @@ static int mgs_pass_t(Orth &o, int ncols) {
 
 int Orth::mgs_pass(int ncols) {
   if (!ones) {
-    PHDE_HIP(hipMalloc(&ones, sizeof(double) * n));
+    PHDE_TRY_RC(dalloc((void **)&ones, sizeof(double) * n));
     int rc = fill_f64(ctx, ones, n, u0);
     if (rc) return rc;
   }
@@ -1264,10 +1257,10 @@ int Orth::cgs_block(const int32_t *const *cols, const double *scales, int nc, in
     return PHDE_ERR_ARG;
   }
   if (!blkA) {
-    PHDE_HIP(hipMalloc(&blkA, sizeof(double) * (size_t)n * kCgsBlock));
-    PHDE_HIP(hipMalloc(&blkAcc, sizeof(double) * (size_t)n * kCgsBlock));
-    PHDE_HIP(hipMalloc(&blkPart, sizeof(double) * (size_t)nblk * (kOnesSlot + 1) * kCgsBlock));
-    PHDE_HIP(hipMalloc(&blkCoef, sizeof(double) * (size_t)(kOnesSlot + 1) * kCgsBlock));
+    PHDE_TRY_RC(dalloc((void **)&blkA, sizeof(double) * (size_t)n * kCgsBlock));
+    PHDE_TRY_RC(dalloc((void **)&blkAcc, sizeof(double) * (size_t)n * kCgsBlock));
+    PHDE_TRY_RC(dalloc((void **)&blkPart, sizeof(double) * (size_t)nblk * (kOnesSlot + 1) * kCgsBlock));
+    PHDE_TRY_RC(dalloc((void **)&blkCoef, sizeof(double) * (size_t)(kOnesSlot + 1) * kCgsBlock));
   }
   // FP32 storage: CGS2 (two projections per candidate); FP64: one, bit for
   // bit as the candidate-by-candidate kernels
@@ -1300,13 +1293,13 @@ int Orth::gram_cgs(const int32_t *const *cols, const double *sumsq, int nc, int 
   if (!gPart) {
     const int64_t nsteps = (n + 3) / 4;
     gBlocks = (int)std::max<int64_t>(1, std::min<int64_t>((nsteps + 3) / 4, 2 * std::max(ctx->num_cus, 1)));
-    PHDE_HIP(hipMalloc(&gPart, sizeof(double) * (size_t)gBlocks * (kGcThreads / kWave) * kGcPart));
-    PHDE_HIP(hipMalloc(&gMax, sizeof(int) * (gBlocks + 1)));
-    PHDE_HIP(hipMalloc(&gG, sizeof(double) * 2 * 64 * 64));
-    PHDE_HIP(hipMalloc(&gM, sizeof(double) * kGramMax * 64));
-    PHDE_HIP(hipMalloc(&gOut, sizeof(double) * (2 * kGramMax + 2)));
-    PHDE_HIP(hipHostMalloc(&h_gOut, sizeof(double) * (2 * kGramMax + 2), hipHostMallocDefault));
-    PHDE_HIP(hipMalloc(&gPack, sizeof(uint4) * (size_t)n));
+    PHDE_TRY_RC(dalloc((void **)&gPart, sizeof(double) * (size_t)gBlocks * (kGcThreads / kWave) * kGcPart));
+    PHDE_TRY_RC(dalloc((void **)&gMax, sizeof(int) * (gBlocks + 1)));
+    PHDE_TRY_RC(dalloc((void **)&gG, sizeof(double) * 2 * 64 * 64));
+    PHDE_TRY_RC(dalloc((void **)&gM, sizeof(double) * kGramMax * 64));
+    PHDE_TRY_RC(dalloc((void **)&gOut, sizeof(double) * (2 * kGramMax + 2)));
+    PHDE_TRY_RC(halloc((void **)&h_gOut, sizeof(double) * (2 * kGramMax + 2)));
+    PHDE_TRY_RC(dalloc((void **)&gPack, sizeof(uint4) * (size_t)n));
     PHDE_HIP(hipFuncSetAttribute((const void *)gram_cgs_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)(sizeof(double) * (2 * 64 * 64 + kGramMax * 64 + 64 * 64 + 64))));
   }
@@ -1384,10 +1377,10 @@ int Orth::gram_lap(double *K_dev, double *B_dev, double *Bfull_host, bool &ok) {
   }
   if (!gLapPart) {
     gLapBlocks = (int)std::max<int64_t>(1, std::min<int64_t>((m + 1023) / 1024, 2 * std::max(ctx->num_cus, 1)));
-    PHDE_HIP(hipMalloc(&gLapPart, sizeof(int) * (size_t)gLapBlocks * (kLgThreads / kWave) * kLgPart));
-    PHDE_HIP(hipMalloc(&gBad, sizeof(int) * 4));
-    PHDE_HIP(hipMalloc(&gKC, sizeof(double) * 64 * 64));
-    PHDE_HIP(hipMalloc(&gW, sizeof(double) * 8 * 64));
+    PHDE_TRY_RC(dalloc((void **)&gLapPart, sizeof(int) * (size_t)gLapBlocks * (kLgThreads / kWave) * kLgPart));
+    PHDE_TRY_RC(dalloc((void **)&gBad, sizeof(int) * 4));
+    PHDE_TRY_RC(dalloc((void **)&gKC, sizeof(double) * 64 * 64));
+    PHDE_TRY_RC(dalloc((void **)&gW, sizeof(double) * 8 * 64));
     PHDE_HIP(hipFuncSetAttribute((const void *)gram_tail_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)(sizeof(double) * (3 * 64 * 64 + kGramMax * 64 + 64))));
   }

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "parhde.h"
@@ -158,6 +159,12 @@ struct phde_ctx {
   phde_mpnn *net = nullptr;  // borrowed (phde_set_pivot_net)
   phde::PartGraph *pg = nullptr;  // partitioned BFS (pbfs.hip); null when not in use
   uint32_t *d_esrc = nullptr;     // source vertex of each directed edge (built on first use)
+  // Temporaries of a call that ran on this context, freed back here for its
+  // later calls (size -> buffers): hipFree waits for the device and costs
+  // ≈250 us, hipMalloc up to a few hundred; every temporary of a context is
+  // used on its one stream, so stream order makes the reuse safe.
+  std::multimap<size_t, void *> dev_cache, host_cache;
+  size_t dev_cached = 0, host_cached = 0;
   phde::Timer timer;
 };
 
@@ -169,16 +176,35 @@ struct phde_ctx {
 
 namespace phde {
 
+// Device and pinned-host temporaries.  Inside a CtxScope (the API call
+// that runs on a context), a buffer freed goes to the context's cache and
+// an allocation of the same size takes it from there; outside, they are
+// plain hipMalloc / hipFree.
+int tmp_malloc(void **p, size_t bytes);
+void tmp_free(void *p, size_t bytes);
+int tmp_host_malloc(void **p, size_t bytes);
+void tmp_host_free(void *p, size_t bytes);
+// Makes c the calling thread's current context for the temporaries above
+// (restores the previous one at the end of the scope).
+struct CtxScope {
+  explicit CtxScope(phde_ctx *c);
+  ~CtxScope();
+  phde_ctx *prev;
+};
+// Frees a context's cached temporaries.
+void tmp_release(phde_ctx *c);
+
 // RAII device buffer for driver temporaries.
 template <typename T>
 struct DBuf {
   T *p = nullptr;
+  size_t bytes = 0;
   int alloc(size_t count) {
-    PHDE_HIP(hipMalloc(&p, sizeof(T) * (count > 0 ? count : 1)));
-    return PHDE_OK;
+    bytes = sizeof(T) * (count > 0 ? count : 1);
+    return tmp_malloc(reinterpret_cast<void **>(&p), bytes);
   }
   ~DBuf() {
-    if (p) (void)hipFree(p);
+    if (p) tmp_free(p, bytes);
   }
 };
 
