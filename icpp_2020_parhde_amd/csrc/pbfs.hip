@@ -1654,6 +1654,7 @@ struct ObfsHost {
   // largest LDS of step 3 (and 1), of a run of step 2 (with its edge
   // blob), and of dmat_kernel with 32-, 16- and 8-bit masks
   int64_t max_final = 0, max_orun = 0, max_mf = 0, max_dm[3] = {0, 0, 0}, max_nb = 0;
+  std::vector<int64_t> fin;  // every part's step-3 LDS (statistics)
   bool ok = true;
   void add_part(int64_t p, uint32_t v0, uint32_t nv, const std::vector<uint32_t> &verts,
                 const std::vector<uint16_t> &lrow, const std::vector<uint16_t> &ladj,
@@ -1698,6 +1699,7 @@ struct ObfsHost {
     coff.push_back((uint32_t)cblob.size());
     doff.push_back(doff.back() + (unsigned long long)(nb * drow(nb)));
     max_final = std::max(max_final, final_bytes(nv, ne, nb));
+    fin.push_back(final_bytes(nv, ne, nb));
     max_mf = std::max(max_mf, mfinal_bytes(nv, ne, nb));
     max_nb = std::max(max_nb, nb);
     const int64_t w[3] = {4, 2, 1};
@@ -1770,10 +1772,17 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   pg->olds = (size_t)align_up(std::max({oh.max_final, oh.max_orun, std::min<int64_t>(oh.max_mf, 152 * 1024)}), 256);
   for (const void *f : {(const void *)pg->oshape.fn, (const void *)pg->oshape.src_fn, (const void *)pg->oshape.mfin_fn})
     PHDE_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pg->olds));
-  if (pg->stats)
+  if (pg->stats) {
     std::fprintf(stderr, "[obfs] boundary %lld (largest part %lld), Dp %.1f MB in %.1f ms (%d-bit masks)%s\n",
                  (long long)pg->nbtot, (long long)oh.max_nb, oh.doff.back() / 1e6, ms, 32 >> mask,
                  h_err ? ": an in-part distance reached 255, not used" : "");
+    std::vector<int64_t> f = oh.fin;
+    std::sort(f.begin(), f.end());
+    if (!f.empty())
+      std::fprintf(stderr, "[obfs] LDS %zu B; step-3 LDS per part: median %lld, 90%% %lld, 99%% %lld, max %lld\n",
+                   pg->olds, (long long)f[f.size() / 2], (long long)f[f.size() * 9 / 10],
+                   (long long)f[f.size() * 99 / 100], (long long)f.back());
+  }
   if (h_err) {
     obfs_release(pg);
     return PHDE_OK;

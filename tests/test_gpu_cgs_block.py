@@ -100,3 +100,22 @@ def test_batched_cgs_against_one_at_a_time(gpu, case, tmp_path):
         for k in range(Y.shape[1]):
             x = X[:, k] if np.dot(X[:, k], Y[:, k]) >= 0 else -X[:, k]
             assert np.max(np.abs(x - Y[:, k])) / np.max(np.abs(Y[:, k])) < 1e-8, k
+
+
+def test_gram_cgs_declines_sums_beyond_2_53(gpu):
+    """A path of 250,000 vertices: max(sum deg, n) * max(d)^2 passes 2^53, so
+    the Gram matrices of its distance columns could not be summed exactly in
+    FP64; the Gram-space CGS takes no candidate and the vector CGS gives the
+    oracle's layout (with the oracle's discard of the far end's column,
+    which equals the ones vector minus the first column)."""
+    from oracle import pyoracle as O
+    from test_gpu_core import FP64_TOL, aligned_rel_err
+
+    g = Graph.path(250000)
+    ref = O.parhde(g, 3, 2, lazy=True)
+    with L.Context(g) as ctx:
+        X, st = ctx.layout(s=3, num_coord=2)
+    assert st.n_gram == 0
+    assert st.n_discarded == 1
+    assert list(st.pivots) == [int(p) for p in ref["pivots"][: len(st.pivots)]]
+    assert aligned_rel_err(X, ref["coords"]) < FP64_TOL
