@@ -29,18 +29,21 @@ constexpr int kEigThreads = 1024;
 constexpr int kEigWaves = kEigThreads / kWave;
 
 // Power method: every sum of the reference runs in one lane, in index
-// order, each product and sum rounded on its own.  Lane i of each wave holds
-// column i of the iterate in registers, padded with zeros to P rows (P =
-// 16, 32 or 64 >= dim): a padded term adds +0.0 to a sum that started at
-// +0.0 -- such a sum is never -0.0 -- so every sum equals the reference's
-// over dim terms, and the loops have no bounds to test.  The product E = K T
-// is split over the waves by rows and exchanged through LDS ([k][i]: row k
-// of every column contiguous); the deflation, normalisation and
-// convergence test then run in every wave alike, column j of the deflation
-// broadcast to the wave's lanes through a per-wave LDS row.  Measured at
-// dim 50 (29-34 iterations): 8.8 -> 3.2 ms; the deflation's 50 dependent
-// steps (a 64-term sum, a division and a 64-entry update each) are most of
-// it.
+// order, each product and sum rounded on its own.  Lane i holds column i of
+// the iterate in registers, padded with zeros to P rows (P = dim rounded up
+// to 8, at least 16): a padded term adds +0.0 to a sum that started at +0.0
+// -- such a sum is never -0.0 -- so every sum equals the reference's over
+// dim terms, and the loops have no bounds to test.  The product E = K T is
+// split over the block's waves by rows and exchanged through LDS ([k][i]:
+// row k of every column contiguous); the deflation, normalisation and
+// convergence test then run in wave 0 alone, column j of the deflation
+// broadcast to its lanes through an LDS row, and the new iterate goes back
+// to the other waves through LDS.  Measured at dim 50 (29 iterations):
+// 8.8 -> 3.2 ms with the columns in lanes; then 2.68 -> 2.33 ms when the
+// deflation lost three redundant copies (each step's broadcast row was read
+// by every wave, the CU's LDS bandwidth shared four ways) and the padding
+// went from 64 to 56 rows.  What is left is one wave's instruction stream:
+// per step, two passes of P multiplies and P adds or subtracts.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -51,18 +54,21 @@ constexpr int kPowerThreads = 256;
 constexpr int kPowerWaves = kPowerThreads / kWave;
 
 // LDS of power_eig_kernel<P>: K padded ([k + j*P]), the iterate and the
-// previous iterate ([k*P + i]), one broadcast row per wave.
-constexpr size_t power_lds_bytes(int P) { return sizeof(double) * (3 * (size_t)P * P + (size_t)kPowerWaves * P); }
+// previous iterate ([k*P + i]), wave 0's broadcast row, the convergence
+// measure.
+constexpr size_t power_lds_bytes(int P) { return sizeof(double) * (3 * (size_t)P * P + (size_t)P + 2); }
 
 template <int P>
 __global__ __launch_bounds__(kPowerThreads) void power_eig_kernel(double *K, const double *init, double *evals,
                                                                   int dim, int max_iter, int *status) {
+  static_assert(P % 8 == 0 && P >= 16 && P <= 64, "rows: a multiple of 8, at most one per lane");
   extern __shared__ __attribute__((aligned(16))) double lds[];
   double *Kp = lds;
   double *E = lds + P * P;
   double *T = lds + 2 * P * P;
+  double *row = lds + 3 * P * P;
+  double *shr = row + P;
   const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
-  double *row = lds + 3 * P * P + wid * P;
   const bool col = lane < dim;
   for (int e = tid; e < P * P; e += kPowerThreads) {
     const int a = e % P, b = e / P;
@@ -89,6 +95,7 @@ __global__ __launch_bounds__(kPowerThreads) void power_eig_kernel(double *K, con
   const double EPSILON = 1 - 1e-3;
   double r = 0;
   int iters = 0, st = 0;
+  __syncthreads();  // E is rewritten below
   while (r < EPSILON) {
     iters++;
     if (isnan(r) || isinf(r)) {
@@ -105,88 +112,97 @@ __global__ __launch_bounds__(kPowerThreads) void power_eig_kernel(double *K, con
       for (int k = 0; k < P; k++) T[k * P + lane] = x[k];
     }
     // eVecs[k + i*p] = sum_j K[k + j*p] * tmpOld[j + i*p], j ascending
-    // (:91-100): wave w takes rows [w*P/W, (w+1)*P/W), four at a time
+    // (:91-100): wave w takes rows [w*P/W, (w+1)*P/W), two at a time
     constexpr int kRows = P / kPowerWaves;
 #pragma unroll
-    for (int g = 0; g < kRows; g += 4) {
+    for (int g = 0; g < kRows; g += 2) {
       const int k0 = wid * kRows + g;
-      double y0 = 0, y1 = 0, y2 = 0, y3 = 0;
+      double y0 = 0, y1 = 0;
 #pragma unroll
       for (int j = 0; j < P; j++) {
         const double2 ka = *reinterpret_cast<const double2 *>(Kp + k0 + j * P);
-        const double2 kb = *reinterpret_cast<const double2 *>(Kp + k0 + 2 + j * P);
         const double xj = x[j];
-        const double p0 = ka.x * xj, p1 = ka.y * xj, p2 = kb.x * xj, p3 = kb.y * xj;
+        const double p0 = ka.x * xj, p1 = ka.y * xj;
         y0 = y0 + p0;
         y1 = y1 + p1;
-        y2 = y2 + p2;
-        y3 = y3 + p3;
       }
       if (lane < P) {
         E[(k0 + 0) * P + lane] = y0;
         E[(k0 + 1) * P + lane] = y1;
-        E[(k0 + 2) * P + lane] = y2;
-        E[(k0 + 3) * P + lane] = y3;
       }
     }
     __syncthreads();
+    if (wid == 0) {
 #pragma unroll
-    for (int k = 0; k < P; k++) x[k] = lane < P ? E[k * P + lane] : 0.0;
-    // deflation (:102-112): for each j, lane t >= j sums E_j . E_t in index
-    // order (t == j: the squared norm of the final column j, the
-    // denominator, recomputed by the reference for every i); lanes t > j
-    // subtract fac E_j, fac = sum / denominator
-    double dj_own = 1;
-    for (int j = 0; j < dim; j++) {
-      if (lane == j) {
+      for (int k = 0; k < P; k++) x[k] = lane < P ? E[k * P + lane] : 0.0;
+      // deflation (:102-112): for each j, lane t >= j sums E_j . E_t in
+      // index order (t == j: the squared norm of the final column j, the
+      // denominator, recomputed by the reference for every i); lanes t > j
+      // subtract fac E_j, fac = sum / denominator
+      double dj_own = 1;
+      for (int j = 0; j < dim; j++) {
+        if (lane == j) {
 #pragma unroll
-        for (int k = 0; k < P; k += 2) *reinterpret_cast<double2 *>(row + k) = make_double2(x[k], x[k + 1]);
-      }
-      wave_lds_sync();
-      double sum = 0;
-#pragma unroll
-      for (int k = 0; k < P; k += 2) {
-        const double2 e = *reinterpret_cast<const double2 *>(row + k);
-        const double p0 = e.x * x[k];
-        sum = sum + p0;
-        const double p1 = e.y * x[k + 1];
-        sum = sum + p1;
-      }
-      const double dj = __shfl(sum, j);
-      if (lane == j) dj_own = sum;
-      if (lane > j) {
-        const double fac = sum / dj;
+          for (int k = 0; k < P; k += 2) *reinterpret_cast<double2 *>(row + k) = make_double2(x[k], x[k + 1]);
+        }
+        wave_lds_sync();
+        double sum = 0;
 #pragma unroll
         for (int k = 0; k < P; k += 2) {
           const double2 e = *reinterpret_cast<const double2 *>(row + k);
-          const double q0 = fac * e.x, q1 = fac * e.y;
-          x[k] = x[k] - q0;
-          x[k + 1] = x[k + 1] - q1;
+          const double p0 = e.x * x[k];
+          sum = sum + p0;
+          const double p1 = e.y * x[k + 1];
+          sum = sum + p1;
         }
+        const double dj = __shfl(sum, j);
+        if (lane == j) dj_own = sum;
+        if (lane > j) {
+          const double fac = sum / dj;
+#pragma unroll
+          for (int k = 0; k < P; k += 2) {
+            const double2 e = *reinterpret_cast<const double2 *>(row + k);
+            const double q0 = fac * e.x, q1 = fac * e.y;
+            x[k] = x[k] - q0;
+            x[k + 1] = x[k + 1] - q1;
+          }
+        }
+        wave_lds_sync();
       }
-      wave_lds_sync();
-    }
-    // normalisation (:113-120): normL2 sums the same squares in the same
-    // order as the denominator above
-    ev = col ? sqrt(dj_own) : 1.0;
+      // normalisation (:113-120): normL2 sums the same squares in the same
+      // order as the denominator above
+      ev = col ? sqrt(dj_own) : 1.0;
 #pragma unroll
-    for (int k = 0; k < P; k++) x[k] = x[k] / ev;
-    // convergence test (:122-130): r = min_i |<e_i, t_i>|
-    double c = 0;
+      for (int k = 0; k < P; k++) x[k] = x[k] / ev;
+      // convergence test (:122-130): r = min_i |<e_i, t_i>|
+      double c = 0;
 #pragma unroll
-    for (int k = 0; k < P; k++) {
-      const double p = x[k] * (lane < P ? T[k * P + lane] : 0.0);
-      c = c + p;
-    }
-    if (c < 0) c *= -1;
-    double rr = col && c < 1 ? c : 1.0;
+      for (int k = 0; k < P; k++) {
+        const double p = x[k] * (lane < P ? T[k * P + lane] : 0.0);
+        c = c + p;
+      }
+      if (c < 0) c *= -1;
+      double rr = col && c < 1 ? c : 1.0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double v = __shfl_xor(rr, o);
-      rr = v < rr ? v : rr;
+      for (int o = 32; o > 0; o >>= 1) {
+        const double v = __shfl_xor(rr, o);
+        rr = v < rr ? v : rr;
+      }
+      r = rr;
+      // the new iterate and r to the other waves
+      if (lane < P) {
+#pragma unroll
+        for (int k = 0; k < P; k++) E[k * P + lane] = x[k];
+      }
+      if (lane == 0) shr[0] = rr;
     }
-    r = rr;
-    __syncthreads();  // T is rewritten at the next iteration's start
+    __syncthreads();
+    if (wid != 0) {
+#pragma unroll
+      for (int k = 0; k < P; k++) x[k] = lane < P ? E[k * P + lane] : 0.0;
+      r = shr[0];
+    }
+    __syncthreads();  // T and E are rewritten at the next iteration's start
   }
   if (st == 0 && wid == 0) {
     // eigenvalue reversal (:132-140); column i goes to column dim-1-i,
@@ -430,9 +446,13 @@ int power_eig_dev(phde_ctx *c, double *K_dev, const double *init_dev, double *ev
     set_error("power method: dim " + std::to_string(dim) + " outside 1..64");
     return PHDE_ERR_ARG;
   }
+  // rows padded to a multiple of 8, at least 16
+  const int P = std::max(16, (dim + 7) / 8 * 8);
   void (*fn)(double *, const double *, double *, int, int, int *) =
-      dim <= 16 ? power_eig_kernel<16> : dim <= 32 ? power_eig_kernel<32> : power_eig_kernel<64>;
-  const size_t shmem = power_lds_bytes(dim <= 16 ? 16 : dim <= 32 ? 32 : 64);
+      P == 16 ? power_eig_kernel<16> : P == 24 ? power_eig_kernel<24> : P == 32 ? power_eig_kernel<32>
+      : P == 40 ? power_eig_kernel<40> : P == 48 ? power_eig_kernel<48> : P == 56 ? power_eig_kernel<56>
+                                                                                    : power_eig_kernel<64>;
+  const size_t shmem = power_lds_bytes(P);
   if (shmem > 65536)
     PHDE_HIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
   hipEvent_t ev;
