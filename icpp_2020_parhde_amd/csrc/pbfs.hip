@@ -638,6 +638,11 @@ struct ObfsArgs {
   const uint32_t *cblob_off;  // per part, in 16-byte units
   const unsigned char *dmat;
   const unsigned long long *doff;  // per part, bytes
+  // with din: every part's in-part distances from each boundary vertex to
+  // each vertex (row per boundary vertex, nv bytes, kDInf: no path inside
+  // the part), so that step 1 reads the source's column instead of a BFS
+  const unsigned char *din;
+  const unsigned long long *dinoff;  // per part, bytes
   int32_t *blevel;   // boundary-major: a part's boundary levels (written by its runs)
   int32_t *bilevel;  // boundary-major: levels lowered from outside (atomicMin)
   uint32_t *st, *ctl;
@@ -1138,8 +1143,23 @@ __global__ __launch_bounds__(Block) void obfs_source_kernel(ObfsArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   __shared__ int s_cnt[3], s_min;
   __shared__ int32_t s_dummy[Block];
+  const int src = blockIdx.x;
+  const uint32_t p = a.src_part[src];
+  if (a.din) {
+    // the source's column of the part's distance table: its in-part
+    // distance to every boundary vertex (symmetric graph)
+    const uint32_t nv = a.vstart[p + 1] - a.vstart[p];
+    const uint32_t b0 = a.bstart[p];
+    const int nb = (int)(a.bstart[p + 1] - b0);
+    const unsigned char *col = a.din + a.dinoff[p] + a.src_loc[src];
+    for (int j = threadIdx.x; j < nb; j += Block) {
+      const unsigned char d = col[(size_t)j * nv];
+      if (d != kDInf) st_dev(&a.bilevel[src * a.nbtot + b0 + j], (int32_t)d);
+    }
+    return;
+  }
   KcAcc acc;
-  part_bfs<Block, Lanes, Claims>(a, a.src_part[blockIdx.x], blockIdx.x, false, smem, s_cnt, &s_min, s_dummy, acc);
+  part_bfs<Block, Lanes, Claims>(a, p, src, false, smem, s_cnt, &s_min, s_dummy, acc);
 }
 
 // Steps 2 and 3: every workgroup resident; the queue of parts, then the
@@ -1468,6 +1488,8 @@ struct DmatArgs {
   const uint4 *cblob;
   unsigned char *dmat;
   const unsigned long long *doff;
+  unsigned char *din;  // nullable: also every boundary vertex's distance to every vertex
+  const unsigned long long *dinoff;
   unsigned int *err;  // set when an in-part distance reaches kDInf
 };
 
@@ -1497,6 +1519,7 @@ __global__ __launch_bounds__(Block) void dmat_kernel(DmatArgs a) {
     __syncthreads();
     for (int j = tid; j < nb; j += Block) bpos[blist[j]] = (uint16_t)j;
     unsigned char *D = a.dmat + a.doff[p];
+    unsigned char *Din = a.din ? a.din + a.dinoff[p] : nullptr;
     for (int s0 = 0; s0 < nb; s0 += W) {
       const int sw = nb - s0 < W ? nb - s0 : W;
       const M full = (M)(sw == W ? ~0ull : ((1ull << sw) - 1ull));
@@ -1506,6 +1529,7 @@ __global__ __launch_bounds__(Block) void dmat_kernel(DmatArgs a) {
         const int v = blist[s0 + i];
         vis[v] = cur[v] = (M)(1ull << i);
         D[(size_t)(s0 + i) * nbp + s0 + i] = 0;
+        if (Din) Din[(size_t)(s0 + i) * nv + v] = 0;
       }
       if (tid == 0) s_any[0] = s_any[1] = 0;
       __syncthreads();
@@ -1524,6 +1548,16 @@ __global__ __launch_bounds__(Block) void dmat_kernel(DmatArgs a) {
         for (int v = tid; v < nv; v += Block) {
           M nw = nxt[v];
           cur[v] = nw;
+          if (Din && nw) {
+            if (L >= (int)kDInf) {
+              atomicOr(a.err, 2u);  // a distance the byte table cannot hold: no table
+            } else {
+              // source by source, so that a wave's byte stores for one
+              // source land in consecutive bytes of that source's row
+              for (int i = 0; i < sw; i++)
+                if ((nw >> i) & 1) Din[(size_t)(s0 + i) * nv + v] = (unsigned char)L;
+            }
+          }
           if (nw) {
             vis[v] |= nw;
             s_any[L & 1] = 1;
@@ -1602,6 +1636,8 @@ struct PartGraph {
   uint4 *cblob = nullptr, *oblob = nullptr;
   unsigned char *dmat = nullptr;
   unsigned long long *doff = nullptr;
+  unsigned char *din = nullptr;  // in-part distance tables for step 1 (nullptr: step 1 runs a BFS)
+  unsigned long long *dinoff = nullptr;
   int32_t *blevel = nullptr, *bilevel = nullptr;
   ObfsShape oshape{};
   size_t olds = 0;  // dynamic LDS of the boundary-distance kernels
@@ -1613,13 +1649,16 @@ struct PartGraph {
 // stays usable).
 static void obfs_release(PartGraph *pg) {
   void *ptr[] = {pg->pverts, pg->bstart, pg->xoff, pg->noff,   pg->cblob_off, pg->oblob_off, pg->cblob,
-                 pg->oblob,  pg->dmat,   pg->doff, pg->blevel, pg->bilevel,   pg->mscr};
+                 pg->oblob,  pg->dmat,   pg->doff, pg->blevel, pg->bilevel,   pg->mscr,      pg->din,
+                 pg->dinoff};
   for (void *x : ptr)
     if (x) (void)hipFree(x);
   pg->pverts = pg->bstart = pg->xoff = pg->noff = pg->cblob_off = pg->oblob_off = nullptr;
   pg->cblob = pg->oblob = nullptr;
   pg->dmat = nullptr;
   pg->doff = nullptr;
+  pg->din = nullptr;
+  pg->dinoff = nullptr;
   pg->blevel = pg->bilevel = nullptr;
   pg->mscr = nullptr;
   pg->ovl = false;
@@ -1649,7 +1688,7 @@ static int upload(T **dst, const std::vector<T> &src) {
 struct ObfsHost {
   std::vector<uint32_t> bmidx, bstart, xoff{0}, noff{0}, coff{0}, eoff{0};
   std::vector<uint4> cblob, oblob;
-  std::vector<unsigned long long> doff{0};
+  std::vector<unsigned long long> doff{0}, dinoff{0};
   int64_t nx_tot = 0, nn_tot = 0;
   // largest LDS of step 3 (and 1), of a run of step 2 (with its edge
   // blob), and of dmat_kernel with 32-, 16- and 8-bit masks
@@ -1698,6 +1737,7 @@ struct ObfsHost {
     std::memcpy(&cblob[at], b16.data(), cb);
     coff.push_back((uint32_t)cblob.size());
     doff.push_back(doff.back() + (unsigned long long)(nb * drow(nb)));
+    dinoff.push_back(dinoff.back() + (unsigned long long)nb * (unsigned long long)nv);
     max_final = std::max(max_final, final_bytes(nv, ne, nb));
     fin.push_back(final_bytes(nv, ne, nb));
     max_mf = std::max(max_mf, mfinal_bytes(nv, ne, nb));
@@ -1739,6 +1779,21 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   PHDE_HIP(hipMalloc(&pg->blevel, sizeof(int32_t) * std::max<int64_t>(pg->nbtot, 1)));
   PHDE_HIP(hipMalloc(&pg->bilevel, sizeof(int32_t) * std::max<int64_t>(pg->nbtot, 1)));
   PHDE_HIP(hipMemsetAsync(pg->dmat, kDInf, dbytes, c->stream));
+  // step 1's tables (every boundary vertex's in-part distance to every
+  // vertex of its part), up to kDinCap bytes; PHDE_OBFS_DIN=0: step 1 runs a
+  // BFS inside the source's part instead
+  constexpr unsigned long long kDinCap = 16ull << 30;
+  bool din = oh.dinoff.back() > 0 && oh.dinoff.back() <= kDinCap;
+  if (const char *e = getenv("PHDE_OBFS_DIN")) din = din && atoi(e) != 0;
+  if (din) {
+    PHDE_TRY_RC(upload(&pg->dinoff, oh.dinoff));
+    if (hipMalloc(&pg->din, oh.dinoff.back()) != hipSuccess) {
+      (void)hipGetLastError();
+      pg->din = nullptr;
+    } else {
+      PHDE_HIP(hipMemsetAsync(pg->din, kDInf, oh.dinoff.back(), c->stream));
+    }
+  }
   DBuf<unsigned int> err;
   PHDE_TRY_RC(err.alloc(1));
   PHDE_HIP(hipMemsetAsync(err.p, 0, sizeof(unsigned int), c->stream));
@@ -1750,6 +1805,8 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   da.cblob = pg->cblob;
   da.dmat = pg->dmat;
   da.doff = pg->doff;
+  da.din = pg->din;
+  da.dinoff = pg->dinoff;
   da.err = err.p;
   void (*dk)(DmatArgs) = mask == 0 ? dmat_kernel<uint32_t, 1024> : mask == 1 ? dmat_kernel<uint16_t, 1024>
                                                                              : dmat_kernel<uint8_t, 1024>;
@@ -1772,10 +1829,18 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   pg->olds = (size_t)align_up(std::max({oh.max_final, oh.max_orun, std::min<int64_t>(oh.max_mf, 152 * 1024)}), 256);
   for (const void *f : {(const void *)pg->oshape.fn, (const void *)pg->oshape.src_fn, (const void *)pg->oshape.mfin_fn})
     PHDE_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pg->olds));
+  if ((h_err & 2u) && pg->din) {
+    // some vertex lies 255 or more in-part hops from a boundary vertex: no
+    // tables, step 1 runs its BFS
+    (void)hipFree(pg->din);
+    pg->din = nullptr;
+  }
   if (pg->stats) {
-    std::fprintf(stderr, "[obfs] boundary %lld (largest part %lld), Dp %.1f MB in %.1f ms (%d-bit masks)%s\n",
+    std::fprintf(stderr, "[obfs] boundary %lld (largest part %lld), Dp %.1f MB in %.1f ms (%d-bit masks)%s; "
+                 "step-1 tables %.1f MB%s\n",
                  (long long)pg->nbtot, (long long)oh.max_nb, oh.doff.back() / 1e6, ms, 32 >> mask,
-                 h_err ? ": an in-part distance reached 255, not used" : "");
+                 (h_err & 1u) ? ": an in-part distance reached 255, not used" : "",
+                 pg->din ? oh.dinoff.back() / 1e6 : 0.0, pg->din ? "" : " (off)");
     std::vector<int64_t> f = oh.fin;
     std::sort(f.begin(), f.end());
     if (!f.empty())
@@ -1783,7 +1848,7 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
                    pg->olds, (long long)f[f.size() / 2], (long long)f[f.size() * 9 / 10],
                    (long long)f[f.size() * 99 / 100], (long long)f.back());
   }
-  if (h_err) {
+  if (h_err & 1u) {
     obfs_release(pg);
     return PHDE_OK;
   }
@@ -2030,6 +2095,8 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
   a.cblob_off = pg->cblob_off;
   a.dmat = pg->dmat;
   a.doff = pg->doff;
+  a.din = pg->din;
+  a.dinoff = pg->dinoff;
   a.blevel = pg->blevel;
   a.bilevel = pg->bilevel;
   a.st = pg->st;

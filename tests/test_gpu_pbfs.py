@@ -22,8 +22,9 @@ pytestmark = pytest.mark.gpu
 
 
 @contextmanager
-def context(g, part_bytes=None, pbfs=True, obfs=True):
-    env = {"PHDE_RELABEL": "1", "PHDE_PBFS": "1" if pbfs else "0", "PHDE_OBFS": "1" if obfs else "0"}
+def context(g, part_bytes=None, pbfs=True, obfs=True, din=True):
+    env = {"PHDE_RELABEL": "1", "PHDE_PBFS": "1" if pbfs else "0", "PHDE_OBFS": "1" if obfs else "0",
+           "PHDE_OBFS_DIN": "1" if din else "0"}
     if part_bytes:
         env["PHDE_PBFS_BYTES"] = str(part_bytes)
     old = {k: os.environ.get(k) for k in env}
@@ -55,13 +56,15 @@ GRAPHS = {
 BUDGETS = [None, 4096, 600]
 
 
-MODES = ["obfs", "pbfs"]  # boundary-distance method, or rounds in LDS at every run
+# boundary-distance method (step 1 from the in-part distance tables, or by
+# a BFS inside the source's part), or rounds in LDS at every run
+MODES = ["obfs", "obfs_bfs1", "pbfs"]
 
 
 def check_mode(ctx, mode):
     parts = ctx.num_parts()
     assert parts >= 1
-    if mode == "obfs":
+    if mode != "pbfs":
         assert ctx.num_boundary() >= 0
     else:
         assert ctx.num_boundary() == -1
@@ -75,7 +78,7 @@ def test_levels_exact(gpu, name, budget, mode):
     g = GRAPHS[name]()
     rng = np.random.default_rng(7)
     srcs = [0, g.n - 1, g.n // 2] + [int(x) for x in rng.choice(g.n, 4, replace=False)]
-    with context(g, budget, obfs=mode == "obfs") as ctx:
+    with context(g, budget, obfs=mode != "pbfs", din=mode != "obfs_bfs1") as ctx:
         parts = check_mode(ctx, mode)
         for s in srcs:
             want = O.seq_bfs(g, s)
@@ -88,7 +91,7 @@ def test_levels_exact(gpu, name, budget, mode):
 @pytest.mark.parametrize("budget", BUDGETS)
 def test_kcenters_same_as_without(gpu, budget, mode):
     g = GRAPHS["holed"]()
-    with context(g, budget, obfs=mode == "obfs") as ctx:
+    with context(g, budget, obfs=mode != "pbfs", din=mode != "obfs_bfs1") as ctx:
         check_mode(ctx, mode)
         a = ctx.kcenters(30, first=11)
     with context(g, pbfs=False) as ctx:
@@ -103,7 +106,7 @@ def test_kcenters_same_as_without(gpu, budget, mode):
 def test_layout_parity(gpu, name, s, budget, mode):
     g = GRAPHS[name]()
     ref = O.parhde(g, s, 2, lazy=True)
-    with context(g, budget, obfs=mode == "obfs") as ctx:
+    with context(g, budget, obfs=mode != "pbfs", din=mode != "obfs_bfs1") as ctx:
         check_mode(ctx, mode)
         X, st = ctx.layout(s=s, num_coord=2)
         Xe, ste = ctx.layout(s=s, num_coord=2, eager=1)
@@ -115,7 +118,7 @@ def test_layout_parity(gpu, name, s, budget, mode):
 @pytest.mark.parametrize("mode", MODES)
 def test_unreached_component(gpu, mode):
     g = two_paths(400)
-    with context(g, 600, obfs=mode == "obfs") as ctx:
+    with context(g, 600, obfs=mode != "pbfs", din=mode != "obfs_bfs1") as ctx:
         check_mode(ctx, mode)
         d = ctx.bfs(3)
         assert d[400:].tolist() == [-1] * 400 and d[:400].tolist() == [abs(i - 3) for i in range(400)]
