@@ -598,6 +598,11 @@ __host__ __device__ constexpr int64_t cblob_bytes(int64_t nv, int64_t ne, int64_
 __host__ __device__ constexpr int64_t final_bytes(int64_t nv, int64_t ne, int64_t nb) {
   return cblob_bytes(nv, ne, nb) + 4 * nv + align_up(4 * (nv + 1), 4) + align_up(2 * (nb + 2), 4) + 4 * (nb + 2);
 }
+// LDS of part_table: levels (int32, nv rounded up to 16), the boundary
+// levels, the entries' levels and rows (uint16).
+__host__ __device__ constexpr int64_t table_bytes(int64_t nv, int64_t nb) {
+  return 4 * align_up(nv, 16) + 8 * nb + align_up(2 * nb, 4) + 16;
+}
 // LDS of a multi-source interior pass (part_bfs_multi): the image, three
 // 32-bit source masks per vertex, the seeds' relative levels (bytes, 32 per
 // boundary vertex) and the sources' lowest levels in the part.
@@ -778,6 +783,182 @@ struct KcAcc {
   U128 sumsq{0, 0};
   int32_t maxd = 0;
 };
+
+// Write-out of part p's levels (LDS, nv entries) with the fused k-centers
+// step: min_dist = min(min_dist, level), the first maximiser (caller's id
+// on ties), exact sum of squares, unreached count, largest level.
+template <int Block>
+__device__ __forceinline__ void part_write_out(const ObfsArgs &a, uint32_t v0, int nv, const int32_t *lvl, int src,
+                                               KcAcc &acc) {
+  for (int v = threadIdx.x; v < nv; v += Block) {
+    const int32_t d = lvl[v];
+    const uint32_t i = a.pverts[v0 + v];
+    a.level[src * a.g.n + i] = d;
+    if (d == kUnvisited) {
+      acc.unreached++;
+    } else {
+      u128_add(acc.sumsq, (unsigned long long)d * (unsigned long long)d);
+      acc.maxd = d > acc.maxd ? d : acc.maxd;
+    }
+    if (a.min_dist) {
+      int32_t md = a.min_dist[i];
+      if (d < md) {
+        md = d;
+        a.min_dist[i] = md;
+      }
+      const unsigned long long kk = ((unsigned long long)(uint32_t)md << 32) |
+                                    (0xffffffffull - (unsigned long long)original_id(a.g, i));
+      acc.key = kk > acc.key ? kk : acc.key;
+    }
+  }
+}
+
+// Step 3 for a part without the source, from the in-part distance tables
+// (single-source search): the last stretch of a shortest path to v inside
+// the part starts at one of its boundary vertices, so level(v) = min over
+// the part's boundary b of L_b + Din[b][v], with L_b the exact boundary
+// levels of step 2.  Only entries count: a boundary vertex j with
+// L_i + Dp[i][j] <= L_j for another i adds nothing, since
+// Din[i][v] <= Dp[i][j] + Din[j][v].  The entries' rows are read 16
+// vertices per load, each thread taking one 16-vertex group and every G-th
+// entry, the groups combined with LDS atomicMin; no rounds, no barrier per
+// level.
+template <int Block>
+__device__ void part_table(const ObfsArgs &a, uint32_t p, unsigned char *smem, int *s_cnt, KcAcc &acc) {
+  const int tid = threadIdx.x;
+  const int lane = tid % kWave;
+  // with stats: [20] boundary levels and entries, [21] the entries' rows,
+  // [23] write-out, [24] entries
+  const bool stats = a.stats != nullptr;
+  unsigned long long tp0 = 0, tp1 = 0;
+  if (stats && tid == 0) tp0 = __builtin_amdgcn_s_memrealtime();
+  auto phase = [&](int slot) {
+    if (stats && tid == 0) {
+      tp1 = __builtin_amdgcn_s_memrealtime();
+      atomicAdd(&a.stats[slot], tp1 - tp0);
+      tp0 = tp1;
+    }
+  };
+  const uint32_t v0 = a.vstart[p];
+  const int nv = (int)(a.vstart[p + 1] - v0);
+  const int nvp = (int)align_up(nv, 16);
+  const uint32_t b0 = a.bstart[p];
+  const int nb = (int)(a.bstart[p + 1] - b0);
+  const int nbp = (int)drow(nb);
+  int32_t *lvl = reinterpret_cast<int32_t *>(smem);
+  int32_t *L = lvl + nvp;
+  int32_t *el = L + nb;
+  uint16_t *ej = reinterpret_cast<uint16_t *>(el + nb);
+  if (tid == 0) s_cnt[0] = 0;
+  for (int v = tid; v < nvp; v += Block) lvl[v] = kUnvisited;
+  for (int j = tid; j < nb; j += Block) L[j] = ld_dev(&a.blevel[b0 + j]);
+  __syncthreads();
+  // dominated boundary vertices: each (j, 16-column group of row j) is an
+  // item of its own, so that every load is independent; a dominated j gets
+  // its level set to kUnvisited in el (used as the flag, entries below)
+  const unsigned char *D = a.dmat + a.doff[p];
+  const int ng = nbp / 16;
+  for (int j = tid; j < nb; j += Block) el[j] = L[j];
+  __syncthreads();
+  // kItems items per thread, their loads issued together (Dp is
+  // symmetric: row j holds the distances to j)
+  constexpr int kItems = 8;
+  for (int it0 = tid; it0 < nb * ng; it0 += kItems * Block) {
+    uint4 w[kItems];
+#pragma unroll
+    for (int q = 0; q < kItems; q++) {
+      const int it = it0 + q * Block;
+      const int j = it < nb * ng ? it / ng : 0, g = it < nb * ng ? it % ng : 0;
+      w[q] = reinterpret_cast<const uint4 *>(D + (size_t)j * nbp)[g];
+    }
+#pragma unroll
+    for (int q = 0; q < kItems; q++) {
+      const int it = it0 + q * Block;
+      if (it >= nb * ng) break;
+      const int j = it / ng, g = it % ng;
+      const int32_t Lj = L[j];
+      if (Lj == kUnvisited) continue;
+      const uint32_t wq[4] = {w[q].x, w[q].y, w[q].z, w[q].w};
+      bool dom = false;
+#pragma unroll
+      for (int c = 0; c < 16; c++) {
+        const int i = 16 * g + c;
+        const int32_t d = (int32_t)((wq[c / 4] >> (8 * (c % 4))) & 0xffu);
+        if (i < nb && i != j && d != kDInf) {
+          const int32_t Li = L[i];
+          dom = dom || (Li != kUnvisited && Li + d <= Lj);
+        }
+      }
+      if (dom) el[j] = kUnvisited;
+    }
+  }
+  __syncthreads();
+  // the entries, listed
+  for (int jj = 0; jj < nb; jj += Block) {
+    const int j = jj + tid;
+    const int32_t Lj = j < nb ? el[j] : kUnvisited;
+    const bool entry = Lj != kUnvisited;
+    __syncthreads();  // every flag read before the list overwrites el
+    const unsigned long long m = __ballot(entry);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&s_cnt[0], __popcll(m));
+      base = __builtin_amdgcn_readlane(base, leader);
+      if (entry) {
+        const int at = base + __popcll(m & lanemask_lt());
+        ej[at] = (uint16_t)j;
+        el[at] = Lj;
+      }
+    }
+  }
+  __syncthreads();
+  const int ne = s_cnt[0];
+  phase(20);
+  if (stats && tid == 0) atomicAdd(&a.stats[24], (unsigned long long)ne);
+  const unsigned char *Din = a.din + a.dinoff[p];
+  const int nw = nvp / 16;
+  const int G = nw < Block ? Block / nw : 1;
+  for (int t = tid; t < nw * G; t += Block) {
+    const int w = t % nw, g = t / nw;
+    int32_t mn[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) mn[c] = kUnvisited;
+    // kRowsAt entries' rows in flight together
+    constexpr int kRowsAt = 8;
+    for (int e0 = g; e0 < ne; e0 += kRowsAt * G) {
+      uint4 q[kRowsAt];
+      int32_t Le[kRowsAt];
+#pragma unroll
+      for (int r = 0; r < kRowsAt; r++) {
+        const int e = e0 + r * G;
+        const int ee = e < ne ? e : e0;
+        q[r] = *reinterpret_cast<const uint4 *>(Din + (size_t)ej[ee] * nvp + 16 * w);
+        Le[r] = e < ne ? el[ee] : kUnvisited;
+      }
+#pragma unroll
+      for (int r = 0; r < kRowsAt; r++) {
+        const uint32_t qq[4] = {q[r].x, q[r].y, q[r].z, q[r].w};
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+          const int32_t d = (int32_t)((qq[c / 4] >> (8 * (c % 4))) & 0xffu);
+          const int32_t cand = d == kDInf || Le[r] == kUnvisited ? kUnvisited : Le[r] + d;
+          mn[c] = cand < mn[c] ? cand : mn[c];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; c++)
+      if (mn[c] != kUnvisited) atomicMin(&lvl[16 * w + c], mn[c]);
+  }
+  __syncthreads();
+  phase(21);
+  part_write_out<Block>(a, v0, nv, lvl, 0, acc);
+  if (stats) {
+    __syncthreads();
+    phase(23);
+  }
+}
 
 // BFS inside part p (steps 1 and 3).  Source mode (final == false): from
 // the source alone; the levels of the part's boundary vertices go to
@@ -973,30 +1154,7 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, int src, bool final_mode,
     }
     return rounds;
   }
-  // write-out with the fused k-centers step: min_dist = min(min_dist,
-  // level), the first maximiser (caller's id on ties), exact sum of
-  // squares, unreached count, largest level
-  for (int v = tid; v < nv; v += Block) {
-    const int32_t d = lvl[v];
-    const uint32_t i = a.pverts[v0 + v];
-    a.level[src * a.g.n + i] = d;
-    if (d == kUnvisited) {
-      acc.unreached++;
-    } else {
-      u128_add(acc.sumsq, (unsigned long long)d * (unsigned long long)d);
-      acc.maxd = d > acc.maxd ? d : acc.maxd;
-    }
-    if (a.min_dist) {
-      int32_t md = a.min_dist[i];
-      if (d < md) {
-        md = d;
-        a.min_dist[i] = md;
-      }
-      const unsigned long long kk = ((unsigned long long)(uint32_t)md << 32) |
-                                    (0xffffffffull - (unsigned long long)original_id(a.g, i));
-      acc.key = kk > acc.key ? kk : acc.key;
-    }
-  }
+  part_write_out<Block>(a, v0, nv, lvl, src, acc);
   if (stats) {
     __syncthreads();
     phase(23);
@@ -1148,12 +1306,12 @@ __global__ __launch_bounds__(Block) void obfs_source_kernel(ObfsArgs a) {
   if (a.din) {
     // the source's column of the part's distance table: its in-part
     // distance to every boundary vertex (symmetric graph)
-    const uint32_t nv = a.vstart[p + 1] - a.vstart[p];
+    const uint32_t nvp = (uint32_t)align_up(a.vstart[p + 1] - a.vstart[p], 16);
     const uint32_t b0 = a.bstart[p];
     const int nb = (int)(a.bstart[p + 1] - b0);
     const unsigned char *col = a.din + a.dinoff[p] + a.src_loc[src];
     for (int j = threadIdx.x; j < nb; j += Block) {
-      const unsigned char d = col[(size_t)j * nv];
+      const unsigned char d = col[(size_t)j * nvp];
       if (d != kDInf) st_dev(&a.bilevel[src * a.nbtot + b0 + j], (int32_t)d);
     }
     return;
@@ -1405,7 +1563,10 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
       __syncthreads();
       const uint32_t p = s_part;
       if ((int64_t)p >= a.k) break;
-      n_fr += part_bfs<Block, Lanes, Claims>(a, p, 0, true, smem, s_cnt, &s_min, s_dummy, acc);
+      if (a.din && p != a.src_part[0])
+        part_table<Block>(a, p, smem, s_cnt, acc);
+      else
+        n_fr += part_bfs<Block, Lanes, Claims>(a, p, 0, true, smem, s_cnt, &s_min, s_dummy, acc);
       __syncthreads();
     }
   }
@@ -1520,6 +1681,7 @@ __global__ __launch_bounds__(Block) void dmat_kernel(DmatArgs a) {
     for (int j = tid; j < nb; j += Block) bpos[blist[j]] = (uint16_t)j;
     unsigned char *D = a.dmat + a.doff[p];
     unsigned char *Din = a.din ? a.din + a.dinoff[p] : nullptr;
+    const int nvp = (int)align_up(nv, 16);  // Din row stride
     for (int s0 = 0; s0 < nb; s0 += W) {
       const int sw = nb - s0 < W ? nb - s0 : W;
       const M full = (M)(sw == W ? ~0ull : ((1ull << sw) - 1ull));
@@ -1529,7 +1691,7 @@ __global__ __launch_bounds__(Block) void dmat_kernel(DmatArgs a) {
         const int v = blist[s0 + i];
         vis[v] = cur[v] = (M)(1ull << i);
         D[(size_t)(s0 + i) * nbp + s0 + i] = 0;
-        if (Din) Din[(size_t)(s0 + i) * nv + v] = 0;
+        if (Din) Din[(size_t)(s0 + i) * nvp + v] = 0;
       }
       if (tid == 0) s_any[0] = s_any[1] = 0;
       __syncthreads();
@@ -1555,7 +1717,7 @@ __global__ __launch_bounds__(Block) void dmat_kernel(DmatArgs a) {
               // source by source, so that a wave's byte stores for one
               // source land in consecutive bytes of that source's row
               for (int i = 0; i < sw; i++)
-                if ((nw >> i) & 1) Din[(size_t)(s0 + i) * nv + v] = (unsigned char)L;
+                if ((nw >> i) & 1) Din[(size_t)(s0 + i) * nvp + v] = (unsigned char)L;
             }
           }
           if (nw) {
@@ -1692,7 +1854,7 @@ struct ObfsHost {
   int64_t nx_tot = 0, nn_tot = 0;
   // largest LDS of step 3 (and 1), of a run of step 2 (with its edge
   // blob), and of dmat_kernel with 32-, 16- and 8-bit masks
-  int64_t max_final = 0, max_orun = 0, max_mf = 0, max_dm[3] = {0, 0, 0}, max_nb = 0;
+  int64_t max_final = 0, max_orun = 0, max_mf = 0, max_dm[3] = {0, 0, 0}, max_nb = 0, max_table = 0;
   std::vector<int64_t> fin;  // every part's step-3 LDS (statistics)
   bool ok = true;
   void add_part(int64_t p, uint32_t v0, uint32_t nv, const std::vector<uint32_t> &verts,
@@ -1737,7 +1899,8 @@ struct ObfsHost {
     std::memcpy(&cblob[at], b16.data(), cb);
     coff.push_back((uint32_t)cblob.size());
     doff.push_back(doff.back() + (unsigned long long)(nb * drow(nb)));
-    dinoff.push_back(dinoff.back() + (unsigned long long)nb * (unsigned long long)nv);
+    dinoff.push_back(dinoff.back() + (unsigned long long)nb * (unsigned long long)align_up(nv, 16));
+    max_table = std::max(max_table, table_bytes(nv, nb));
     max_final = std::max(max_final, final_bytes(nv, ne, nb));
     fin.push_back(final_bytes(nv, ne, nb));
     max_mf = std::max(max_mf, mfinal_bytes(nv, ne, nb));
@@ -1826,7 +1989,8 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   // LDS: step 3's largest part, a run's state with its edge blob and, up to
   // 152 KB (the kernel's static LDS takes 4 KB more), the multi-source
   // interior pass (parts beyond it take their sources one at a time)
-  pg->olds = (size_t)align_up(std::max({oh.max_final, oh.max_orun, std::min<int64_t>(oh.max_mf, 152 * 1024)}), 256);
+  pg->olds = (size_t)align_up(
+      std::max({oh.max_final, oh.max_orun, oh.max_table, std::min<int64_t>(oh.max_mf, 152 * 1024)}), 256);
   for (const void *f : {(const void *)pg->oshape.fn, (const void *)pg->oshape.src_fn, (const void *)pg->oshape.mfin_fn})
     PHDE_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pg->olds));
   if ((h_err & 2u) && pg->din) {
@@ -2167,9 +2331,11 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
                  s[3] * us / na, s[9] * us, s[10], s[10] / (double)pg->k);
     std::fprintf(stderr, "[obfs]   queue latency (queued to taken): %llu parts, mean %.2f us, %llu above 10 us, max %.1f us\n",
                  s[17], s[16] * 0.01 / (double)std::max(1ull, s[17]), s[18], s[19] * 0.01);
-    std::fprintf(stderr, "[obfs]   step 3 per part: image %.2f seeds %.2f rounds %.2f write-out %.2f us; %.3f us per round\n",
+    std::fprintf(stderr,
+                 "[obfs]   step 3 per part: image or entries %.2f seeds or table rows %.2f rounds %.2f write-out %.2f "
+                 "us; %.3f us per round; %.1f entries per part\n",
                  s[20] * us / pg->k, s[21] * us / pg->k, s[22] * us / pg->k, s[23] * us / pg->k,
-                 s[22] * us / (double)std::max(1ull, s[10]));
+                 s[22] * us / (double)std::max(1ull, s[10]), s[24] / (double)pg->k);
   }
   *ran = true;
   return PHDE_OK;
