@@ -601,7 +601,7 @@ __host__ __device__ constexpr int64_t final_bytes(int64_t nv, int64_t ne, int64_
 // LDS of part_table: levels (int32, nv rounded up to 16), the boundary
 // levels, the entries' levels and rows (uint16).
 __host__ __device__ constexpr int64_t table_bytes(int64_t nv, int64_t nb) {
-  return 4 * align_up(nv, 16) + 8 * nb + align_up(2 * nb, 4) + 16;
+  return 4 * align_up(nv, 16) + 12 * nb + 2 * align_up(2 * nb, 4) + 16;
 }
 // LDS of a multi-source interior pass (part_bfs_multi): the image, three
 // 32-bit source masks per vertex, the seeds' relative levels (bytes, 32 per
@@ -648,6 +648,11 @@ struct ObfsArgs {
   // the part), so that step 1 reads the source's column instead of a BFS
   const unsigned char *din;
   const unsigned long long *dinoff;  // per part, bytes
+  // single-source search with tables: per boundary vertex, 1 when its level
+  // came from outside the part (a seed) and no other boundary vertex of the
+  // part gave it through the part (0: its min-plus did) -- written with the
+  // level at every change, read by step 3 as the entry candidates
+  int32_t *entf;
   int32_t *blevel;   // boundary-major: a part's boundary levels (written by its runs)
   int32_t *bilevel;  // boundary-major: levels lowered from outside (atomicMin)
   uint32_t *st, *ctl;
@@ -846,38 +851,58 @@ __device__ void part_table(const ObfsArgs &a, uint32_t p, unsigned char *smem, i
   const int nb = (int)(a.bstart[p + 1] - b0);
   const int nbp = (int)drow(nb);
   int32_t *lvl = reinterpret_cast<int32_t *>(smem);
-  int32_t *L = lvl + nvp;
-  int32_t *el = L + nb;
-  uint16_t *ej = reinterpret_cast<uint16_t *>(el + nb);
-  if (tid == 0) s_cnt[0] = 0;
+  int32_t *L = lvl + nvp;  // boundary levels
+  int32_t *Lc = L + nb;    // the candidates' levels (kUnvisited elsewhere)
+  int32_t *el = Lc + nb;   // flags, then the entries' levels
+  uint16_t *ej = reinterpret_cast<uint16_t *>(el + nb);                          // the entries
+  uint16_t *cj = reinterpret_cast<uint16_t *>(reinterpret_cast<unsigned char *>(ej) + align_up(2 * nb, 4));
+  if (tid == 0) s_cnt[0] = s_cnt[1] = 0;
   for (int v = tid; v < nvp; v += Block) lvl[v] = kUnvisited;
-  for (int j = tid; j < nb; j += Block) L[j] = ld_dev(&a.blevel[b0 + j]);
+  // candidates: the boundary vertices step 2 left flagged as reached from
+  // outside (a superset of the entries: ties are not flagged off)
+  for (int jj = 0; jj < nb; jj += Block) {
+    const int j = jj + tid;
+    int32_t Lj = kUnvisited;
+    bool cand = false;
+    if (j < nb) {
+      Lj = ld_dev(&a.blevel[b0 + j]);
+      cand = Lj != kUnvisited && ld_dev(&a.entf[b0 + j]) != 0;
+      L[j] = Lj;
+      Lc[j] = cand ? Lj : kUnvisited;
+      el[j] = cand ? Lj : kUnvisited;
+    }
+    const unsigned long long m = __ballot(cand);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&s_cnt[1], __popcll(m));
+      base = __builtin_amdgcn_readlane(base, leader);
+      if (cand) cj[base + __popcll(m & lanemask_lt())] = (uint16_t)j;
+    }
+  }
   __syncthreads();
-  // dominated boundary vertices: each (j, 16-column group of row j) is an
-  // item of its own, so that every load is independent; a dominated j gets
-  // its level set to kUnvisited in el (used as the flag, entries below)
+  const int nc = s_cnt[1];
+  // the candidates dominated by another candidate (enough: a vertex
+  // dominated at all is dominated by an entry, by transitivity): each
+  // (candidate, 16-column group of its Dp row) an item of its own, the
+  // loads issued together (Dp is symmetric: row j holds the distances to j)
   const unsigned char *D = a.dmat + a.doff[p];
   const int ng = nbp / 16;
-  for (int j = tid; j < nb; j += Block) el[j] = L[j];
-  __syncthreads();
-  // kItems items per thread, their loads issued together (Dp is
-  // symmetric: row j holds the distances to j)
   constexpr int kItems = 8;
-  for (int it0 = tid; it0 < nb * ng; it0 += kItems * Block) {
+  for (int it0 = tid; it0 < nc * ng; it0 += kItems * Block) {
     uint4 w[kItems];
 #pragma unroll
     for (int q = 0; q < kItems; q++) {
       const int it = it0 + q * Block;
-      const int j = it < nb * ng ? it / ng : 0, g = it < nb * ng ? it % ng : 0;
+      const int j = it < nc * ng ? cj[it / ng] : 0, g = it < nc * ng ? it % ng : 0;
       w[q] = reinterpret_cast<const uint4 *>(D + (size_t)j * nbp)[g];
     }
 #pragma unroll
     for (int q = 0; q < kItems; q++) {
       const int it = it0 + q * Block;
-      if (it >= nb * ng) break;
-      const int j = it / ng, g = it % ng;
+      if (it >= nc * ng) break;
+      const int j = cj[it / ng], g = it % ng;
       const int32_t Lj = L[j];
-      if (Lj == kUnvisited) continue;
       const uint32_t wq[4] = {w[q].x, w[q].y, w[q].z, w[q].w};
       bool dom = false;
 #pragma unroll
@@ -885,7 +910,7 @@ __device__ void part_table(const ObfsArgs &a, uint32_t p, unsigned char *smem, i
         const int i = 16 * g + c;
         const int32_t d = (int32_t)((wq[c / 4] >> (8 * (c % 4))) & 0xffu);
         if (i < nb && i != j && d != kDInf) {
-          const int32_t Li = L[i];
+          const int32_t Li = Lc[i];
           dom = dom || (Li != kUnvisited && Li + d <= Lj);
         }
       }
@@ -1431,7 +1456,7 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
         bool add = false;
         if (in && v < lv[j]) {
           lv[j] = v;
-          chg[j] = 1;
+          chg[j] = 2;  // from outside (the min-plus below sets 1 if it lowers j further)
           add = true;
         }
         const unsigned long long m = __ballot(add);
@@ -1480,6 +1505,7 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
       for (int j = tid; j < nb; j += Block)
         if (chg[j]) {
           st_dev(&blev[b0 + j], lv[j]);
+          if (a.entf && a.nsrc == 1) st_dev(&a.entf[b0 + j], chg[j] == 2 ? 1 : 0);
           if (stats) atomicAdd(&a.stats[11], 1ull);
         }
       for (int xx = 0; xx < nx; xx += 4 * Block) {
@@ -1800,6 +1826,7 @@ struct PartGraph {
   unsigned long long *doff = nullptr;
   unsigned char *din = nullptr;  // in-part distance tables for step 1 (nullptr: step 1 runs a BFS)
   unsigned long long *dinoff = nullptr;
+  int32_t *entf = nullptr;  // step 3's entry candidates (with din)
   int32_t *blevel = nullptr, *bilevel = nullptr;
   ObfsShape oshape{};
   size_t olds = 0;  // dynamic LDS of the boundary-distance kernels
@@ -1812,7 +1839,7 @@ struct PartGraph {
 static void obfs_release(PartGraph *pg) {
   void *ptr[] = {pg->pverts, pg->bstart, pg->xoff, pg->noff,   pg->cblob_off, pg->oblob_off, pg->cblob,
                  pg->oblob,  pg->dmat,   pg->doff, pg->blevel, pg->bilevel,   pg->mscr,      pg->din,
-                 pg->dinoff};
+                 pg->dinoff, pg->entf};
   for (void *x : ptr)
     if (x) (void)hipFree(x);
   pg->pverts = pg->bstart = pg->xoff = pg->noff = pg->cblob_off = pg->oblob_off = nullptr;
@@ -1821,6 +1848,7 @@ static void obfs_release(PartGraph *pg) {
   pg->doff = nullptr;
   pg->din = nullptr;
   pg->dinoff = nullptr;
+  pg->entf = nullptr;
   pg->blevel = pg->bilevel = nullptr;
   pg->mscr = nullptr;
   pg->ovl = false;
@@ -1999,6 +2027,7 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
     (void)hipFree(pg->din);
     pg->din = nullptr;
   }
+  if (pg->din) PHDE_HIP(hipMalloc(&pg->entf, sizeof(int32_t) * (size_t)std::max<int64_t>(pg->nbtot, 1)));
   if (pg->stats) {
     std::fprintf(stderr, "[obfs] boundary %lld (largest part %lld), Dp %.1f MB in %.1f ms (%d-bit masks)%s; "
                  "step-1 tables %.1f MB%s\n",
@@ -2261,6 +2290,7 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
   a.doff = pg->doff;
   a.din = pg->din;
   a.dinoff = pg->dinoff;
+  a.entf = pg->entf;
   a.blevel = pg->blevel;
   a.bilevel = pg->bilevel;
   a.st = pg->st;
