@@ -2,7 +2,7 @@
 // kernel launch per level, with the k-centers update fused into the last
 // level kernel.
 //
-// Reference behaviour followed: DOBFS (gapbs/bfs.cc:144-185).  Levels are
+// Reference behaviour followed: DOBFS (gapbs/bfs.cc:144-184).  Levels are
 // exact hop counts and do not depend on scheduling; the top-down /
 // bottom-up switch uses the reference's alpha/beta rule (bfs.cc:161,
 // 172-173), which only affects speed.  The epilogue restates the k-centers

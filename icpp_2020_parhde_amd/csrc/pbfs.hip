@@ -1,6 +1,6 @@
 // pbfs.hip — partitioned breadth-first search for graphs with a deep BFS.
 //
-// Reference behaviour followed: DOBFS (gapbs/bfs.cc:144-185) — exact hop
+// Reference behaviour followed: DOBFS (gapbs/bfs.cc:144-184) — exact hop
 // levels from one source — and the k-centers step of ParHDE.cpp:124-140
 // (min over columns, first maximiser, lowest index on ties) fused at the end.
 //

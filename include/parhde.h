@@ -162,7 +162,7 @@ int phde_num_parts(const phde_ctx *ctx, int64_t *parts);
  * distance of 255 or more). */
 int phde_num_boundary(const phde_ctx *ctx, int64_t *boundary);
 
-/* Breadth-first levels from one source: DOBFS (gapbs/bfs.cc:144-185).
+/* Breadth-first levels from one source: DOBFS (gapbs/bfs.cc:144-184).
  * dist[v] = hop count, -1 when unreached.  alpha/beta <= 0 use 15/18. */
 int phde_bfs(phde_ctx *ctx, int64_t source, int alpha, int beta, int32_t *dist);
 int phde_bfs_dev(phde_ctx *ctx, int64_t source, int alpha, int beta, int32_t *dist_dev);

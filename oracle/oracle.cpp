@@ -306,7 +306,7 @@ void or_seq_bfs(const or_graph *g, int64_t src, int32_t *dist) {
 }
 
 void or_dobfs(const or_graph *g, int64_t src, double *dists, int alpha, int beta) {
-  // DOBFS (gapbs/bfs.cc:144-185) with InitParent (:134-141), TDStep
+  // DOBFS (gapbs/bfs.cc:144-184) with InitParent (:134-141), TDStep
   // (:76-106), BUStep (:48-73), QueueToBitmap / BitmapToQueue (:109-132).
   const int64_t n = g->n;
   std::vector<int64_t> parent(n);

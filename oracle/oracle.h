@@ -47,7 +47,7 @@ enum { OR_OK = 0, OR_ERR_ARG = -1, OR_ERR_PIVOTS = -2, OR_ERR_EIG = -3, OR_ERR_W
 
 /* Textbook queue BFS (DrawingBase.cpp:246-271 shape).  dist[v] = level, -1 if unreached. */
 void or_seq_bfs(const or_graph *g, int64_t src, int32_t *dist);
-/* Direction-optimising BFS (gapbs/bfs.cc:134-185), levels written as
+/* Direction-optimising BFS (gapbs/bfs.cc:134-184), levels written as
  * doubles to reached vertices only. */
 void or_dobfs(const or_graph *g, int64_t src, double *dists, int alpha, int beta);
 /* Delta-stepping SSSP (gapbs/sssp.cc:53-140); unreached = 2^30-1. */
