@@ -1359,8 +1359,16 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
   // the item whose levels are in LDS (and its version), and the part whose
   // edge blob is
   uint32_t cached = kNoPart, cached_ver = 0, cached_part = kNoPart;
-  unsigned long long t_wait = 0, t_seed = 0, t_mp = 0, t_push = 0, n_act = 0, n_seed = 0, n_img = 0;
-  unsigned long long t_first = ~0ull, t_last = 0, t0 = 0, t1 = 0;
+  // thread 0's timings with stats, kept in LDS rather than in registers
+  __shared__ unsigned long long s_tm[11];
+  unsigned long long &t_wait = s_tm[0], &t_seed = s_tm[1], &t_mp = s_tm[2], &t_push = s_tm[3], &n_act = s_tm[4],
+                     &n_seed = s_tm[5], &n_img = s_tm[6], &t_first = s_tm[7], &t_last = s_tm[8], &t0 = s_tm[9],
+                     &t1 = s_tm[10];
+  if (tid == 0) {
+    for (int i = 0; i < 11; i++) s_tm[i] = 0;
+    t_first = ~0ull;
+  }
+  __syncthreads();
   for (;;) {
     if (stats && tid == 0) t0 = __builtin_amdgcn_s_memrealtime();
     if (tid == 0) {
