@@ -754,7 +754,7 @@ __global__ __launch_bounds__(kWave) void gram_cgs_kernel(const double *G, const 
 // ------------------------------------------------------------ Gram tail
 // S^T L S without S.  L = D - A and the graph is symmetric, so
 // C^T L C = 1/2 sum over directed edges (u, v) of delta delta^T with
-// delta = C_u - C_v.  BFS levels differ by at most one across an edge, so
+// delta = C_u - C_v, that is the sum over the edges with u < v.  BFS levels differ by at most one across an edge, so
 // every entry of delta is -1, 0 or 1 and follows from the levels mod 4:
 // delta_c = ((d_c(u) - d_c(v)) mod 4) read as 0, 1 or -1 (3); a residue
 // difference of 2 cannot come from BFS levels and flags the columns as
@@ -783,7 +783,8 @@ typedef int lg_v16i __attribute__((ext_vector_type(16)));
 
 // Per-wave partial sums of delta delta^T over a contiguous range of
 // directed edges, 64 at a time: lane l forms the residue differences of
-// edge l (gathering the two endpoints' packed residues) into LDS, then for
+// edge l (gathering the two endpoints' packed residues; zero unless its
+// source is below its target) into LDS, then for
 // each 16-edge group lane l (column c = l % 32, edges 8 (l / 32) .. + 7)
 // packs delta_c and delta_{c+32} of its 8 edges as int8 and the wave
 // issues v_mfma_i32_32x32x16_i8 for the tiles (0,0), (0,1), (1,1) of the
@@ -805,8 +806,11 @@ __global__ __launch_bounds__(kLgThreads) void lap_gram_kernel(const uint32_t *es
   for (int64_t eb = e0; eb < e1; eb += kWave) {
     const int64_t e = eb + lane;
     uint4 d = make_uint4(0u, 0u, 0u, 0u);
-    if (e < e1) {
-      const uint4 pu = pack[esrc[e]], pv = pack[adj[e]];
+    // each undirected edge once (u < v): the graph is symmetric, so the
+    // sum over the directed edges is twice this one
+    const uint32_t u = e < e1 ? esrc[e] : 0u, v = e < e1 ? adj[e] : 0u;
+    if (u < v) {
+      const uint4 pu = pack[u], pv = pack[v];
       d = make_uint4(field_sub(pu.x, pv.x), field_sub(pu.y, pv.y), field_sub(pu.z, pv.z), field_sub(pu.w, pv.w));
       badl |= field_two(d.x) | field_two(d.y) | field_two(d.z) | field_two(d.w);
     }
@@ -842,8 +846,9 @@ __global__ __launch_bounds__(kLgThreads) void lap_gram_kernel(const uint32_t *es
   if (badl) atomicOr(bad, 1);
 }
 
-// KC[i][j] (64 x 64, both triangles) = 1/2 the sum of the waves' tiles, one
-// wave per entry, in 64-bit integers (exact).
+// KC[i][j] (64 x 64, both triangles) = the sum of the waves' tiles (each
+// undirected edge taken once), one wave per entry, in 64-bit integers
+// (exact).
 __global__ void lap_sum_kernel(const int *part, int nparts, double *KC) {
   const int lane = threadIdx.x % kWave;
   const int e = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
@@ -856,7 +861,7 @@ __global__ void lap_sum_kernel(const int *part, int nparts, double *KC) {
   const int t = e / 1024, l = (e / 16) % kWave, v = e % 16;
   const int bi = t == 2 ? 1 : 0, bj = t == 0 ? 0 : 1;
   const int i = 32 * bi + 8 * (v / 4) + 4 * (l / 32) + v % 4, j = 32 * bj + l % 32;
-  const double val = (double)(s / 2);
+  const double val = (double)s;
   KC[i * 64 + j] = val;
   KC[j * 64 + i] = val;
 }
