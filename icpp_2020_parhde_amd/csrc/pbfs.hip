@@ -889,6 +889,7 @@ __device__ void part_table(const ObfsArgs &a, uint32_t p, unsigned char *smem, i
   const unsigned char *D = a.dmat + a.doff[p];
   const int ng = nbp / 16;
   constexpr int kItems = 8;
+  if (stats && tid == 0) atomicAdd(&a.stats[25], (unsigned long long)nc);
   for (int it0 = tid; it0 < nc * ng; it0 += kItems * Block) {
     uint4 w[kItems];
 #pragma unroll
@@ -2371,9 +2372,9 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
                  s[17], s[16] * 0.01 / (double)std::max(1ull, s[17]), s[18], s[19] * 0.01);
     std::fprintf(stderr,
                  "[obfs]   step 3 per part: image or entries %.2f seeds or table rows %.2f rounds %.2f write-out %.2f "
-                 "us; %.3f us per round; %.1f entries per part\n",
+                 "us; %.3f us per round; %.1f entries of %.1f candidates per part\n",
                  s[20] * us / pg->k, s[21] * us / pg->k, s[22] * us / pg->k, s[23] * us / pg->k,
-                 s[22] * us / (double)std::max(1ull, s[10]), s[24] / (double)pg->k);
+                 s[22] * us / (double)std::max(1ull, s[10]), s[24] / (double)pg->k, s[25] / (double)pg->k);
   }
   *ran = true;
   return PHDE_OK;
