@@ -32,6 +32,7 @@
 #include "device_util.h"
 #include "hops.h"
 #include "learned.h"
+#include "pbfs.h"
 #include "traverse.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -663,6 +664,10 @@ struct SelArgs {
   uint32_t qcap;  // queue entries usable (0: n + 1)
   unsigned long long *cnt64;
   unsigned int *overflow;
+  // one_round: the launch makes the picks of one round, writes their
+  // number to *nround and returns (the caller lowers sd from them)
+  int one_round;
+  int *nround;
 };
 
 __device__ __forceinline__ uint32_t orderable(float f) {
@@ -771,6 +776,10 @@ __global__ __launch_bounds__(256) void select_kernel(SelArgs a, unsigned int *ar
       nr++;
       if (gtid == 0) a.picks[i] = po;
       i++;
+    }
+    if (a.one_round) {
+      if (gtid == 0) st_dev(a.nround, nr);
+      return;
     }
     // ---- bounded BFS from the round's picks
     if (a.hops > 1) {
@@ -1079,6 +1088,65 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
   a.qcap = c->bfs_qcap;
   a.cnt64 = reinterpret_cast<unsigned long long *>(c->d_slots) + 4;
   a.overflow = c->d_bar + 2;
+  a.one_round = 0;
+  a.nround = nullptr;
+  unsigned int *arrive = c->d_bar, *err = c->d_bar + 1;
+  void *args[] = {&a, &arrive, &err};
+  if (pbfs_seeds_ok(c)) {
+    // round by round: the kernel makes a round's picks, then one search
+    // from all of them together (the boundary-distance method, bounded at
+    // r) lowers sd to the exact distance to the nearest pick within r
+    // hops, and elsewhere leaves it above r or as it was -- as the
+    // kernel's own bounded search does up to values above r, which a key
+    // does not tell apart (it takes min(sd, r + 1)).
+    DBuf<int32_t> col;
+    DBuf<int> nr_dev;
+    PHDE_TRY_RC(col.alloc(n));
+    PHDE_TRY_RC(nr_dev.alloc(1));
+    a.one_round = 1;
+    a.nround = nr_dev.p;
+    std::vector<int64_t> h(K);
+    for (int i = first; i < K;) {
+      a.first = i;
+      PHDE_TRY_RC(fill_u32(c, c->d_bar, 3, 0u));
+      hipEvent_t ev;
+      c->timer.begin(c->stream, PHDE_KERNEL_OTHER, &ev);
+      const hipError_t e = launch_resident((const void *)select_kernel, c->bfs_grid, 256, args, 0, c->stream);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error(std::string("pivot selection needs a persistent kernel: ") + hipGetErrorString(e));
+        return kSelectUnavailable;
+      }
+      c->timer.end(c->stream, PHDE_KERNEL_OTHER, ev);
+      int nr = 0;
+      unsigned int herr = 0;
+      PHDE_HIP(hipMemcpyAsync(&nr, nr_dev.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      PHDE_HIP(hipMemcpyAsync(&herr, err, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+      PHDE_HIP(hipStreamSynchronize(c->stream));
+      if (herr) {
+        set_error("pivot selection grid barrier timed out");
+        return kSelectUnavailable;
+      }
+      if (nr < 1 || nr > K - i) {
+        set_error("pivot selection made no pick");
+        return PHDE_ERR_DEVICE;
+      }
+      PHDE_HIP(hipMemcpyAsync(h.data() + i, pk.p + i, sizeof(int64_t) * nr, hipMemcpyDeviceToHost, c->stream));
+      PHDE_HIP(hipStreamSynchronize(c->stream));
+      std::vector<int64_t> seeds(nr);
+      for (int j = 0; j < nr; j++) seeds[j] = to_internal(c, h[i + j]);
+      bool ran = false;
+      PHDE_TRY_RC(pbfs_seeds_run(c, seeds.data(), nr, col.p, c->lp_sd, a.r, &ran));
+      if (!ran) {
+        // the method stopped: a search from each pick (with the fused
+        // min into sd) gives the same sd
+        for (int j = 0; j < nr; j++) PHDE_TRY_RC(bfs_run(c, seeds[j], 15, 18, col.p, c->lp_sd, true));
+      }
+      i += nr;
+    }
+    for (int i = first; i < K; i++) picks[i] = h[i];
+    return PHDE_OK;
+  }
   // the multi-hop search can (rarely) outgrow its queue: keep the state it
   // starts from, to repeat the selection level by level then
   DBuf<int32_t> sd0;
@@ -1087,8 +1155,6 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
     PHDE_HIP(hipMemcpyAsync(sd0.p, c->lp_sd, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, c->stream));
   }
   PHDE_TRY_RC(fill_u32(c, c->d_bar, 3, 0u));
-  unsigned int *arrive = c->d_bar, *err = c->d_bar + 1;
-  void *args[] = {&a, &arrive, &err};
   hipEvent_t ev;
   c->timer.begin(c->stream, PHDE_KERNEL_OTHER, &ev);
   const hipError_t e = launch_resident((const void *)select_kernel, c->bfs_grid, 256, args, 0, c->stream);

@@ -24,5 +24,17 @@ int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, boo
 // kUnvisited when unreached).  *ran is false when the context does not use
 // that method (the caller takes another traversal).
 int pbfs_multi_run(phde_ctx *c, const int64_t *src, int k, int32_t *D, bool *ran);
+// One search from k seeds (internal ids) together, each at level 0: col
+// gets every vertex's distance to the nearest seed and min_dist the least
+// of that and its own value -- exact up to bound; a distance above bound
+// leaves a value above bound, or none (kUnvisited: no bound).  *ran is
+// false when the context does not run the boundary-distance method with
+// its distance tables (the caller takes another search); the searches of
+// the groups before a decline have then already lowered min_dist, which
+// another search from all k seeds leaves correct.
+int pbfs_seeds_run(phde_ctx *c, const int64_t *seeds, int k, int32_t *col, int32_t *min_dist, int32_t bound,
+                   bool *ran);
+// Whether pbfs_seeds_run can run on this context.
+bool pbfs_seeds_ok(const phde_ctx *c);
 
 }  // namespace phde

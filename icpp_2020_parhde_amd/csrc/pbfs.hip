@@ -41,6 +41,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "device_util.h"
@@ -665,6 +666,16 @@ struct ObfsArgs {
   int nsrc;
   int64_t nbtot;
   uint32_t src_part[kMaxSources], src_loc[kMaxSources];
+  // seeds at level 0: nseed == nsrc, seed s the source s; or, with one
+  // search (nsrc == 1), several seeds at level 0 (the distance to the
+  // nearest of them), at most two in a part.  The items queued first:
+  // qitem[0 .. nq)
+  int nseed, nq;
+  uint32_t qitem[kMaxSources];
+  // levels above bound are not pushed to other parts, and parts with no
+  // entry are not written out: levels up to bound are exact, the others
+  // above it or unwritten (kUnvisited: no bound)
+  int32_t bound;
   // multi-source interiors: each vertex's arrival round per source of the
   // current group of 32 (bytes, part-major rows of 32; 255: not reached)
   unsigned char *mscr;
@@ -750,11 +761,15 @@ __global__ void obfs_init_kernel(ObfsArgs a) {
     a.blevel[j] = kUnvisited;
     a.bilevel[j] = kUnvisited;
   }
-  // every source's own part queued: items s * k + src_part[s] in slots s
-  for (int64_t t = gtid; t < a.nsrc * a.k; t += gsize) a.st[t] = t % a.k == a.src_part[t / a.k] ? 1u : 0u;
+  // the seeds' parts queued (items qitem[r] in slots r)
+  for (int64_t t = gtid; t < a.nsrc * a.k; t += gsize) {
+    bool q = false;
+    for (int r = 0; r < a.nq; r++) q = q || a.qitem[r] == (uint32_t)t;
+    a.st[t] = q ? 1u : 0u;
+  }
   for (int64_t r = gtid; r < a.ring_n; r += gsize)
-    a.ring[r * kSlotStride] = r < a.nsrc ? ((unsigned long long)(r + 1) << 32) | (r * a.k + a.src_part[r] + 1u) : 0ull;
-  if (gtid < kCtlWords) a.ctl[gtid] = gtid == kCtlTP || gtid == kCtlTP + 1 ? (uint32_t)a.nsrc : 0u;  // tail, pending
+    a.ring[r * kSlotStride] = r < a.nq ? ((unsigned long long)(r + 1) << 32) | (a.qitem[r] + 1u) : 0ull;
+  if (gtid < kCtlWords) a.ctl[gtid] = gtid == kCtlTP || gtid == kCtlTP + 1 ? (uint32_t)a.nq : 0u;  // tail, pending
   if (gtid == 0) {
     a.kc->key = 0;
     a.kc->sumsq[0] = a.kc->sumsq[1] = 0;
@@ -940,6 +955,7 @@ __device__ void part_table(const ObfsArgs &a, uint32_t p, unsigned char *smem, i
   }
   __syncthreads();
   const int ne = s_cnt[0];
+  if (ne == 0 && a.bound != kUnvisited) return;  // nothing within the bound
   phase(20);
   if (stats && tid == 0) atomicAdd(&a.stats[24], (unsigned long long)ne);
   const unsigned char *Din = a.din + a.dinoff[p];
@@ -1032,8 +1048,10 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, int src, bool final_mode,
   int32_t *sl = reinterpret_cast<int32_t *>(reinterpret_cast<unsigned char *>(sv) + align_up(2 * (nb + 2), 4));
   for (int v = tid; v < nv; v += Block) lvl[v] = kUnvisited;
   __syncthreads();
-  // seeds: the boundary at its exact levels (final mode), the source
-  for (int jj = 0; jj <= nb; jj += Block) {
+  // seeds: the boundary at its exact levels (final mode), the source (one
+  // search with several seeds: every seed in p)
+  const int nsd = final_mode && a.nsrc == 1 ? a.nseed : 1;
+  for (int jj = 0; jj < nb + nsd; jj += Block) {
     const int j = jj + tid;
     int u = 0;
     int32_t L = kUnvisited;
@@ -1042,9 +1060,12 @@ __device__ int part_bfs(const ObfsArgs &a, uint32_t p, int src, bool final_mode,
         u = blist[j];
         L = ld_dev(&a.blevel[src * a.nbtot + b0 + j]);
       }
-    } else if (j == nb && p == a.src_part[src]) {
-      u = (int)a.src_loc[src];
-      L = 0;
+    } else if (j < nb + nsd) {
+      const int t = nsd > 1 ? j - nb : src;
+      if (p == a.src_part[t]) {
+        u = (int)a.src_loc[t];
+        L = 0;
+      }
     }
     const bool add = L != kUnvisited;
     if (add) {
@@ -1327,18 +1348,20 @@ __global__ __launch_bounds__(Block) void obfs_source_kernel(ObfsArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   __shared__ int s_cnt[3], s_min;
   __shared__ int32_t s_dummy[Block];
-  const int src = blockIdx.x;
-  const uint32_t p = a.src_part[src];
+  const int t = blockIdx.x;  // the seed
+  const int src = a.nsrc > 1 ? t : 0;
+  const uint32_t p = a.src_part[t];
   if (a.din) {
-    // the source's column of the part's distance table: its in-part
-    // distance to every boundary vertex (symmetric graph)
+    // the seed's column of the part's distance table: its in-part
+    // distance to every boundary vertex (symmetric graph); seeds sharing
+    // a part keep the least
     const uint32_t nvp = (uint32_t)align_up(a.vstart[p + 1] - a.vstart[p], 16);
     const uint32_t b0 = a.bstart[p];
     const int nb = (int)(a.bstart[p + 1] - b0);
-    const unsigned char *col = a.din + a.dinoff[p] + a.src_loc[src];
+    const unsigned char *col = a.din + a.dinoff[p] + a.src_loc[t];
     for (int j = threadIdx.x; j < nb; j += Block) {
       const unsigned char d = col[(size_t)j * nvp];
-      if (d != kDInf) st_dev(&a.bilevel[src * a.nbtot + b0 + j], (int32_t)d);
+      if (d != kDInf) atomicMin(&a.bilevel[src * a.nbtot + b0 + j], (int32_t)d);
     }
     return;
   }
@@ -1527,8 +1550,8 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
           nl[q] = 0;
           if (x < nx) {
             const int j = xs[x];
-            on[q] = chg[j] != 0;
             nl[q] = lv[j] + 1;
+            on[q] = chg[j] != 0 && nl[q] <= a.bound;
           }
         }
 #pragma unroll
@@ -1598,7 +1621,9 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
       __syncthreads();
       const uint32_t p = s_part;
       if ((int64_t)p >= a.k) break;
-      if (a.din && p != a.src_part[0])
+      bool seeded = false;
+      for (int t = 0; t < a.nseed; t++) seeded = seeded || p == a.src_part[t];
+      if (a.din && !seeded)
         part_table<Block>(a, p, smem, s_cnt, acc);
       else
         n_fr += part_bfs<Block, Lanes, Claims>(a, p, 0, true, smem, s_cnt, &s_min, s_dummy, acc);
@@ -2276,7 +2301,10 @@ static int obfs_reserve(phde_ctx *c, PartGraph *pg, int nsrc) {
 // sources (internal ids) together; levels to out (n per source, internal
 // ids).  min_dist (one source only): the fused k-centers step.
 static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, int32_t *out, int32_t *min_dist,
-                    bool *ran) {
+                    bool *ran, bool seeds = false, int32_t bound = kUnvisited) {
+  // seeds: one search from the nsrc sources together at level 0
+  const int nseed = nsrc;
+  if (seeds) nsrc = 1;
   PHDE_TRY_RC(obfs_reserve(c, pg, nsrc));
   const int64_t src = srcs[0];
   ObfsArgs a;
@@ -2309,9 +2337,16 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
   a.nsrc = nsrc;
   a.nbtot = pg->nbtot;
   a.mscr = pg->mscr;
-  for (int i = 0; i < nsrc; i++) {
+  a.nseed = nseed;
+  a.bound = bound;
+  a.nq = 0;
+  for (int i = 0; i < nseed; i++) {
     a.src_part[i] = pg->h_part_of[srcs[i]];
     a.src_loc[i] = pg->h_loc[srcs[i]];
+    const uint32_t item = seeds ? a.src_part[i] : (uint32_t)(i * pg->k) + a.src_part[i];
+    bool dup = false;
+    for (int r = 0; r < a.nq; r++) dup = dup || a.qitem[r] == item;
+    if (!dup) a.qitem[a.nq++] = item;
   }
   a.stats = pg->stats;
   if (pg->stats) {
@@ -2332,7 +2367,7 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
       1, std::min<int64_t>((std::max<int64_t>(nsrc * pg->nbtot, pg->ring_n) + 255) / 256, (int64_t)c->num_cus * 8));
   obfs_init_kernel<<<ib, 256, 0, c->stream>>>(a);
   PHDE_HIP(hipGetLastError());
-  pg->oshape.src_fn<<<nsrc, block, pg->olds, c->stream>>>(a);
+  pg->oshape.src_fn<<<nseed, block, pg->olds, c->stream>>>(a);
   PHDE_HIP(hipGetLastError());
   void *args[] = {&a};
   const hipError_t e = launch_resident((const void *)fn, grid, block, args, pg->olds, c->stream);
@@ -2385,6 +2420,41 @@ int pbfs_multi_run(phde_ctx *c, const int64_t *src, int k, int32_t *D, bool *ran
   PartGraph *pg = c->pg;
   if (!pg || pg->disabled || !c->bfs_persistent || !pg->ovl || k < 1 || k > kMaxSources) return PHDE_OK;
   return obfs_run(c, pg, src, k, D, nullptr, ran);
+}
+
+bool pbfs_seeds_ok(const phde_ctx *c) {
+  const PartGraph *pg = c->pg;
+  return pg && !pg->disabled && c->bfs_persistent && pg->ovl && pg->din;
+}
+
+int pbfs_seeds_run(phde_ctx *c, const int64_t *seeds, int k, int32_t *col, int32_t *min_dist, int32_t bound,
+                   bool *ran) {
+  *ran = false;
+  PartGraph *pg = c->pg;
+  if (!pbfs_seeds_ok(c) || k < 1) return PHDE_OK;
+  // groups of at most kMaxSources seeds and two in a part (the interior
+  // pass of a part lists its boundary and two more vertices); the distance
+  // to the nearest seed is the least over the groups
+  std::vector<int64_t> rest(seeds, seeds + k);
+  while (!rest.empty()) {
+    std::vector<int64_t> grp, later;
+    std::map<uint32_t, int> in_part;
+    for (const int64_t v : rest) {
+      int &cnt = in_part[pg->h_part_of[v]];
+      if (cnt < 2 && (int)grp.size() < kMaxSources) {
+        cnt++;
+        grp.push_back(v);
+      } else {
+        later.push_back(v);
+      }
+    }
+    bool r = false;
+    PHDE_TRY_RC(obfs_run(c, pg, grp.data(), (int)grp.size(), col, min_dist, &r, true, bound));
+    if (!r) return PHDE_OK;
+    rest.swap(later);
+  }
+  *ran = true;
+  return PHDE_OK;
 }
 
 int pbfs_run(phde_ctx *c, int64_t src, int32_t *dist_out, int32_t *min_dist, bool *ran) {

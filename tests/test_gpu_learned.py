@@ -117,6 +117,27 @@ def test_greedy_selection_bit_exact(gpu, name, kind):
         assert got.tolist() == want.tolist(), (k, r)
 
 
+@pytest.mark.parametrize("budget", [None, 4096, 600])
+@pytest.mark.parametrize("name", ["mesh", "rgg", "holed"])
+def test_greedy_selection_partitioned(gpu, name, budget):
+    """Cut into parts with distance tables (small part budgets make many),
+    each round's picks lower the distance to the picks through one
+    boundary-distance search from all of them together: the same picks."""
+    from test_gpu_pbfs import context
+    g = GRAPHS[name]()
+    rng = np.random.default_rng(9)
+    score = rng.random(g.n).astype(np.float32)
+    d1 = O.seq_bfs(g, 0)
+    second = int(np.argmax(d1))
+    d2 = O.seq_bfs(g, second)
+    for k, r in [(64, 3), (20, 1), (40, 9)]:
+        with context(g, budget) as ctx:
+            assert ctx.num_boundary() >= 0
+            got = ctx.select_pivots(score, k, r, 0)
+        want = OL.greedy_select(g, score, d1, d2, 0, second, k, r)
+        assert got.tolist() == want.tolist(), (k, r)
+
+
 def _libc_sources(seed, count, n):
     libc = ctypes.CDLL("libc.so.6")
     libc.srand(seed)
