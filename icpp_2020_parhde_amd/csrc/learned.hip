@@ -1120,8 +1120,10 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
       c->timer.end(c->stream, PHDE_KERNEL_OTHER, ev);
       int nr = 0;
       unsigned int herr = 0;
+      // the count, the flag and every pick slot from here on, in one wait
       PHDE_HIP(hipMemcpyAsync(&nr, nr_dev.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       PHDE_HIP(hipMemcpyAsync(&herr, err, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+      PHDE_HIP(hipMemcpyAsync(h.data() + i, pk.p + i, sizeof(int64_t) * (K - i), hipMemcpyDeviceToHost, c->stream));
       PHDE_HIP(hipStreamSynchronize(c->stream));
       if (herr) {
         set_error("pivot selection grid barrier timed out");
@@ -1131,8 +1133,6 @@ int greedy_select(phde_ctx *c, const float *score_dev, int first, int K, int r, 
         set_error("pivot selection made no pick");
         return PHDE_ERR_DEVICE;
       }
-      PHDE_HIP(hipMemcpyAsync(h.data() + i, pk.p + i, sizeof(int64_t) * nr, hipMemcpyDeviceToHost, c->stream));
-      PHDE_HIP(hipStreamSynchronize(c->stream));
       std::vector<int64_t> seeds(nr);
       for (int j = 0; j < nr; j++) seeds[j] = to_internal(c, h[i + j]);
       bool ran = false;

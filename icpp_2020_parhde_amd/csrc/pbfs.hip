@@ -881,7 +881,9 @@ __device__ void part_table(const ObfsArgs &a, uint32_t p, unsigned char *smem, i
     bool cand = false;
     if (j < nb) {
       Lj = ld_dev(&a.blevel[b0 + j]);
-      cand = Lj != kUnvisited && ld_dev(&a.entf[b0 + j]) != 0;
+      // (a candidate above the bound gives no level within it, nor does
+      // any candidate it dominates)
+      cand = Lj != kUnvisited && Lj <= a.bound && ld_dev(&a.entf[b0 + j]) != 0;
       L[j] = Lj;
       Lc[j] = cand ? Lj : kUnvisited;
       el[j] = cand ? Lj : kUnvisited;
