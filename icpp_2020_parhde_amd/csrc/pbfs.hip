@@ -1618,8 +1618,14 @@ __global__ __launch_bounds__(Block) void obfs_kernel(ObfsArgs a) {
   unsigned long long t_fin = 0, n_fr = 0;
   if (stats && tid == 0) t0 = __builtin_amdgcn_s_memrealtime();
   {
+    // each part's successor is claimed while the part runs
+    uint32_t next = 0;
+    if (tid == 0) next = atomicAdd(&a.ctl[kCtlFin], 1u);
     for (;;) {
-      if (tid == 0) s_part = atomicAdd(&a.ctl[kCtlFin], 1u);
+      if (tid == 0) {
+        s_part = next;
+        if ((int64_t)next < a.k) next = atomicAdd(&a.ctl[kCtlFin], 1u);
+      }
       __syncthreads();
       const uint32_t p = s_part;
       if ((int64_t)p >= a.k) break;
@@ -1665,9 +1671,15 @@ __global__ __launch_bounds__(Block) void obfs_multi_final_kernel(ObfsArgs a) {
   const int tid = threadIdx.x;
   KcAcc acc;
   unsigned long long n_fr = 0;
-    // part by part: groups of 32 sources at once where the part fits
+    // part by part: groups of 32 sources at once where the part fits; each
+    // part's successor claimed while the part runs
+    uint32_t next = 0;
+    if (tid == 0) next = atomicAdd(&a.ctl[kCtlFin], 1u);
     for (;;) {
-      if (tid == 0) s_part = atomicAdd(&a.ctl[kCtlFin], 1u);
+      if (tid == 0) {
+        s_part = next;
+        if ((int64_t)next < a.k) next = atomicAdd(&a.ctl[kCtlFin], 1u);
+      }
       __syncthreads();
       const uint32_t p = s_part;
       if ((int64_t)p >= a.k) break;
