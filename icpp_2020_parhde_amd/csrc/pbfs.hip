@@ -1876,7 +1876,8 @@ struct PartGraph {
   unsigned long long *dinoff = nullptr;
   int32_t *entf = nullptr;  // step 3's entry candidates (with din)
   int32_t *blevel = nullptr, *bilevel = nullptr;
-  ObfsShape oshape{};
+  ObfsShape oshape{};  // one search (k-centers, the selection's rounds)
+  ObfsShape mshape{};  // several sources searched together
   size_t olds = 0;  // dynamic LDS of the boundary-distance kernels
   int ocap = 1;     // sources the item state, ring and boundary levels have room for
   unsigned char *mscr = nullptr;  // multi-source interiors: arrival rounds (n x 32 bytes)
@@ -2059,15 +2060,23 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   PHDE_HIP(hipMemcpyAsync(&h_err, err.p, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
   PHDE_HIP(hipStreamSynchronize(c->stream));
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  pg->oshape = kObfsShapes[1];
+  // one search: 512 threads -- one workgroup per CU all the same (LDS),
+  // two waves per SIMD with room for 256 VGPRs each and eight waves per
+  // barrier (RGG 4.2M: 0.860 ms per traversal; 1,024 threads 0.941, 256
+  // threads 1.056); several sources together: 1,024 threads (50 sources
+  // at RGG 4.2M 20.1 ms, at 512 threads 20.7; 16.8M 85.3 and 88.0)
+  const int nshapes = (int)(sizeof(kObfsShapes) / sizeof(kObfsShapes[0]));
+  pg->oshape = kObfsShapes[2];
+  pg->mshape = kObfsShapes[1];
   if (const char *e = getenv("PHDE_OBFS_SHAPE"))
-    pg->oshape = kObfsShapes[std::max(0, std::min((int)(sizeof(kObfsShapes) / sizeof(kObfsShapes[0])) - 1, atoi(e)))];
+    pg->oshape = pg->mshape = kObfsShapes[std::max(0, std::min(nshapes - 1, atoi(e)))];
   // LDS: step 3's largest part, a run's state with its edge blob and, up to
   // 152 KB (the kernel's static LDS takes 4 KB more), the multi-source
   // interior pass (parts beyond it take their sources one at a time)
   pg->olds = (size_t)align_up(
       std::max({oh.max_final, oh.max_orun, oh.max_table, std::min<int64_t>(oh.max_mf, 152 * 1024)}), 256);
-  for (const void *f : {(const void *)pg->oshape.fn, (const void *)pg->oshape.src_fn, (const void *)pg->oshape.mfin_fn})
+  for (const void *f : {(const void *)pg->oshape.fn, (const void *)pg->oshape.src_fn, (const void *)pg->oshape.mfin_fn,
+                        (const void *)pg->mshape.fn, (const void *)pg->mshape.src_fn, (const void *)pg->mshape.mfin_fn})
     PHDE_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pg->olds));
   if ((h_err & 2u) && pg->din) {
     // some vertex lies 255 or more in-part hops from a boundary vertex: no
@@ -2367,8 +2376,9 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
     const unsigned long long init[32] = {0, 0, 0, 0, 0, 0, 0, ~0ull, 0};
     PHDE_HIP(hipMemcpyAsync(pg->stats, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
   }
-  const auto fn = pg->oshape.fn;
-  const int block = pg->oshape.block;
+  const ObfsShape &sh = nsrc > 1 ? pg->mshape : pg->oshape;
+  const auto fn = sh.fn;
+  const int block = sh.block;
   int resident = 0;
   PHDE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, block, pg->olds));
   // at most 4 workgroups per CU (the queue's ring holds the items + 4 per CU)
@@ -2381,7 +2391,7 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
       1, std::min<int64_t>((std::max<int64_t>(nsrc * pg->nbtot, pg->ring_n) + 255) / 256, (int64_t)c->num_cus * 8));
   obfs_init_kernel<<<ib, 256, 0, c->stream>>>(a);
   PHDE_HIP(hipGetLastError());
-  pg->oshape.src_fn<<<nseed, block, pg->olds, c->stream>>>(a);
+  sh.src_fn<<<nseed, block, pg->olds, c->stream>>>(a);
   PHDE_HIP(hipGetLastError());
   void *args[] = {&a};
   const hipError_t e = launch_resident((const void *)fn, grid, block, args, pg->olds, c->stream);
@@ -2391,7 +2401,7 @@ static int obfs_run(phde_ctx *c, PartGraph *pg, const int64_t *srcs, int nsrc, i
     return PHDE_OK;
   }
   if (nsrc > 1) {
-    pg->oshape.mfin_fn<<<grid, block, pg->olds, c->stream>>>(a);
+    sh.mfin_fn<<<grid, block, pg->olds, c->stream>>>(a);
     PHDE_HIP(hipGetLastError());
   }
   unsigned int flag = 0;
