@@ -2070,6 +2070,10 @@ static int obfs_setup(phde_ctx *c, PartGraph *pg, ObfsHost &oh, const std::vecto
   pg->mshape = kObfsShapes[1];
   if (const char *e = getenv("PHDE_OBFS_SHAPE"))
     pg->oshape = pg->mshape = kObfsShapes[std::max(0, std::min(nshapes - 1, atoi(e)))];
+  // PHDE_OBFS_MSHAPE: the several-source shape alone (random-pivot columns
+  // at RGG 4.2M: 24.1 ms at 1,024 threads × 8 lanes, 24.2 at 16 lanes,
+  // 24.7 and 25.1 at 512 threads)
+  if (const char *e = getenv("PHDE_OBFS_MSHAPE")) pg->mshape = kObfsShapes[std::max(0, std::min(nshapes - 1, atoi(e)))];
   // LDS: step 3's largest part, a run's state with its edge blob and, up to
   // 152 KB (the kernel's static LDS takes 4 KB more), the multi-source
   // interior pass (parts beyond it take their sources one at a time)
